@@ -1,0 +1,507 @@
+"""cabanamd_amd -- MI355X-native CabanaMD timestep engine (Python host side).
+
+The classes mirror the reference's module classes and method names
+(CabanaMD src/): System, Comm, Binning, NeighborVerlet, ForceLJ, Integrator,
+Temperature/KinE/PotE, InputFile and CbnMD.  Every call goes through the C
+ABI of libcabanamd_amd.so (include/cabanamd_amd.h) into hand-written HIP
+kernels; nothing here computes physics on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CmdError, Domain, Params, PREC_FP64, PREC_MIXED, call, d3,
+                   dp, ip, lib, lp)
+
+__all__ = ["System", "Comm", "Binning", "NeighborVerlet", "ForceLJ",
+           "Integrator", "Temperature", "KinE", "PotE", "InputFile", "CbnMD",
+           "CmdError", "Domain", "Params", "PREC_FP64", "PREC_MIXED",
+           "create_domain", "device_count", "lattice_params"]
+
+
+def device_count():
+    c = C.c_int(0)
+    call("cmd_device_count", C.byref(c))
+    return c.value
+
+
+def create_domain(low, high, ghost_cutoff):
+    """System::create_domain (system.h:149-205) as a plain struct."""
+    d = Domain()
+    call("cmd_domain_create", d3(low), d3(high), C.c_double(ghost_cutoff),
+         C.byref(d))
+    return d
+
+
+class System:
+    """System<Device, layout> (system.h:65-279): device-resident atoms."""
+
+    def __init__(self, ntypes=1):
+        h = C.c_void_p()
+        call("cmd_system_create", ntypes, C.byref(h))
+        self._h = h
+        self.ntypes = ntypes
+        self.mass = np.ones(ntypes)
+        self.dt, self.mvv2e, self.boltz = 0.005, 1.0, 1.0
+        self.domain = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cmd_system_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_units(self, dt, mvv2e=1.0, boltz=1.0):
+        self.dt, self.mvv2e, self.boltz = dt, mvv2e, boltz
+        call("cmd_system_set_units", self._h, dt, mvv2e, boltz)
+
+    def set_mass(self, mass):
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        self.mass = m
+        call("cmd_system_set_mass", self._h, len(m), dp(m))
+
+    def create_domain(self, low, high, ghost_cutoff=None):
+        if ghost_cutoff is None:
+            ghost_cutoff = max(high[0] - low[0], high[2] - low[1],
+                               high[2] - low[2])
+        self.domain = create_domain(low, high, ghost_cutoff)
+        call("cmd_system_set_domain", self._h, C.byref(self.domain))
+        return self.domain
+
+    def set_domain(self, d):
+        self.domain = d
+        call("cmd_system_set_domain", self._h, C.byref(d))
+
+    def resize(self, n):
+        call("cmd_system_reserve", self._h, int(n))
+
+    def upload(self, x, v=None, f=None, type=None, id=None, n_ghost=0):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        t = (np.zeros(n, dtype=np.int32) if type is None
+             else np.ascontiguousarray(type, dtype=np.int32))
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        f = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        i = None if id is None else np.ascontiguousarray(id, dtype=np.int32)
+        call("cmd_system_upload", self._h, n - n_ghost, n_ghost, dp(x), dp(v),
+             dp(f), ip(t), ip(i))
+
+    def counts(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_system_counts", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    @property
+    def N(self):
+        return self.counts()[0]
+
+    @property
+    def N_local(self):
+        return self.counts()[1]
+
+    @property
+    def N_ghost(self):
+        return self.counts()[2]
+
+    def download(self):
+        """(x, v, f, type, id) of local + ghost atoms as host arrays."""
+        _, nl, ng = self.counts()
+        n = nl + ng
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        t = np.zeros(n, dtype=np.int32)
+        i = np.zeros(n, dtype=np.int32)
+        if n:
+            call("cmd_system_download", self._h, dp(x), dp(v), dp(f), ip(t),
+                 ip(i))
+        return x, v, f, t, i
+
+    # slice_* of the reference return views; here they return host copies
+    def slice_x(self):
+        return self.download()[0]
+
+    def slice_v(self):
+        return self.download()[1]
+
+    def slice_f(self):
+        return self.download()[2]
+
+    def slice_type(self):
+        return self.download()[3]
+
+    def slice_id(self):
+        return self.download()[4]
+
+    def footprint(self):
+        b = C.c_int64()
+        call("cmd_system_footprint", self._h, C.byref(b))
+        return b.value
+
+
+class Integrator:
+    """Integrator<System> (integrator_nve.h:56-110): velocity Verlet NVE."""
+
+    def __init__(self, system):
+        self.system = system
+
+    def initial_integrate(self, system=None):
+        call("cmd_nve_initial", (system or self.system).handle)
+
+    def final_integrate(self, system=None):
+        call("cmd_nve_final", (system or self.system).handle)
+
+    def name(self):
+        return "Integrator:NVE"
+
+
+class Comm:
+    """Comm<System> on one rank (comm_mpi_impl.h:191-441)."""
+
+    def __init__(self, system, comm_depth):
+        self.system = system
+        self.comm_depth = comm_depth
+
+    def exchange(self):
+        call("cmd_comm_exchange", self.system.handle)
+
+    def exchange_halo(self):
+        call("cmd_comm_exchange_halo", self.system.handle, self.comm_depth)
+
+    def update_halo(self):
+        call("cmd_comm_update_halo", self.system.handle)
+
+    def update_force(self):
+        call("cmd_comm_update_force", self.system.handle)
+
+    def ghost_lists(self):
+        counts = np.zeros(6, dtype=np.int32)
+        call("cmd_comm_ghost_lists", self.system.handle, None, ip(counts))
+        send = np.zeros(max(int(counts.sum()), 1), dtype=np.int32)
+        call("cmd_comm_ghost_lists", self.system.handle, ip(send), ip(counts))
+        return send[:counts.sum()], counts
+
+    def name(self):
+        return "Comm:HIP"
+
+
+class Binning:
+    """Binning<System> (binning_cabana_impl.h:57-113)."""
+
+    def __init__(self, system):
+        self.system = system
+
+    def create_binning(self, dx, dy, dz, halo_depth=1, do_local=True,
+                       do_ghost=False, sort=True):
+        n = self.system.N_local
+        perm = np.zeros(max(n, 1), dtype=np.int32)
+        call("cmd_binning_create", self.system.handle, dx, dy, dz, halo_depth,
+             int(do_local), int(do_ghost), int(sort), ip(perm))
+        return perm[:n]
+
+    def name(self):
+        return "Binning:HIPLinkedCell"
+
+
+class NeighborVerlet:
+    """NeighborVerlet<System, Full|Half, CSR> (neighbor_verlet.h:19-70)."""
+
+    def __init__(self, neigh_cut, half_neigh, max_neigh_guess=50):
+        h = C.c_void_p()
+        call("cmd_neigh_create", neigh_cut, int(half_neigh),
+             int(max_neigh_guess), C.byref(h))
+        self._h = h
+        self.neigh_cut = neigh_cut
+        self.half_neigh = bool(half_neigh)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self, "_own", True):
+            lib().cmd_neigh_destroy(self._h)
+            self._h = None
+
+    @classmethod
+    def _borrow(cls, handle, neigh_cut, half):
+        o = cls.__new__(cls)
+        o._h = handle
+        o._own = False
+        o.neigh_cut = neigh_cut
+        o.half_neigh = bool(half)
+        return o
+
+    @property
+    def handle(self):
+        return self._h
+
+    def create(self, system):
+        call("cmd_neigh_build", self._h, system.handle)
+
+    def info(self):
+        r, nnz, mr, g = C.c_int(), C.c_int64(), C.c_int(), C.c_int()
+        call("cmd_neigh_info", self._h, C.byref(r), C.byref(nnz), C.byref(mr),
+             C.byref(g))
+        return dict(n_rows=r.value, nnz=nnz.value, max_row=mr.value,
+                    max_neigh_guess=g.value)
+
+    @property
+    def max_neigh_guess(self):
+        return self.info()["max_neigh_guess"]
+
+    def get(self):
+        """CSR list (offsets[n_rows+1] int64, idx[nnz] int32) on the host."""
+        inf = self.info()
+        off = np.zeros(inf["n_rows"] + 1, dtype=np.int64)
+        idx = np.zeros(max(inf["nnz"], 1), dtype=np.int32)
+        call("cmd_neigh_download", self._h, lp(off), ip(idx))
+        return off, idx[:inf["nnz"]]
+
+    def name(self):
+        return ("Neighbor:HIPVerletHalf" if self.half_neigh
+                else "Neighbor:HIPVerletFull")
+
+
+class ForceLJ:
+    """ForceLJ (force_types/force_lj_cabana_neigh_impl.h:49-377)."""
+
+    def __init__(self, system, precision=PREC_FP64, lanes=0):
+        h = C.c_void_p()
+        call("cmd_force_lj_create", system.ntypes, C.byref(h))
+        self._h = h
+        self._own = True
+        call("cmd_force_set_precision", h, precision)
+        call("cmd_force_set_lanes", h, lanes)
+
+    @classmethod
+    def _borrow(cls, handle):
+        o = cls.__new__(cls)
+        o._h = handle
+        o._own = False
+        return o
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self, "_own", True):
+            lib().cmd_force_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def init_coeff(self, args):
+        """args: pair_coeff lines as word lists, e.g.
+        [["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]]."""
+        for a in args:
+            call("cmd_force_lj_init_coeff", self._h, int(a[1]), int(a[2]),
+                 float(a[3]), float(a[4]), float(a[5]))
+
+    def set_precision(self, prec):
+        call("cmd_force_set_precision", self._h, prec)
+
+    def set_lanes(self, lanes):
+        call("cmd_force_set_lanes", self._h, lanes)
+
+    @property
+    def lanes(self):
+        g = C.c_int()
+        call("cmd_force_get_lanes", self._h, C.byref(g))
+        return g.value
+
+    def compute(self, system, neighbor):
+        call("cmd_force_compute", self._h, system.handle, neighbor.handle)
+
+    def compute_energy(self, system, neighbor):
+        e = C.c_double()
+        call("cmd_force_energy", self._h, system.handle, neighbor.handle,
+             C.byref(e))
+        return e.value
+
+    def compute_virial(self, system, neighbor):
+        w = C.c_double()
+        call("cmd_force_virial", self._h, system.handle, neighbor.handle,
+             C.byref(w))
+        return w.value
+
+    def name(self):
+        return "Force:LJHIP"
+
+
+class Temperature:
+    """Temperature (property_temperature_impl.h:55-77)."""
+
+    def __init__(self, comm=None):
+        self.comm = comm
+
+    def compute(self, system):
+        t = C.c_double()
+        call("cmd_thermo_temperature", system.handle, C.byref(t))
+        return t.value
+
+
+class KinE:
+    """KinE (property_kine_impl.h:55-76)."""
+
+    def __init__(self, comm=None):
+        self.comm = comm
+
+    def compute(self, system):
+        k = C.c_double()
+        call("cmd_thermo_kinetic", system.handle, C.byref(k))
+        return k.value
+
+
+class PotE:
+    """PotE (property_pote_impl.h:55-63)."""
+
+    def __init__(self, comm=None):
+        self.comm = comm
+
+    def compute(self, system, force, neighbor):
+        return force.compute_energy(system, neighbor)
+
+
+class InputFile:
+    """InputFile<System>::read_file (inputFile_impl.h:86-534)."""
+
+    def __init__(self, force_iteration=None):
+        self.force_iteration = force_iteration
+        self.params = Params()
+        call("cmd_params_default", C.byref(self.params))
+
+    def read_file(self, path):
+        fi = -1 if self.force_iteration is None else int(self.force_iteration)
+        call("cmd_params_parse_deck", str(path).encode(), fi,
+             C.byref(self.params))
+        return self.params
+
+    def create_lattice(self, domain):
+        """Host-side lattice + velocities (inputFile_impl.h:536-865)."""
+        p = self.params
+        n = C.c_int()
+        call("cmd_lattice_create", C.byref(p), C.byref(domain), C.byref(n),
+             None, None, None, None)
+        x = np.zeros((n.value, 3))
+        v = np.zeros((n.value, 3))
+        t = np.zeros(n.value, dtype=np.int32)
+        i = np.zeros(n.value, dtype=np.int32)
+        call("cmd_lattice_create", C.byref(p), C.byref(domain), C.byref(n),
+             dp(x), dp(v), ip(t), ip(i))
+        return x, v, t, i
+
+
+def lattice_params(ncells=(20, 20, 20), rho=0.8442, temp=1.44, seed=87287,
+                   mass=1.0, cutoff=2.5, skin=0.3, every=20, half=False,
+                   nsteps=100, thermo=10, dt=0.005, gaussian=False,
+                   ghost_cutoff=None):
+    """Run parameters of the fcc LJ workload (an in.lj-style deck)."""
+    p = Params()
+    call("cmd_params_default", C.byref(p))
+    p.units = 1
+    p.lattice_style = 1
+    p.lattice_constant = (4.0 / rho) ** (1.0 / 3.0)
+    for k in range(3):
+        p.region_lo[k] = 0.0
+        p.region_hi[k] = float(ncells[k])
+    p.ntypes = 1
+    p.mass[0] = mass
+    p.temp_target = temp
+    p.temp_seed = seed
+    p.vel_gaussian = 1 if gaussian else 0
+    p.force_cutoff = cutoff
+    p.skin = skin
+    p.every = every
+    if ghost_cutoff is not None:
+        p.comm_ghost_cutoff = ghost_cutoff
+    p.half = 1 if half else 0
+    p.nsteps = nsteps
+    p.thermo_rate = thermo
+    p.dt = dt
+    p.mvv2e = 1.0
+    p.boltz = 1.0
+    p.ncoeff = 1
+    p.coeff_i[0] = 1
+    p.coeff_j[0] = 1
+    p.coeff_eps[0] = 1.0
+    p.coeff_sigma[0] = 1.0
+    p.coeff_cut[0] = cutoff
+    p.force_type = 0
+    return p
+
+
+class CbnMD:
+    """CbnMD<System, Neighbor> (cabanamd_impl.h:70-432): init() then run()."""
+
+    def __init__(self, params, precision=PREC_FP64, lanes=0):
+        self.params = params
+        h = C.c_void_p()
+        call("cmd_md_create", C.byref(params), C.byref(h))
+        self._h = h
+        call("cmd_md_set_precision", h, precision)
+        call("cmd_md_set_lanes", h, lanes)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cmd_md_destroy(self._h)
+            self._h = None
+
+    def init(self):
+        call("cmd_md_init", self._h)
+        s = C.c_void_p()
+        call("cmd_md_system", self._h, C.byref(s))
+        self._sys = s
+        n = C.c_void_p()
+        call("cmd_md_neigh", self._h, C.byref(n))
+        self.neighbor = NeighborVerlet._borrow(
+            n, self.params.force_cutoff + self.params.skin, self.params.half)
+        fh = C.c_void_p()
+        call("cmd_md_force", self._h, C.byref(fh))
+        self.force = ForceLJ._borrow(fh)
+
+    def run(self, nsteps=None):
+        call("cmd_md_run", self._h,
+             int(self.params.nsteps if nsteps is None else nsteps))
+
+    def arrays(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_system_counts", self._sys, C.byref(a), C.byref(b),
+             C.byref(c))
+        n = b.value + c.value
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        t = np.zeros(n, dtype=np.int32)
+        i = np.zeros(n, dtype=np.int32)
+        call("cmd_system_download", self._sys, dp(x), dp(v), dp(f), ip(t),
+             ip(i))
+        return x, v, f, t, i
+
+    @property
+    def n_local(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_system_counts", self._sys, C.byref(a), C.byref(b),
+             C.byref(c))
+        return b.value
+
+    def thermo(self):
+        """Rows (step, T, PE/N, (PE+KE)/N, virial)."""
+        n = C.c_int()
+        call("cmd_md_thermo", self._h, 0, None, C.byref(n))
+        rows = np.zeros((max(n.value, 1), 5))
+        call("cmd_md_thermo", self._h, n.value, dp(rows), C.byref(n))
+        return rows[:n.value]
+
+    def timers(self):
+        t = np.zeros(6)
+        call("cmd_md_timers", self._h, dp(t))
+        return dict(zip(["force", "neigh", "comm", "int", "other", "total"],
+                        t))
+
+    def profile(self, enable=True):
+        call("cmd_md_profile", self._h, int(enable))
+
+    def kernel_stats(self):
+        n, ms = C.c_int(), C.c_double()
+        call("cmd_md_kernel_stats", self._h, C.byref(n), C.byref(ms))
+        return n.value, ms.value

@@ -1,0 +1,174 @@
+"""ctypes binding of libcabanamd_amd.so (the C ABI in include/cabanamd_amd.h).
+
+The library must be built (python -m cabanamd_amd.build); loading fails
+loudly otherwise -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcabanamd_amd.so")
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+LP = C.POINTER(C.c_int64)
+VP = C.c_void_p
+
+CMD_OK = 0
+CMD_ERR_NO_DEVICE = 7
+PREC_FP64 = 0
+PREC_MIXED = 1
+MAX_TYPES = 8
+
+
+class CmdError(RuntimeError):
+    """Raised for any non-zero cmd_status (the reference's log_err throws
+    std::runtime_error, output.h:37-54)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"cmd_status {status}: {msg}")
+        self.status = status
+
+
+class Domain(C.Structure):
+    _fields_ = [("low", C.c_double * 3), ("high", C.c_double * 3),
+                ("L", C.c_double * 3), ("cell", C.c_double * 3),
+                ("local_lo", C.c_double * 3), ("local_hi", C.c_double * 3),
+                ("local_mesh", C.c_double * 3),
+                ("ghost_lo", C.c_double * 3), ("ghost_hi", C.c_double * 3),
+                ("halo_width", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [("units", C.c_int), ("lattice_style", C.c_int),
+                ("lattice_constant", C.c_double),
+                ("lattice_offset", C.c_double * 3),
+                ("region_lo", C.c_double * 3), ("region_hi", C.c_double * 3),
+                ("ntypes", C.c_int), ("mass", C.c_double * MAX_TYPES),
+                ("temp_target", C.c_double), ("temp_seed", C.c_int),
+                ("vel_gaussian", C.c_int),
+                ("force_cutoff", C.c_double), ("skin", C.c_double),
+                ("every", C.c_int), ("max_neigh_guess", C.c_int),
+                ("comm_ghost_cutoff", C.c_double), ("half", C.c_int),
+                ("nsteps", C.c_int), ("thermo_rate", C.c_int),
+                ("dt", C.c_double), ("mvv2e", C.c_double),
+                ("boltz", C.c_double), ("ncoeff", C.c_int),
+                ("coeff_i", C.c_int * 16), ("coeff_j", C.c_int * 16),
+                ("coeff_eps", C.c_double * 16),
+                ("coeff_sigma", C.c_double * 16),
+                ("coeff_cut", C.c_double * 16),
+                ("force_type", C.c_int), ("nnp_dir", C.c_char * 256)]
+
+
+# name -> argtypes (restype is always c_int = cmd_status unless listed)
+_PROTOS = {
+    "cmd_last_error": ([], C.c_char_p),
+    "cmd_version": ([], C.c_int),
+    "cmd_device_init": ([C.c_int], None),
+    "cmd_device_count": ([IP], None),
+    "cmd_device_synchronize": ([], None),
+    "cmd_domain_create": ([DP, DP, C.c_double, C.POINTER(Domain)], None),
+    "cmd_system_create": ([C.c_int, C.POINTER(VP)], None),
+    "cmd_system_destroy": ([VP], None),
+    "cmd_system_set_stream": ([VP, VP], None),
+    "cmd_system_set_units": ([VP, C.c_double, C.c_double, C.c_double], None),
+    "cmd_system_set_mass": ([VP, C.c_int, DP], None),
+    "cmd_system_set_domain": ([VP, C.POINTER(Domain)], None),
+    "cmd_system_reserve": ([VP, C.c_int], None),
+    "cmd_system_upload": ([VP, C.c_int, C.c_int, DP, DP, DP, IP, IP], None),
+    "cmd_system_download": ([VP, DP, DP, DP, IP, IP], None),
+    "cmd_system_counts": ([VP, IP, IP, IP], None),
+    "cmd_system_footprint": ([VP, LP], None),
+    "cmd_nve_initial": ([VP], None),
+    "cmd_nve_final": ([VP], None),
+    "cmd_comm_exchange": ([VP], None),
+    "cmd_comm_exchange_halo": ([VP, C.c_double], None),
+    "cmd_comm_update_halo": ([VP], None),
+    "cmd_comm_update_force": ([VP], None),
+    "cmd_comm_ghost_lists": ([VP, IP, IP], None),
+    "cmd_binning_create": ([VP, C.c_double, C.c_double, C.c_double, C.c_int,
+                            C.c_int, C.c_int, C.c_int, IP], None),
+    "cmd_neigh_create": ([C.c_double, C.c_int, C.c_int, C.POINTER(VP)], None),
+    "cmd_neigh_destroy": ([VP], None),
+    "cmd_neigh_build": ([VP, VP], None),
+    "cmd_neigh_info": ([VP, IP, LP, IP, IP], None),
+    "cmd_neigh_download": ([VP, LP, IP], None),
+    "cmd_force_lj_create": ([C.c_int, C.POINTER(VP)], None),
+    "cmd_force_destroy": ([VP], None),
+    "cmd_force_lj_init_coeff": ([VP, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double], None),
+    "cmd_force_set_precision": ([VP, C.c_int], None),
+    "cmd_force_set_lanes": ([VP, C.c_int], None),
+    "cmd_force_get_lanes": ([VP, IP], None),
+    "cmd_force_compute": ([VP, VP, VP], None),
+    "cmd_force_energy": ([VP, VP, VP, DP], None),
+    "cmd_force_virial": ([VP, VP, VP, DP], None),
+    "cmd_thermo_temperature": ([VP, DP], None),
+    "cmd_thermo_kinetic": ([VP, DP], None),
+    "cmd_params_default": ([C.POINTER(Params)], None),
+    "cmd_params_parse_deck": ([C.c_char_p, C.c_int, C.POINTER(Params)], None),
+    "cmd_lattice_create": ([C.POINTER(Params), C.POINTER(Domain), IP, DP, DP,
+                            IP, IP], None),
+    "cmd_md_create": ([C.POINTER(Params), C.POINTER(VP)], None),
+    "cmd_md_destroy": ([VP], None),
+    "cmd_md_set_precision": ([VP, C.c_int], None),
+    "cmd_md_set_lanes": ([VP, C.c_int], None),
+    "cmd_md_init": ([VP], None),
+    "cmd_md_run": ([VP, C.c_int], None),
+    "cmd_md_system": ([VP, C.POINTER(VP)], None),
+    "cmd_md_neigh": ([VP, C.POINTER(VP)], None),
+    "cmd_md_force": ([VP, C.POINTER(VP)], None),
+    "cmd_md_thermo": ([VP, C.c_int, DP, IP], None),
+    "cmd_md_timers": ([VP, DP], None),
+    "cmd_md_profile": ([VP, C.c_int], None),
+    "cmd_md_kernel_stats": ([VP, IP, DP], None),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the engine library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with python -m cabanamd_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res if res is not None else C.c_int
+        _lib = L
+    return _lib
+
+
+def exported_names():
+    return list(_PROTOS)
+
+
+def check(status):
+    if status != CMD_OK:
+        msg = lib().cmd_last_error()
+        raise CmdError(status, msg.decode() if msg else "")
+    return status
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def dp(a):
+    return a.ctypes.data_as(DP) if a is not None else None
+
+
+def ip(a):
+    return a.ctypes.data_as(IP) if a is not None else None
+
+
+def lp(a):
+    return a.ctypes.data_as(LP) if a is not None else None
+
+
+def d3(v):
+    return (C.c_double * 3)(*[float(t) for t in v])

@@ -1,0 +1,1223 @@
+// core.hip -- device management, System state, scans, linked-cell sort,
+// velocity-Verlet, periodic wrap, 6-phase ghosts and thermo reductions.
+//
+// Reference lines restated here (CabanaMD source tree):
+//   integrator_nve.h:91-110, integrator_nve_impl.h:49-83   (NVE)
+//   comm_mpi.h:141-168, comm_mpi_impl.h:191-441            (wrap, ghosts)
+//   binning_cabana_impl.h:57-113                           (cell sort)
+//   property_temperature*.h, property_kine*.h              (thermo)
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+namespace cmd
+{
+static thread_local std::string g_err;
+
+void set_error( const char *fmt, ... )
+{
+    char buf[1024];
+    va_list ap;
+    va_start( ap, fmt );
+    vsnprintf( buf, sizeof( buf ), fmt, ap );
+    va_end( ap );
+    g_err = buf;
+}
+
+static constexpr int TPB = 256;
+static inline int nblocks_for( int64_t n, int tpb = TPB )
+{
+    return (int)std::max<int64_t>( 1, ( n + tpb - 1 ) / tpb );
+}
+
+// ---------------------------------------------------------------- fills
+template <class T>
+__global__ void k_fill( T *__restrict__ p, T v, size_t n )
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for ( ; i < n; i += stride )
+        p[i] = v;
+}
+
+cmd_status fill_i32( int *p, int v, size_t n, hipStream_t st )
+{
+    if ( n == 0 )
+        return CMD_OK;
+    int nb = (int)std::min<size_t>( ( n + TPB - 1 ) / TPB, 65536 );
+    k_fill<int><<<nb, TPB, 0, st>>>( p, v, n );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+cmd_status fill_f64( double *p, double v, size_t n, hipStream_t st )
+{
+    if ( n == 0 )
+        return CMD_OK;
+    int nb = (int)std::min<size_t>( ( n + TPB - 1 ) / TPB, 65536 );
+    k_fill<double><<<nb, TPB, 0, st>>>( p, v, n );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// ---------------------------------------------------------------- scans
+// Reduce-then-scan over tiles of SCAN_TILE elements: per-tile sums, one
+// block scans the tile sums, then each tile rescans with its offset.
+static constexpr int SCAN_ITEMS = 8;
+static constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ int64_t block_exclusive_scan( int64_t v, int64_t *total )
+{
+    __shared__ int64_t wsum[TPB / 64];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    int64_t s = v;
+#pragma unroll
+    for ( int o = 1; o < 64; o <<= 1 )
+    {
+        int64_t t = __shfl_up( s, o, 64 );
+        if ( lane >= o )
+            s += t;
+    }
+    if ( lane == 63 )
+        wsum[wid] = s;
+    __syncthreads();
+    int64_t woff = 0, tot = 0;
+#pragma unroll
+    for ( int w = 0; w < TPB / 64; ++w )
+    {
+        if ( w < wid )
+            woff += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return woff + s - v;
+}
+
+template <class InT>
+__global__ void k_scan_tile_sums( const InT *__restrict__ in, int n,
+                                  int64_t *__restrict__ blk )
+{
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+    for ( int k = 0; k < SCAN_ITEMS; ++k )
+    {
+        int64_t i = base + (int64_t)k * TPB + threadIdx.x;
+        if ( i < n )
+            s += in[i];
+    }
+    int64_t tot;
+    block_exclusive_scan( s, &tot );
+    if ( threadIdx.x == 0 )
+        blk[blockIdx.x] = tot;
+}
+
+template <class OutT>
+__global__ void k_scan_tile_offsets( int64_t *__restrict__ blk, int nblk,
+                                     OutT *__restrict__ out, int n,
+                                     int64_t *__restrict__ total_slot )
+{
+    int64_t carry = 0;
+    for ( int base = 0; base < nblk; base += TPB )
+    {
+        int i = base + threadIdx.x;
+        int64_t v = ( i < nblk ) ? blk[i] : 0;
+        int64_t tot;
+        int64_t e = block_exclusive_scan( v, &tot );
+        if ( i < nblk )
+            blk[i] = carry + e;
+        carry += tot;
+    }
+    if ( threadIdx.x == 0 )
+    {
+        out[n] = (OutT)carry;
+        if ( total_slot )
+            *total_slot = carry;
+    }
+}
+
+template <class InT, class OutT>
+__global__ void k_scan_tile_apply( const InT *__restrict__ in, int n,
+                                   const int64_t *__restrict__ blk,
+                                   OutT *__restrict__ out )
+{
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE +
+                         (int64_t)threadIdx.x * SCAN_ITEMS;
+    InT vals[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for ( int k = 0; k < SCAN_ITEMS; ++k )
+    {
+        int64_t i = base + k;
+        vals[k] = ( i < n ) ? in[i] : (InT)0;
+        s += vals[k];
+    }
+    int64_t tot;
+    int64_t e = block_exclusive_scan( s, &tot ) + blk[blockIdx.x];
+#pragma unroll
+    for ( int k = 0; k < SCAN_ITEMS; ++k )
+    {
+        int64_t i = base + k;
+        if ( i < n )
+            out[i] = (OutT)e;
+        e += vals[k];
+    }
+}
+
+template <class OutT>
+static cmd_status scan_impl( const int *in, OutT *out, int n, Scratch &sc,
+                             hipStream_t st, int64_t *total_host )
+{
+    const int nblk = std::max( 1, ( n + SCAN_TILE - 1 ) / SCAN_TILE );
+    CMD_TRY( sc.blk.reserve( nblk + 2 ) );
+    int64_t *slot = sc.blk.p + nblk + 1;
+    if ( n > 0 )
+    {
+        k_scan_tile_sums<int><<<nblk, TPB, 0, st>>>( in, n, sc.blk.p );
+        CMD_LAUNCH_CHECK();
+    }
+    else
+    {
+        CMD_TRY( fill_i32( (int *)sc.blk.p, 0, 2, st ) );
+    }
+    k_scan_tile_offsets<OutT><<<1, TPB, 0, st>>>( sc.blk.p, n > 0 ? nblk : 0,
+                                                  out, n, slot );
+    CMD_LAUNCH_CHECK();
+    if ( n > 0 )
+    {
+        k_scan_tile_apply<int, OutT><<<nblk, TPB, 0, st>>>( in, n, sc.blk.p,
+                                                            out );
+        CMD_LAUNCH_CHECK();
+    }
+    if ( total_host )
+    {
+        CMD_HIP( hipMemcpyAsync( total_host, slot, sizeof( int64_t ),
+                                 hipMemcpyDeviceToHost, st ) );
+        CMD_HIP( hipStreamSynchronize( st ) );
+    }
+    return CMD_OK;
+}
+
+cmd_status scan_exclusive( const int *in, int *out, int n, Scratch &sc,
+                           hipStream_t st, int64_t *total_host )
+{
+    return scan_impl<int>( in, out, n, sc, st, total_host );
+}
+
+cmd_status scan_exclusive( const int *in, int64_t *out, int n, Scratch &sc,
+                           hipStream_t st, int64_t *total_host )
+{
+    return scan_impl<int64_t>( in, out, n, sc, st, total_host );
+}
+
+// ---------------------------------------------------------------- cell sort
+__device__ __forceinline__ int cell_coord( double c, double mn, double rd,
+                                           int n, int floor_mode )
+{
+    double t = ( c - mn ) * rd;
+    int ic = floor_mode ? (int)floor( t ) : (int)t;
+    if ( ic == n )
+        ic = n - 1;
+    ic = ic < 0 ? 0 : ( ic > n - 1 ? n - 1 : ic );
+    return ic;
+}
+
+__device__ __forceinline__ int cell_of_pos( const CellGrid &g, double4 p )
+{
+    int a = cell_coord( p.x, g.min[0], g.rd[0], g.n[0], g.floor_mode );
+    int b = cell_coord( p.y, g.min[1], g.rd[1], g.n[1], g.floor_mode );
+    int c = cell_coord( p.z, g.min[2], g.rd[2], g.n[2], g.floor_mode );
+    return ( a * g.n[1] + b ) * g.n[2] + c;
+}
+
+__global__ void k_cell_count( const double4 *__restrict__ x, int n,
+                              CellGrid g, int *__restrict__ cell_of,
+                              int *__restrict__ count )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    int c = cell_of_pos( g, x[i] );
+    cell_of[i] = c;
+    atomicAdd( &count[c], 1 );
+}
+
+__global__ void k_cell_fill( int n, const int *__restrict__ cell_of,
+                             const int *__restrict__ start,
+                             int *__restrict__ cursor, int *__restrict__ perm )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    int c = cell_of[i];
+    int slot = start[c] + atomicAdd( &cursor[c], 1 );
+    perm[slot] = i;
+}
+
+// Restore index order inside each cell (insertion sort; cells hold tens of
+// atoms), which makes the sort stable and deterministic.
+__global__ void k_cell_order( int ncell, const int *__restrict__ start,
+                              int *__restrict__ perm )
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( c >= ncell )
+        return;
+    const int b = start[c], e = start[c + 1];
+    for ( int k = b + 1; k < e; ++k )
+    {
+        int v = perm[k];
+        int m = k - 1;
+        while ( m >= b && perm[m] > v )
+        {
+            perm[m + 1] = perm[m];
+            --m;
+        }
+        perm[m + 1] = v;
+    }
+}
+
+cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
+                      Scratch &sc, hipStream_t st )
+{
+    const int ncell = g.n[0] * g.n[1] * g.n[2];
+    CMD_REQUIRE( ncell > 0, CMD_ERR_ARG, "cell grid is empty" );
+    cs.ncell = ncell;
+    CMD_TRY( cs.count.reserve( ncell ) );
+    CMD_TRY( cs.cursor.reserve( ncell ) );
+    CMD_TRY( cs.start.reserve( ncell + 1 ) );
+    CMD_TRY( cs.cell_of.reserve( std::max( n, 1 ) ) );
+    CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
+    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
+    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    if ( n > 0 )
+    {
+        k_cell_count<<<nblocks_for( n ), TPB, 0, st>>>( x, n, g, cs.cell_of.p,
+                                                       cs.count.p );
+        CMD_LAUNCH_CHECK();
+    }
+    CMD_TRY( scan_exclusive( cs.count.p, cs.start.p, ncell, sc, st, nullptr ) );
+    if ( n > 0 )
+    {
+        k_cell_fill<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
+                                                      cs.start.p, cs.cursor.p,
+                                                      cs.perm.p );
+        CMD_LAUNCH_CHECK();
+        k_cell_order<<<nblocks_for( ncell ), TPB, 0, st>>>( ncell, cs.start.p,
+                                                           cs.perm.p );
+        CMD_LAUNCH_CHECK();
+    }
+    return CMD_OK;
+}
+
+// ---------------------------------------------------------------- NVE
+// integrator_nve.h:91-101 (TagInitial) and :103-110 (TagFinal).
+__global__ void k_nve_initial( int n, int cap, double4 *__restrict__ x,
+                               double *__restrict__ v,
+                               const double *__restrict__ f, MassTable mt,
+                               double dtf, double dtv )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    double4 p = x[i];
+    const double dtfm = dtf / mt.m[(int)p.w];
+    double vx = v[i] + dtfm * f[i];
+    double vy = v[cap + i] + dtfm * f[cap + i];
+    double vz = v[2 * cap + i] + dtfm * f[2 * cap + i];
+    v[i] = vx;
+    v[cap + i] = vy;
+    v[2 * cap + i] = vz;
+    p.x = p.x + dtv * vx;
+    p.y = p.y + dtv * vy;
+    p.z = p.z + dtv * vz;
+    x[i] = p;
+}
+
+__global__ void k_nve_final( int n, int cap, const double4 *__restrict__ x,
+                             double *__restrict__ v,
+                             const double *__restrict__ f, MassTable mt,
+                             double dtf )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    const double dtfm = dtf / mt.m[(int)x[i].w];
+    v[i] = v[i] + dtfm * f[i];
+    v[cap + i] = v[cap + i] + dtfm * f[cap + i];
+    v[2 * cap + i] = v[2 * cap + i] + dtfm * f[2 * cap + i];
+}
+
+// Partial sums of one block (256 threads) into partial[blockIdx*4 + k].
+__device__ void block_partials4( double a, double b, double c, double d,
+                                 double *__restrict__ partial )
+{
+    __shared__ double red[4][TPB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for ( int o = 32; o > 0; o >>= 1 )
+    {
+        a += __shfl_down( a, o, 64 );
+        b += __shfl_down( b, o, 64 );
+        c += __shfl_down( c, o, 64 );
+        d += __shfl_down( d, o, 64 );
+    }
+    if ( lane == 0 )
+    {
+        red[0][wid] = a;
+        red[1][wid] = b;
+        red[2][wid] = c;
+        red[3][wid] = d;
+    }
+    __syncthreads();
+    if ( threadIdx.x < 4 )
+    {
+        double s = 0.0;
+        for ( int w = 0; w < TPB / 64; ++w )
+            s += red[threadIdx.x][w];
+        partial[blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+// Fused final kick of this step, thermo partials, and (optionally) the next
+// step's half kick + drift (+ wrap) into x_next.  Used after half-list
+// forces, whose per-atom totals are only known after the atomics finish.
+__global__ void k_final_initial( int n, int cap, const double4 *__restrict__ x,
+                                 double4 *__restrict__ x_next,
+                                 double *__restrict__ v,
+                                 const double *__restrict__ f, MassTable mt,
+                                 StepCtl ctl, double *__restrict__ partial )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double mv2 = 0.0;
+    if ( i < n )
+    {
+        double4 p = x[i];
+        const double m = mt.m[(int)p.w];
+        const double dtfm = ctl.dtf / m;
+        const double fx = f[i], fy = f[cap + i], fz = f[2 * cap + i];
+        double vx = v[i] + dtfm * fx;
+        double vy = v[cap + i] + dtfm * fy;
+        double vz = v[2 * cap + i] + dtfm * fz;
+        mv2 = ( vx * vx + vy * vy + vz * vz ) * m;
+        if ( ctl.do_next_initial )
+        {
+            vx = vx + dtfm * fx;
+            vy = vy + dtfm * fy;
+            vz = vz + dtfm * fz;
+            p.x = p.x + ctl.dtv * vx;
+            p.y = p.y + ctl.dtv * vy;
+            p.z = p.z + ctl.dtv * vz;
+            if ( ctl.wrap_next )
+            {
+                const double cx = p.x, cy = p.y, cz = p.z;
+                if ( cx > ctl.L[0] )
+                    p.x = cx - ctl.L[0];
+                if ( cx < 0 )
+                    p.x = cx + ctl.L[0];
+                if ( cy > ctl.L[1] )
+                    p.y = cy - ctl.L[1];
+                if ( cy < 0 )
+                    p.y = cy + ctl.L[1];
+                if ( cz > ctl.L[2] )
+                    p.z = cz - ctl.L[2];
+                if ( cz < 0 )
+                    p.z = cz + ctl.L[2];
+            }
+            x_next[i] = p;
+        }
+        v[i] = vx;
+        v[cap + i] = vy;
+        v[2 * cap + i] = vz;
+    }
+    if ( ctl.thermo )
+        block_partials4( mv2, 0.0, 0.0, 0.0, partial );
+}
+
+cmd_status nve_final_initial( cmd_system *S, const StepCtl &ctl,
+                              double4 *x_next, double *partial,
+                              int *nblocks_out )
+{
+    const int n = S->n_local;
+    const int nb = nblocks_for( n );
+    if ( nblocks_out )
+        *nblocks_out = nb;
+    k_final_initial<<<nb, TPB, 0, S->stream>>>( n, S->cap, S->x.p, x_next,
+                                                S->v.p, S->f.p,
+                                                S->mass_table(), ctl, partial );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// Deterministic reduction of nblocks rows of 4 partials (fixed order).
+__global__ void k_reduce_partials( const double *__restrict__ partial,
+                                   int nblocks, double *__restrict__ out )
+{
+    double s[4] = { 0.0, 0.0, 0.0, 0.0 };
+    for ( int b = threadIdx.x; b < nblocks; b += TPB )
+        for ( int k = 0; k < 4; ++k )
+            s[k] += partial[b * 4 + k];
+    double tmp[4];
+    for ( int k = 0; k < 4; ++k )
+        tmp[k] = s[k];
+    // reuse the block helper with blockIdx.x == 0 semantics via out
+    __shared__ double red[4][TPB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for ( int k = 0; k < 4; ++k )
+    {
+        double a = tmp[k];
+#pragma unroll
+        for ( int o = 32; o > 0; o >>= 1 )
+            a += __shfl_down( a, o, 64 );
+        if ( lane == 0 )
+            red[k][wid] = a;
+    }
+    __syncthreads();
+    if ( threadIdx.x < 4 )
+    {
+        double a = 0.0;
+        for ( int w = 0; w < TPB / 64; ++w )
+            a += red[threadIdx.x][w];
+        out[threadIdx.x] = a;
+    }
+}
+
+cmd_status reduce_partials( const double *partial, int nblocks, double *out,
+                            hipStream_t st )
+{
+    k_reduce_partials<<<1, TPB, 0, st>>>( partial, nblocks, out );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// ---------------------------------------------------------------- wrap
+// comm_mpi.h:141-168: per dimension, x > L -> x - L, x < 0 -> x + L, both
+// tested on the value before the update.
+__global__ void k_wrap( int n, double4 *__restrict__ x, double Lx, double Ly,
+                        double Lz )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    double4 p = x[i];
+    const double cx = p.x, cy = p.y, cz = p.z;
+    if ( cx > Lx )
+        p.x = cx - Lx;
+    if ( cx < 0 )
+        p.x = cx + Lx;
+    if ( cy > Ly )
+        p.y = cy - Ly;
+    if ( cy < 0 )
+        p.y = cy + Ly;
+    if ( cz > Lz )
+        p.z = cz - Lz;
+    if ( cz < 0 )
+        p.z = cz + Lz;
+    x[i] = p;
+}
+
+cmd_status wrap_local( cmd_system *S )
+{
+    if ( S->n_local == 0 )
+        return CMD_OK;
+    k_wrap<<<nblocks_for( S->n_local ), TPB, 0, S->stream>>>(
+        S->n_local, S->x.p, S->dom.L[0], S->dom.L[1], S->dom.L[2] );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// ---------------------------------------------------------------- ghosts
+// TagHaloPack (comm_mpi.h:240-319): flag atoms of the current set near the
+// face of phase p.
+__global__ void k_ghost_flag( int n, const double4 *__restrict__ x, int dim,
+                              int even, double thr, int *__restrict__ flag )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    const double4 p = x[i];
+    const double c = dim == 0 ? p.x : ( dim == 1 ? p.y : p.z );
+    flag[i] = even ? ( c >= thr ) : ( c <= thr );
+}
+
+// Cabana::gather + TagHaloPBC (comm_mpi.h:323-353): append the flagged
+// atoms as ghosts in pack order, shifted by -L (even phase) or +L (odd).
+__global__ void k_ghost_scatter( int n, double4 *__restrict__ x,
+                                 int *__restrict__ id,
+                                 const int *__restrict__ flag,
+                                 const int *__restrict__ pos, int base,
+                                 int n_local, int send_off, int phase,
+                                 double L, int *__restrict__ send_idx,
+                                 int *__restrict__ root, int *__restrict__ code )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n || !flag[i] )
+        return;
+    const int k = pos[i];
+    const int g = base + k;
+    double4 p = x[i];
+    const int dim = phase / 2;
+    const bool even = ( phase % 2 ) == 0;
+    if ( dim == 0 )
+        p.x = even ? p.x - L : p.x + L;
+    else if ( dim == 1 )
+        p.y = even ? p.y - L : p.y + L;
+    else
+        p.z = even ? p.z - L : p.z + L;
+    x[g] = p;
+    id[g] = id[i];
+    send_idx[send_off + k] = i;
+    const int gi = g - n_local;
+    if ( i < n_local )
+    {
+        root[gi] = i;
+        code[gi] = 1 << phase;
+    }
+    else
+    {
+        root[gi] = root[i - n_local];
+        code[gi] = code[i - n_local] | ( 1 << phase );
+    }
+}
+
+// update_halo (comm_mpi_impl.h:369-408), resolved: every ghost coordinate is
+// its root's coordinate with at most one +-L per dimension, exactly the
+// value the phase-ordered gather + shift produces.
+__global__ void k_ghost_refresh( int n_local, int n_ghost,
+                                 double4 *__restrict__ x,
+                                 const int *__restrict__ root,
+                                 const int *__restrict__ code, double Lx,
+                                 double Ly, double Lz )
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( g >= n_ghost )
+        return;
+    double4 p = x[root[g]];
+    const int c = code[g];
+    if ( c & 1 )
+        p.x = p.x - Lx;
+    if ( c & 2 )
+        p.x = p.x + Lx;
+    if ( c & 4 )
+        p.y = p.y - Ly;
+    if ( c & 8 )
+        p.y = p.y + Ly;
+    if ( c & 16 )
+        p.z = p.z - Lz;
+    if ( c & 32 )
+        p.z = p.z + Lz;
+    x[n_local + g] = p;
+}
+
+// update_force (comm_mpi_impl.h:410-441): add ghost forces into their roots.
+__global__ void k_ghost_fold( int n_local, int n_ghost, int cap,
+                              double *__restrict__ f,
+                              const int *__restrict__ root )
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( g >= n_ghost )
+        return;
+    const int r = root[g];
+    const int s = n_local + g;
+    atomicAdd( &f[r], f[s] );
+    atomicAdd( &f[cap + r], f[cap + s] );
+    atomicAdd( &f[2 * cap + r], f[2 * cap + s] );
+}
+
+cmd_status ghost_refresh( cmd_system *S )
+{
+    if ( S->n_ghost == 0 )
+        return CMD_OK;
+    k_ghost_refresh<<<nblocks_for( S->n_ghost ), TPB, 0, S->stream>>>(
+        S->n_local, S->n_ghost, S->x.p, S->ghost_root.p, S->ghost_code.p,
+        S->dom.L[0], S->dom.L[1], S->dom.L[2] );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// ---------------------------------------------------------------- permute
+__global__ void k_permute( int n, int cap, const int *__restrict__ perm,
+                           const double4 *__restrict__ x,
+                           double4 *__restrict__ xo,
+                           const double *__restrict__ v,
+                           double *__restrict__ vo,
+                           const double *__restrict__ f,
+                           double *__restrict__ fo, const int *__restrict__ id,
+                           int *__restrict__ ido )
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( k >= n )
+        return;
+    const int s = perm[k];
+    xo[k] = x[s];
+    vo[k] = v[s];
+    vo[cap + k] = v[cap + s];
+    vo[2 * cap + k] = v[2 * cap + s];
+    fo[k] = f[s];
+    fo[cap + k] = f[cap + s];
+    fo[2 * cap + k] = f[2 * cap + s];
+    ido[k] = id[s];
+}
+
+// ---------------------------------------------------------------- thermo
+__global__ void k_mv2_partials( int n, int cap, const double4 *__restrict__ x,
+                                const double *__restrict__ v, MassTable mt,
+                                double *__restrict__ partial )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if ( i < n )
+    {
+        const double vx = v[i], vy = v[cap + i], vz = v[2 * cap + i];
+        s = ( vx * vx + vy * vy + vz * vz ) * mt.m[(int)x[i].w];
+    }
+    block_partials4( s, 0.0, 0.0, 0.0, partial );
+}
+
+static cmd_status sum_mv2( cmd_system *S, double *out )
+{
+    const int n = S->n_local;
+    const int nb = nblocks_for( n );
+    CMD_TRY( S->sc.dpart.reserve( 4 * (size_t)nb ) );
+    CMD_TRY( S->sc.dres.reserve( 8 ) );
+    k_mv2_partials<<<nb, TPB, 0, S->stream>>>( n, S->cap, S->x.p, S->v.p,
+                                              S->mass_table(), S->sc.dpart.p );
+    CMD_LAUNCH_CHECK();
+    CMD_TRY( reduce_partials( S->sc.dpart.p, nb, S->sc.dres.p, S->stream ) );
+    double h[4];
+    CMD_HIP( hipMemcpyAsync( h, S->sc.dres.p, sizeof( h ),
+                             hipMemcpyDeviceToHost, S->stream ) );
+    CMD_HIP( hipStreamSynchronize( S->stream ) );
+    *out = h[0];
+    return CMD_OK;
+}
+
+} // namespace cmd
+
+using namespace cmd;
+
+// =================================================================== ABI
+extern "C" {
+
+const char *cmd_last_error( void ) { return g_err.c_str(); }
+
+int cmd_version( void ) { return 1; }
+
+cmd_status cmd_device_count( int *count )
+{
+    CMD_REQUIRE( count, CMD_ERR_ARG, "count is NULL" );
+    int c = 0;
+    hipError_t e = hipGetDeviceCount( &c );
+    if ( e != hipSuccess )
+        c = 0;
+    *count = c;
+    return CMD_OK;
+}
+
+cmd_status cmd_device_init( int device )
+{
+    int c = 0;
+    if ( hipGetDeviceCount( &c ) != hipSuccess || c == 0 )
+    {
+        set_error( "no HIP device visible" );
+        return CMD_ERR_NO_DEVICE;
+    }
+    CMD_REQUIRE( device >= 0 && device < c, CMD_ERR_ARG,
+                 "device %d out of range (%d visible)", device, c );
+    CMD_HIP( hipSetDevice( device ) );
+    return CMD_OK;
+}
+
+cmd_status cmd_device_synchronize( void )
+{
+    CMD_HIP( hipDeviceSynchronize() );
+    return CMD_OK;
+}
+
+cmd_status cmd_domain_create( const double low[3], const double high[3],
+                              double ghost_cutoff, cmd_domain *d )
+{
+    CMD_REQUIRE( low && high && d, CMD_ERR_ARG, "NULL argument" );
+    for ( int k = 0; k < 3; ++k )
+    {
+        CMD_REQUIRE( high[k] > low[k], CMD_ERR_ARG,
+                     "domain high <= low in dim %d", k );
+        d->low[k] = low[k];
+        d->high[k] = high[k];
+        d->L[k] = high[k] - low[k];
+        d->cell[k] = ( high[k] - low[k] ) / 100.0;
+    }
+    const double min_cell = std::min( std::min( d->cell[0], d->cell[1] ),
+                                      d->cell[2] );
+    d->halo_width = (int)std::ceil( ghost_cutoff / min_cell );
+    for ( int k = 0; k < 3; ++k )
+    {
+        d->local_lo[k] = low[k] + d->cell[k] * 0.0;
+        d->local_hi[k] = low[k] + d->cell[k] * 100.0;
+        d->local_mesh[k] = d->local_hi[k] - d->local_lo[k];
+        d->ghost_lo[k] = low[k] + d->cell[k] * ( 0.0 - d->halo_width );
+        d->ghost_hi[k] = low[k] + d->cell[k] * ( 100.0 + d->halo_width );
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_system_create( int ntypes, cmd_system **out )
+{
+    CMD_REQUIRE( out, CMD_ERR_ARG, "out is NULL" );
+    CMD_REQUIRE( ntypes >= 1 && ntypes <= CMD_MAX_TYPES, CMD_ERR_ARG,
+                 "ntypes %d outside [1,%d]", ntypes, CMD_MAX_TYPES );
+    int c = 0;
+    if ( hipGetDeviceCount( &c ) != hipSuccess || c == 0 )
+    {
+        set_error( "no HIP device visible" );
+        return CMD_ERR_NO_DEVICE;
+    }
+    cmd_system *s = new cmd_system;
+    s->ntypes = ntypes;
+    for ( int k = 0; k < CMD_MAX_TYPES; ++k )
+        s->mass[k] = 1.0;
+    hipError_t e = hipStreamCreateWithFlags( &s->stream, hipStreamNonBlocking );
+    if ( e != hipSuccess )
+    {
+        delete s;
+        set_error( "hipStreamCreate failed: %s", hipGetErrorString( e ) );
+        return CMD_ERR_HIP;
+    }
+    s->own_stream = true;
+    *out = s;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_destroy( cmd_system *s )
+{
+    if ( !s )
+        return CMD_OK;
+    if ( s->stream )
+        (void)hipStreamSynchronize( s->stream );
+    if ( s->own_stream && s->stream )
+        (void)hipStreamDestroy( s->stream );
+    delete s;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_set_stream( cmd_system *s, void *hip_stream )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    CMD_HIP( hipStreamSynchronize( s->stream ) );
+    if ( s->own_stream && s->stream )
+        (void)hipStreamDestroy( s->stream );
+    s->stream = (hipStream_t)hip_stream;
+    s->own_stream = false;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_set_units( cmd_system *s, double dt, double mvv2e,
+                                 double boltz )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    CMD_REQUIRE( mvv2e > 0 && boltz > 0, CMD_ERR_ARG,
+                 "mvv2e and boltz must be positive" );
+    s->dt = dt;
+    s->mvv2e = mvv2e;
+    s->boltz = boltz;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_set_mass( cmd_system *s, int ntypes, const double *mass )
+{
+    CMD_REQUIRE( s && mass, CMD_ERR_ARG, "NULL argument" );
+    CMD_REQUIRE( ntypes >= 1 && ntypes <= CMD_MAX_TYPES, CMD_ERR_ARG,
+                 "ntypes %d outside [1,%d]", ntypes, CMD_MAX_TYPES );
+    for ( int k = 0; k < ntypes; ++k )
+    {
+        CMD_REQUIRE( mass[k] > 0, CMD_ERR_ARG, "mass of type %d is not positive",
+                     k + 1 );
+        s->mass[k] = mass[k];
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_system_set_domain( cmd_system *s, const cmd_domain *d )
+{
+    CMD_REQUIRE( s && d, CMD_ERR_ARG, "NULL argument" );
+    s->dom = *d;
+    s->has_domain = true;
+    return CMD_OK;
+}
+
+cmd_status cmd_system::reserve( int n )
+{
+    if ( n <= cap )
+        return CMD_OK;
+    const int newcap = std::max( n, cap + cap / 4 );
+    const int used = n_local + n_ghost;
+    // x, id, ghost maps: linear layout, copy the used prefix
+    CMD_TRY( x.reserve( newcap, stream, true, used ) );
+    CMD_TRY( x_alt.reserve( newcap ) );
+    CMD_TRY( id.reserve( newcap, stream, true, used ) );
+    CMD_TRY( id_alt.reserve( newcap ) );
+    CMD_TRY( ghost_root.reserve( newcap, stream, true, n_ghost ) );
+    CMD_TRY( ghost_code.reserve( newcap, stream, true, n_ghost ) );
+    CMD_TRY( send_idx.reserve( newcap, stream, true, n_ghost ) );
+    CMD_TRY( flag.reserve( newcap ) );
+    CMD_TRY( pos.reserve( newcap + 1 ) );
+    // planar v and f: copy plane by plane
+    auto regrow_planes = [&]( DevBuf<double> &b ) -> cmd_status
+    {
+        DevBuf<double> nb;
+        CMD_TRY( nb.reserve( 3 * (size_t)newcap ) );
+        if ( b.p && used > 0 )
+        {
+            for ( int k = 0; k < 3; ++k )
+                CMD_HIP( hipMemcpyAsync( nb.p + (size_t)k * newcap,
+                                         b.p + (size_t)k * cap,
+                                         sizeof( double ) * used,
+                                         hipMemcpyDeviceToDevice, stream ) );
+            CMD_HIP( hipStreamSynchronize( stream ) );
+        }
+        b.swap( nb );
+        return CMD_OK;
+    };
+    CMD_TRY( regrow_planes( v ) );
+    CMD_TRY( regrow_planes( f ) );
+    v_alt.release();
+    f_alt.release();
+    CMD_TRY( v_alt.reserve( 3 * (size_t)newcap ) );
+    CMD_TRY( f_alt.reserve( 3 * (size_t)newcap ) );
+    cap = newcap;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_reserve( cmd_system *s, int n )
+{
+    CMD_REQUIRE( s && n >= 0, CMD_ERR_ARG, "bad argument" );
+    return s->reserve( n );
+}
+
+cmd_status cmd_system_upload( cmd_system *s, int n_local, int n_ghost,
+                              const double *x, const double *v,
+                              const double *f, const int *type, const int *id )
+{
+    CMD_REQUIRE( s && x && type, CMD_ERR_ARG, "system, x and type are required" );
+    CMD_REQUIRE( n_local >= 0 && n_ghost >= 0, CMD_ERR_ARG, "negative count" );
+    const int n = n_local + n_ghost;
+    for ( int i = 0; i < n; ++i )
+        CMD_REQUIRE( type[i] >= 0 && type[i] < s->ntypes, CMD_ERR_ARG,
+                     "atom %d has type %d outside [0,%d)", i, type[i],
+                     s->ntypes );
+    s->n_local = 0;
+    s->n_ghost = 0;
+    CMD_TRY( s->reserve( std::max( n, 1 ) ) );
+    const int cap = s->cap;
+    std::vector<double4> hx( std::max( n, 1 ) );
+    std::vector<double> hv( 3 * (size_t)cap, 0.0 ), hf( 3 * (size_t)cap, 0.0 );
+    std::vector<int> hid( std::max( n, 1 ) );
+    for ( int i = 0; i < n; ++i )
+    {
+        hx[i] = make_double4( x[3 * i], x[3 * i + 1], x[3 * i + 2],
+                              (double)type[i] );
+        for ( int k = 0; k < 3; ++k )
+        {
+            hv[(size_t)k * cap + i] = v ? v[3 * i + k] : 0.0;
+            hf[(size_t)k * cap + i] = f ? f[3 * i + k] : 0.0;
+        }
+        hid[i] = id ? id[i] : i + 1;
+    }
+    if ( n > 0 )
+    {
+        CMD_HIP( hipMemcpyAsync( s->x.p, hx.data(), sizeof( double4 ) * n,
+                                 hipMemcpyHostToDevice, s->stream ) );
+        CMD_HIP( hipMemcpyAsync( s->id.p, hid.data(), sizeof( int ) * n,
+                                 hipMemcpyHostToDevice, s->stream ) );
+    }
+    CMD_HIP( hipMemcpyAsync( s->v.p, hv.data(), sizeof( double ) * 3 * cap,
+                             hipMemcpyHostToDevice, s->stream ) );
+    CMD_HIP( hipMemcpyAsync( s->f.p, hf.data(), sizeof( double ) * 3 * cap,
+                             hipMemcpyHostToDevice, s->stream ) );
+    CMD_HIP( hipStreamSynchronize( s->stream ) );
+    s->n_local = n_local;
+    s->n_ghost = n_ghost;
+    s->N = n_local;
+    for ( int k = 0; k < 6; ++k )
+        s->counts[k] = 0;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_download( cmd_system *s, double *x, double *v, double *f,
+                                int *type, int *id )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    const int n = s->n_local + s->n_ghost;
+    const int cap = s->cap;
+    if ( n == 0 )
+        return CMD_OK;
+    if ( x || type )
+    {
+        std::vector<double4> hx( n );
+        CMD_HIP( hipMemcpyAsync( hx.data(), s->x.p, sizeof( double4 ) * n,
+                                 hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+        for ( int i = 0; i < n; ++i )
+        {
+            if ( x )
+            {
+                x[3 * i] = hx[i].x;
+                x[3 * i + 1] = hx[i].y;
+                x[3 * i + 2] = hx[i].z;
+            }
+            if ( type )
+                type[i] = (int)hx[i].w;
+        }
+    }
+    auto planes = [&]( const double *d, double *h ) -> cmd_status
+    {
+        std::vector<double> t( 3 * (size_t)cap );
+        CMD_HIP( hipMemcpyAsync( t.data(), d, sizeof( double ) * 3 * cap,
+                                 hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+        for ( int i = 0; i < n; ++i )
+            for ( int k = 0; k < 3; ++k )
+                h[3 * i + k] = t[(size_t)k * cap + i];
+        return CMD_OK;
+    };
+    if ( v )
+        CMD_TRY( planes( s->v.p, v ) );
+    if ( f )
+        CMD_TRY( planes( s->f.p, f ) );
+    if ( id )
+    {
+        CMD_HIP( hipMemcpyAsync( id, s->id.p, sizeof( int ) * n,
+                                 hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_system_counts( const cmd_system *s, int *n_global, int *n_local,
+                              int *n_ghost )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    if ( n_global )
+        *n_global = s->N;
+    if ( n_local )
+        *n_local = s->n_local;
+    if ( n_ghost )
+        *n_ghost = s->n_ghost;
+    return CMD_OK;
+}
+
+cmd_status cmd_system_footprint( const cmd_system *s, int64_t *bytes )
+{
+    CMD_REQUIRE( s && bytes, CMD_ERR_ARG, "NULL argument" );
+    // x, x_alt (32 B each), v, v_alt, f, f_alt (24 B each), id, id_alt,
+    // ghost maps, flags, positions (4 B each)
+    *bytes = (int64_t)s->cap * ( 2 * 32 + 4 * 24 + 7 * 4 );
+    return CMD_OK;
+}
+
+cmd_status cmd_nve_initial( cmd_system *s )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    if ( s->n_local == 0 )
+        return CMD_OK;
+    const double dtf = 0.5 * s->dt / s->mvv2e;
+    k_nve_initial<<<nblocks_for( s->n_local ), TPB, 0, s->stream>>>(
+        s->n_local, s->cap, s->x.p, s->v.p, s->f.p, s->mass_table(), dtf,
+        s->dt );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+cmd_status cmd_nve_final( cmd_system *s )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    if ( s->n_local == 0 )
+        return CMD_OK;
+    const double dtf = 0.5 * s->dt / s->mvv2e;
+    k_nve_final<<<nblocks_for( s->n_local ), TPB, 0, s->stream>>>(
+        s->n_local, s->cap, s->x.p, s->v.p, s->f.p, s->mass_table(), dtf );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+cmd_status cmd_comm_exchange( cmd_system *s )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
+    s->n_ghost = 0; // Comm::exchange resizes to N_local (comm_mpi_impl.h:270-271)
+    return wrap_local( s );
+}
+
+cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
+    const int n_local = s->n_local;
+    int n_ghost = 0, send_total = 0;
+    for ( int phase = 0; phase < 6; ++phase )
+    {
+        const int dim = phase / 2;
+        const int even = ( phase % 2 ) == 0;
+        const int nparticles =
+            n_local + n_ghost - ( even ? 0 : s->counts[phase - 1] );
+        const double thr = even ? s->dom.local_hi[dim] - comm_depth
+                                : s->dom.local_lo[dim] + comm_depth;
+        int64_t cnt = 0;
+        if ( nparticles > 0 )
+        {
+            k_ghost_flag<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
+                nparticles, s->x.p, dim, even, thr, s->flag.p );
+            CMD_LAUNCH_CHECK();
+        }
+        CMD_TRY( scan_exclusive( s->flag.p, s->pos.p, nparticles, s->sc,
+                                 s->stream, &cnt ) );
+        const int base = n_local + n_ghost;
+        if ( base + cnt > s->cap )
+        {
+            s->n_ghost = n_ghost; // keep the ghosts built so far on regrow
+            CMD_TRY( s->reserve( (int)( ( base + cnt ) * 1.1 ) + 64 ) );
+            // flags and positions are scratch: rebuild them at the new size
+            k_ghost_flag<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
+                nparticles, s->x.p, dim, even, thr, s->flag.p );
+            CMD_LAUNCH_CHECK();
+            CMD_TRY( scan_exclusive( s->flag.p, s->pos.p, nparticles, s->sc,
+                                     s->stream, &cnt ) );
+        }
+        if ( cnt > 0 )
+        {
+            k_ghost_scatter<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
+                nparticles, s->x.p, s->id.p, s->flag.p, s->pos.p, base,
+                n_local, send_total, phase, s->dom.L[dim], s->send_idx.p,
+                s->ghost_root.p, s->ghost_code.p );
+            CMD_LAUNCH_CHECK();
+        }
+        s->counts[phase] = (int)cnt;
+        send_total += (int)cnt;
+        n_ghost += (int)cnt;
+    }
+    s->n_ghost = n_ghost;
+    return CMD_OK;
+}
+
+cmd_status cmd_comm_update_halo( cmd_system *s )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    return ghost_refresh( s );
+}
+
+cmd_status cmd_comm_update_force( cmd_system *s )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    if ( s->n_ghost == 0 )
+        return CMD_OK;
+    k_ghost_fold<<<nblocks_for( s->n_ghost ), TPB, 0, s->stream>>>(
+        s->n_local, s->n_ghost, s->cap, s->f.p, s->ghost_root.p );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+cmd_status cmd_comm_ghost_lists( cmd_system *s, int *send_idx, int counts[6] )
+{
+    CMD_REQUIRE( s && counts, CMD_ERR_ARG, "NULL argument" );
+    int tot = 0;
+    for ( int k = 0; k < 6; ++k )
+    {
+        counts[k] = s->counts[k];
+        tot += s->counts[k];
+    }
+    if ( send_idx && tot > 0 )
+    {
+        CMD_HIP( hipMemcpyAsync( send_idx, s->send_idx.p, sizeof( int ) * tot,
+                                 hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+    }
+    return CMD_OK;
+}
+
+// Binning::create_binning (binning_cabana_impl.h:57-113) with the Cabana
+// LinkedCellList grid (floor, top-edge clamp, cardinal (i*ny + j)*nz + k).
+cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
+                               double dz_in, int halo_depth, int do_local,
+                               int do_ghost, int sort, int *perm_out )
+{
+    CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
+    CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
+    CMD_REQUIRE( dx_in > 0 && dy_in > 0 && dz_in > 0, CMD_ERR_ARG,
+                 "bin sizes must be positive" );
+    if ( !( do_local || do_ghost ) )
+        return CMD_OK;
+    CMD_REQUIRE( do_local && !do_ghost, CMD_ERR_ARG,
+                 "only local-atom binning (do_local=1, do_ghost=0) is supported, "
+                 "as used by CbnMD" );
+    const double din[3] = { dx_in, dy_in, dz_in };
+    int nbin[3];
+    double delta[3];
+    for ( int k = 0; k < 3; ++k )
+    {
+        nbin[k] = (int)( s->dom.local_mesh[k] / din[k] );
+        if ( nbin[k] == 0 )
+            nbin[k] = 1;
+        delta[k] = s->dom.local_mesh[k] / nbin[k];
+    }
+    const double eps = delta[0] / 1000;
+    CellGrid g;
+    g.floor_mode = 1;
+    for ( int k = 0; k < 3; ++k )
+    {
+        const double mn = -delta[k] * halo_depth - eps + s->dom.local_lo[k];
+        const double mx = delta[k] * halo_depth + eps + s->dom.local_hi[k];
+        g.min[k] = mn;
+        g.n[k] = (int)std::floor( ( mx - mn ) * ( 1.0 / delta[k] ) );
+        CMD_REQUIRE( g.n[k] >= 1, CMD_ERR_ARG, "empty bin grid" );
+        const double c = ( mx - mn ) / g.n[k];
+        g.rd[k] = 1.0 / c;
+    }
+    const int n = s->n_local;
+    CMD_TRY( cell_sort( s->x.p, n, g, s->bins, s->sc, s->stream ) );
+    if ( sort && n > 0 )
+    {
+        const int cap = s->cap;
+        k_permute<<<nblocks_for( n ), TPB, 0, s->stream>>>(
+            n, cap, s->bins.perm.p, s->x.p, s->x_alt.p, s->v.p, s->v_alt.p,
+            s->f.p, s->f_alt.p, s->id.p, s->id_alt.p );
+        CMD_LAUNCH_CHECK();
+        s->x.swap( s->x_alt );
+        s->v.swap( s->v_alt );
+        s->f.swap( s->f_alt );
+        s->id.swap( s->id_alt );
+        s->n_ghost = 0; // ghosts are not part of the permuted range
+    }
+    if ( perm_out && n > 0 )
+    {
+        CMD_HIP( hipMemcpyAsync( perm_out, s->bins.perm.p, sizeof( int ) * n,
+                                 hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_thermo_temperature( cmd_system *s, double *T )
+{
+    CMD_REQUIRE( s && T, CMD_ERR_ARG, "NULL argument" );
+    double mv2 = 0.0;
+    CMD_TRY( sum_mv2( s, &mv2 ) );
+    const int dof = 3 * s->N - 3;
+    const double factor = s->mvv2e / ( 1.0 * dof * s->boltz );
+    *T = mv2 * factor;
+    return CMD_OK;
+}
+
+cmd_status cmd_thermo_kinetic( cmd_system *s, double *ke )
+{
+    CMD_REQUIRE( s && ke, CMD_ERR_ARG, "NULL argument" );
+    double mv2 = 0.0;
+    CMD_TRY( sum_mv2( s, &mv2 ) );
+    *ke = mv2 * ( 0.5 * s->mvv2e );
+    return CMD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,626 @@
+// host_input.cpp -- input deck, run parameters and the initial state.
+//
+// Restates InputFile<System> (inputFile.h:150-275, inputFile_impl.h:56-865)
+// on the host: the LAMMPS-subset parser, the sc/fcc lattice generator and
+// the LAMMPS "loop geom" velocity generator, so a deck produces the same
+// atoms, ids, types and velocities as the reference.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cabanamd_amd.h"
+
+namespace cmd
+{
+void set_error( const char *fmt, ... );
+}
+
+namespace
+{
+// split on spaces, tabs, CR and LF, dropping empty words
+// (inputFile_impl.h:56-69)
+std::vector<std::string> split_words( const std::string &line )
+{
+    std::vector<std::string> w;
+    std::string cur;
+    for ( char c : line )
+    {
+        if ( c == ' ' || c == '\t' || c == '\r' || c == '\n' )
+        {
+            if ( !cur.empty() )
+                w.push_back( cur );
+            cur.clear();
+        }
+        else
+            cur.push_back( c );
+    }
+    if ( !cur.empty() )
+        w.push_back( cur );
+    return w;
+}
+
+// ---- LAMMPS_RandomVelocityGeom (inputFile.h:66-148) ------------------------
+// Park-Miller minimal standard generator (Schrage's method), Marsaglia polar
+// gaussian, seeded by a Jenkins one-at-a-time hash of the base seed's bytes
+// and the atom's 24 coordinate bytes (plain char: signed on x86), keeping 27
+// bits, followed by 5 warm-up draws.
+class VelocityRng
+{
+    int seed_ = 0;
+
+  public:
+    double uniform()
+    {
+        const int IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836;
+        int k = seed_ / IQ;
+        seed_ = IA * ( seed_ - k * IQ ) - IR * k;
+        if ( seed_ < 0 )
+            seed_ += IM;
+        return ( 1.0 / IM ) * seed_;
+    }
+    double gaussian()
+    {
+        double v1, v2, rsq;
+        do
+        {
+            v1 = 2.0 * uniform() - 1.0;
+            v2 = 2.0 * uniform() - 1.0;
+            rsq = v1 * v1 + v2 * v2;
+        } while ( ( rsq >= 1.0 ) || ( rsq == 0.0 ) );
+        return v2 * std::sqrt( -2.0 * std::log( rsq ) / rsq );
+    }
+    void reset( int ibase, const double *coord )
+    {
+        unsigned int h = 0;
+        auto mix = [&h]( const char *bytes, int nb )
+        {
+            for ( int i = 0; i < nb; ++i )
+            {
+                h += bytes[i];
+                h += ( h << 10 );
+                h ^= ( h >> 6 );
+            }
+        };
+        mix( reinterpret_cast<const char *>( &ibase ), (int)sizeof( int ) );
+        mix( reinterpret_cast<const char *>( coord ), 3 * (int)sizeof( double ) );
+        h += ( h << 3 );
+        h ^= ( h >> 11 );
+        h += ( h << 15 );
+        seed_ = (int)( h & 0x7ffffff );
+        if ( !seed_ )
+            seed_ = 1;
+        for ( int i = 0; i < 5; ++i )
+            uniform();
+    }
+};
+
+// Lattice loop bounds and the half-open site test (inputFile_impl.h:562-588,
+// 598-606, 692-699).
+struct SiteWalk
+{
+    int lo[3], hi[3];
+    double maxv[3];
+};
+
+SiteWalk site_walk( const cmd_params *p, const cmd_domain *d )
+{
+    SiteWalk w;
+    const double a = p->lattice_constant;
+    for ( int k = 0; k < 3; ++k )
+    {
+        w.maxv[k] = a * p->region_hi[k];
+        w.lo[k] = (int)( d->local_lo[k] / a - 0.5 );
+        w.hi[k] = (int)std::max( std::min( w.maxv[k] / a, d->local_hi[k] / a + 0.5 ),
+                                 static_cast<double>( w.lo[k] ) );
+        if ( w.lo[k] == w.hi[k] )
+            w.hi[k] -= 1;
+    }
+    return w;
+}
+
+inline bool site_inside( const cmd_domain *d, const SiteWalk &w, double x,
+                         double y, double z )
+{
+    return ( x >= d->local_lo[0] ) && ( y >= d->local_lo[1] ) &&
+           ( z >= d->local_lo[2] ) && ( x < d->local_hi[0] ) &&
+           ( y < d->local_hi[1] ) && ( z < d->local_hi[2] ) && ( x < w.maxv[0] ) &&
+           ( y < w.maxv[1] ) && ( z < w.maxv[2] );
+}
+
+template <class Emit>
+int walk_sites( const cmd_params *p, const cmd_domain *d, Emit emit )
+{
+    const double a = p->lattice_constant;
+    const SiteWalk w = site_walk( p, d );
+    int n = 0;
+    if ( p->lattice_style == 0 )
+    {
+        // simple cubic (inputFile_impl.h:590-655)
+        for ( int iz = w.lo[2]; iz <= w.hi[2]; ++iz )
+        {
+            const double z = a * ( iz + p->lattice_offset[2] );
+            for ( int iy = w.lo[1]; iy <= w.hi[1]; ++iy )
+            {
+                const double y = a * ( iy + p->lattice_offset[1] );
+                for ( int ix = w.lo[0]; ix <= w.hi[0]; ++ix )
+                {
+                    const double x = a * ( ix + p->lattice_offset[0] );
+                    if ( site_inside( d, w, x, y, z ) )
+                        emit( n++, x, y, z );
+                }
+            }
+        }
+        return n;
+    }
+    // face-centred cubic, 4-site basis (inputFile_impl.h:658-792)
+    const double base[4][3] = { { 0.0, 0.0, 0.0 },
+                                { 0.5, 0.5, 0.0 },
+                                { 0.5, 0.0, 0.5 },
+                                { 0.0, 0.5, 0.5 } };
+    double basis[4][3];
+    for ( int b = 0; b < 4; ++b )
+        for ( int k = 0; k < 3; ++k )
+            basis[b][k] = base[b][k] + p->lattice_offset[k];
+    for ( int iz = w.lo[2]; iz <= w.hi[2]; ++iz )
+        for ( int iy = w.lo[1]; iy <= w.hi[1]; ++iy )
+            for ( int ix = w.lo[0]; ix <= w.hi[0]; ++ix )
+                for ( int b = 0; b < 4; ++b )
+                {
+                    const double x = a * ( 1.0 * ix + basis[b][0] );
+                    const double y = a * ( 1.0 * iy + basis[b][1] );
+                    const double z = a * ( 1.0 * iz + basis[b][2] );
+                    if ( site_inside( d, w, x, y, z ) )
+                        emit( n++, x, y, z );
+                }
+    return n;
+}
+
+cmd_status deck_error( const std::string &msg )
+{
+    cmd::set_error( "%s", msg.c_str() );
+    return CMD_ERR_IO;
+}
+
+} // namespace
+
+extern "C" {
+
+// Defaults of InputFile (inputFile.h:150-275, inputFile_impl.h:70-84) with
+// units lj applied.
+cmd_status cmd_params_default( cmd_params *p )
+{
+    if ( !p )
+    {
+        cmd::set_error( "params is NULL" );
+        return CMD_ERR_ARG;
+    }
+    std::memset( p, 0, sizeof( *p ) );
+    p->units = 1;
+    p->lattice_style = 1;
+    p->lattice_constant = 0.8442;
+    p->ntypes = 1;
+    for ( int k = 0; k < CMD_MAX_TYPES; ++k )
+        p->mass[k] = 1.0;
+    p->force_cutoff = 2.5;
+    p->skin = 0.0;
+    p->every = 20;
+    p->max_neigh_guess = 50;
+    p->comm_ghost_cutoff = std::pow( ( 4.0 / 0.8442 ), ( 1.0 / 3.0 ) ) * 20.0;
+    p->half = 0;
+    p->nsteps = 100;
+    p->thermo_rate = 10;
+    p->dt = 0.005;
+    p->mvv2e = 1.0;
+    p->boltz = 1.0;
+    p->force_type = 0;
+    return CMD_OK;
+}
+
+// check_lammps_command (inputFile_impl.h:125-534) over every line of a deck.
+// force_iteration: -1 keeps the deck's newton setting, 0 forces a full list,
+// 1 a half list (the --force-iteration override, inputFile_impl.h:489-515).
+cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
+                                  cmd_params *p )
+{
+    if ( !path || !p )
+    {
+        cmd::set_error( "NULL argument" );
+        return CMD_ERR_ARG;
+    }
+    std::ifstream in( path );
+    if ( !in )
+        return deck_error( std::string( "cannot open input deck " ) + path );
+    cmd_params_default( p );
+    bool timestepflag = false;
+    bool have_region = false;
+    std::string line;
+    int lineno = 0;
+    while ( std::getline( in, line ) )
+    {
+        ++lineno;
+        auto w = split_words( line );
+        if ( w.empty() || w[0][0] == '#' )
+            continue;
+        const std::string &key = w[0];
+        auto need = [&]( size_t n ) -> bool { return w.size() >= n; };
+        auto bad = [&]( const std::string &msg ) -> cmd_status
+        {
+            std::ostringstream os;
+            os << "LAMMPS-Command: " << msg << " (line " << lineno << ": " << line
+               << ")";
+            return deck_error( os.str() );
+        };
+        try
+        {
+            if ( key == "variable" )
+                return bad( "'variable' keyword is not supported in CabanaMD" );
+            else if ( key == "units" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'units' needs a style" );
+                if ( w[1] == "metal" )
+                {
+                    p->units = 2;
+                    p->boltz = 8.617343e-5;
+                    p->mvv2e = 1.0364269e-4;
+                    p->dt = 0.001;
+                }
+                else if ( w[1] == "real" )
+                {
+                    p->units = 0;
+                    p->boltz = 0.0019872067;
+                    p->mvv2e = 48.88821291 * 48.88821291;
+                    if ( !timestepflag )
+                        p->dt = 1.0;
+                }
+                else if ( w[1] == "lj" )
+                {
+                    p->units = 1;
+                    p->boltz = 1.0;
+                    p->mvv2e = 1.0;
+                    if ( !timestepflag )
+                        p->dt = 0.005;
+                }
+                else
+                    return bad( "'units' command only supports 'metal', 'real', "
+                                "and 'lj' in CabanaMD" );
+            }
+            else if ( key == "atom_style" )
+            {
+                if ( !need( 2 ) || ( w[1] != "atomic" && w[1] != "charge" ) )
+                    return bad( "'atom_style' command only supports 'atomic' and "
+                                "'charge' in CabanaMD" );
+            }
+            else if ( key == "lattice" )
+            {
+                if ( !need( 3 ) )
+                    return bad( "'lattice' needs a style and a value" );
+                if ( w[1] == "sc" )
+                {
+                    p->lattice_style = 0;
+                    p->lattice_constant = std::stod( w[2] );
+                }
+                else if ( w[1] == "fcc" )
+                {
+                    p->lattice_style = 1;
+                    p->lattice_constant =
+                        ( p->units == 1 )
+                            ? std::pow( ( 4.0 / std::stod( w[2] ) ), ( 1.0 / 3.0 ) )
+                            : std::stod( w[2] );
+                }
+                else
+                    return bad( "'lattice' command only supports 'sc' and 'fcc' "
+                                "in CabanaMD" );
+                if ( w.size() > 3 )
+                {
+                    if ( w[3] == "origin" && need( 7 ) )
+                    {
+                        p->lattice_offset[0] = std::stod( w[4] );
+                        p->lattice_offset[1] = std::stod( w[5] );
+                        p->lattice_offset[2] = std::stod( w[6] );
+                    }
+                    else
+                        return bad( "'lattice' command only supports 'origin' "
+                                    "additional option in CabanaMD" );
+                }
+            }
+            else if ( key == "region" )
+            {
+                if ( !need( 9 ) || w[2] != "block" )
+                    return bad( "'region' command only supports 'block' option "
+                                "in CabanaMD" );
+                if ( have_region )
+                    return bad( "only one region is supported by this engine" );
+                have_region = true;
+                p->region_lo[0] = std::stod( w[3] );
+                p->region_hi[0] = std::stod( w[4] );
+                p->region_lo[1] = std::stod( w[5] );
+                p->region_hi[1] = std::stod( w[6] );
+                p->region_lo[2] = std::stod( w[7] );
+                p->region_hi[2] = std::stod( w[8] );
+            }
+            else if ( key == "create_box" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'create_box' needs a type count" );
+                p->ntypes = std::stoi( w[1] );
+                if ( p->ntypes < 1 || p->ntypes > CMD_MAX_TYPES )
+                    return bad( "create_box type count outside [1,8]" );
+            }
+            else if ( key == "create_atoms" )
+            {
+                if ( !need( 3 ) || ( w[2] != "box" && w[2] != "region" ) )
+                    return bad( "'create_atoms' command only supports 'region' "
+                                "option in CabanaMD" );
+                if ( std::stoi( w[1] ) != 1 )
+                    return bad( "create_atoms supports atom type 1 only in this "
+                                "engine" );
+            }
+            else if ( key == "mass" )
+            {
+                if ( !need( 3 ) )
+                    return bad( "'mass' needs a type and a value" );
+                const int t = std::stoi( w[1] ) - 1;
+                if ( t < 0 || t >= CMD_MAX_TYPES )
+                    return bad( "mass type outside [1,8]" );
+                p->mass[t] = std::stod( w[2] );
+            }
+            else if ( key == "read_data" || key == "write_data" )
+                return bad( "'" + key +
+                            "' (LAMMPS data files) is not supported by this "
+                            "engine yet" );
+            else if ( key == "dump" )
+            {
+                // VTK output is outside this engine; the command is accepted
+                // with the reference's argument checks and otherwise ignored.
+                if ( !need( 6 ) || w[3] != "vtk" )
+                    return bad( "'dump' command only supports 'vtk' in CabanaMD" );
+                if ( w[2] != "all" )
+                    return bad( "'dump' command only supports dumping 'all' types "
+                                "in CabanaMD" );
+            }
+            else if ( key == "pair_style" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'pair_style' needs a style" );
+                if ( w[1] == "lj/cut" && need( 3 ) )
+                {
+                    p->force_type = 0;
+                    p->force_cutoff = std::stod( w[2] );
+                }
+                else if ( w[1] == "nnp" )
+                {
+                    p->force_type = 2;
+                    if ( need( 4 ) )
+                        std::snprintf( p->nnp_dir, sizeof( p->nnp_dir ), "%s",
+                                       w[3].c_str() );
+                }
+                else
+                    return bad( "'pair_style' command only supports 'lj/cut' and "
+                                "'nnp' style in CabanaMD" );
+            }
+            else if ( key == "pair_coeff" )
+            {
+                if ( p->force_type == 2 )
+                {
+                    if ( !need( 4 ) )
+                        return bad( "'pair_coeff' for nnp needs a cutoff" );
+                    p->force_cutoff = std::stod( w[3] );
+                }
+                else
+                {
+                    if ( !need( 6 ) )
+                        return bad( "'pair_coeff' needs i j eps sigma cut" );
+                    if ( p->ncoeff >= 16 )
+                        return bad( "too many pair_coeff lines" );
+                    const int c = p->ncoeff++;
+                    p->coeff_i[c] = std::stoi( w[1] );
+                    p->coeff_j[c] = std::stoi( w[2] );
+                    p->coeff_eps[c] = std::stod( w[3] );
+                    p->coeff_sigma[c] = std::stod( w[4] );
+                    p->coeff_cut[c] = std::stod( w[5] );
+                }
+            }
+            else if ( key == "velocity" )
+            {
+                if ( !need( 5 ) || w[2] != "create" )
+                    return bad( "'velocity' command can only be used with option "
+                                "'create' in CabanaMD" );
+                const int t = ( w[1] == "all" ) ? 1 : std::stoi( w[1] );
+                if ( t != 1 )
+                    return bad( "velocity create is supported for 'all' (type 1) "
+                                "only in this engine" );
+                p->temp_target = std::stod( w[3] );
+                p->temp_seed = std::stoi( w[4] );
+                for ( size_t k = 5; k + 1 < w.size(); ++k )
+                    if ( w[k] == "dist" && w[k + 1] == "gaussian" )
+                        p->vel_gaussian = 1;
+            }
+            else if ( key == "neighbor" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'neighbor' needs a skin" );
+                p->skin = std::stod( w[1] );
+            }
+            else if ( key == "neigh_modify" )
+            {
+                size_t i = 1;
+                while ( i < w.size() )
+                {
+                    if ( w[i] == "every" && i + 1 < w.size() )
+                    {
+                        p->every = std::stoi( w[i + 1] );
+                        i += 2;
+                    }
+                    else if ( w[i] == "one" && i + 1 < w.size() )
+                    {
+                        p->max_neigh_guess = std::stoi( w[i + 1] );
+                        i += 2;
+                    }
+                    else
+                        return bad( "'neigh_modify' only supports 'every' and "
+                                    "'one' in CabanaMD" );
+                }
+            }
+            else if ( key == "comm_modify" )
+            {
+                if ( !need( 4 ) || w[1] != "cutoff" || w[2] != "*" )
+                    return bad( "'comm_modify' command only supports single "
+                                "cutoff 'cutoff' for all atom types '*' in "
+                                "CabanaMD" );
+                p->comm_ghost_cutoff = std::stod( w[3] );
+            }
+            else if ( key == "fix" )
+            {
+                if ( !need( 4 ) || w[3] != "nve" )
+                    return bad( "'fix' command only supports 'nve' style in "
+                                "CabanaMD" );
+            }
+            else if ( key == "run" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'run' needs a step count" );
+                p->nsteps = std::stoi( w[1] );
+            }
+            else if ( key == "thermo" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'thermo' needs a rate" );
+                p->thermo_rate = std::stoi( w[1] );
+            }
+            else if ( key == "timestep" )
+            {
+                if ( !need( 2 ) )
+                    return bad( "'timestep' needs a value" );
+                p->dt = std::stod( w[1] );
+                timestepflag = true;
+            }
+            else if ( key == "newton" )
+            {
+                if ( !need( 2 ) || ( w[1] != "on" && w[1] != "off" ) )
+                    return bad( "'newton' must be followed by 'on' or 'off'" );
+                if ( force_iteration < 0 )
+                    p->half = ( w[1] == "on" ) ? 1 : 0;
+            }
+            else if ( key == "group" )
+            {
+                if ( !need( 4 ) || w[2] != "region" )
+                    return bad( "'group' command can only be used with 'region' "
+                                "in CabanaMD" );
+            }
+            else
+                return bad( "Unknown input file keyword" );
+        }
+        catch ( const std::exception & )
+        {
+            return bad( "malformed number" );
+        }
+    }
+    if ( force_iteration >= 0 )
+        p->half = force_iteration ? 1 : 0;
+    if ( !have_region )
+        return deck_error( "the deck defines no region block" );
+    return CMD_OK;
+}
+
+// create_lattice (inputFile_impl.h:536-865): sites, ids and types, then
+// velocities (loop-geom uniform or gaussian), zero net momentum and a
+// rescale to the target temperature.  Atoms are type 0 (create_atoms 1).
+cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
+                               double *x, double *v, int *type, int *id )
+{
+    if ( !p || !d || !n )
+    {
+        cmd::set_error( "NULL argument" );
+        return CMD_ERR_ARG;
+    }
+    if ( p->lattice_constant <= 0 )
+    {
+        cmd::set_error( "lattice constant must be positive" );
+        return CMD_ERR_ARG;
+    }
+    const int count = walk_sites( p, d, []( int, double, double, double ) {} );
+    *n = count;
+    if ( !x )
+        return CMD_OK;
+    walk_sites( p, d,
+                [&]( int k, double xs, double ys, double zs )
+                {
+                    x[3 * k + 0] = xs;
+                    x[3 * k + 1] = ys;
+                    x[3 * k + 2] = zs;
+                    if ( type )
+                        type[k] = 0;
+                    if ( id )
+                        id[k] = k + 1;
+                } );
+    if ( !v )
+        return CMD_OK;
+    double total_mass = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    VelocityRng rng;
+    for ( int i = 0; i < count; ++i )
+    {
+        const int t = type ? type[i] : 0;
+        // type_to_temperature holds the 'velocity all' entry under key 1
+        const int seed_base = ( t + 1 == 1 ) ? p->temp_seed : 0;
+        rng.reset( seed_base, &x[3 * i] );
+        const double m = p->mass[t];
+        double vx, vy, vz;
+        if ( p->vel_gaussian )
+        {
+            vx = rng.gaussian();
+            vy = rng.gaussian();
+            vz = rng.gaussian();
+        }
+        else
+        {
+            vx = rng.uniform() - 0.5;
+            vy = rng.uniform() - 0.5;
+            vz = rng.uniform() - 0.5;
+        }
+        v[3 * i + 0] = vx / std::sqrt( m );
+        v[3 * i + 1] = vy / std::sqrt( m );
+        v[3 * i + 2] = vz / std::sqrt( m );
+        total_mass += m;
+        px += m * v[3 * i + 0];
+        py += m * v[3 * i + 1];
+        pz += m * v[3 * i + 2];
+    }
+    const double sx = px / total_mass, sy = py / total_mass,
+                 sz = pz / total_mass;
+    double mv2 = 0.0;
+    for ( int i = 0; i < count; ++i )
+    {
+        v[3 * i + 0] -= sx;
+        v[3 * i + 1] -= sy;
+        v[3 * i + 2] -= sz;
+    }
+    for ( int i = 0; i < count; ++i )
+    {
+        const int t = type ? type[i] : 0;
+        mv2 += ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                 v[3 * i + 2] * v[3 * i + 2] ) *
+               p->mass[t];
+    }
+    const int dof = 3 * count - 3;
+    const double T = mv2 * ( p->mvv2e / ( 1.0 * dof * p->boltz ) );
+    for ( int i = 0; i < count; ++i )
+    {
+        const int t = type ? type[i] : 0;
+        const double target = ( t + 1 == 1 ) ? p->temp_target : 0.0;
+        const double s = std::sqrt( target / T );
+        v[3 * i + 0] *= s;
+        v[3 * i + 1] *= s;
+        v[3 * i + 2] *= s;
+    }
+    return CMD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,285 @@
+// internal.h -- shared declarations of the HIP engine behind cabanamd_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/cabanamd_amd.h"
+
+namespace cmd
+{
+void set_error( const char *fmt, ... );
+} // namespace cmd
+
+#define CMD_HIP( call )                                                      \
+    do                                                                       \
+    {                                                                        \
+        hipError_t e_ = ( call );                                            \
+        if ( e_ != hipSuccess )                                              \
+        {                                                                    \
+            cmd::set_error( "%s failed: %s (%s:%d)", #call,                  \
+                            hipGetErrorString( e_ ), __FILE__, __LINE__ );   \
+            return CMD_ERR_HIP;                                              \
+        }                                                                    \
+    } while ( 0 )
+
+#define CMD_TRY( call )                                                      \
+    do                                                                       \
+    {                                                                        \
+        cmd_status s_ = ( call );                                            \
+        if ( s_ != CMD_OK )                                                  \
+            return s_;                                                       \
+    } while ( 0 )
+
+#define CMD_REQUIRE( cond, code, ... )                                       \
+    do                                                                       \
+    {                                                                        \
+        if ( !( cond ) )                                                     \
+        {                                                                    \
+            cmd::set_error( __VA_ARGS__ );                                   \
+            return code;                                                     \
+        }                                                                    \
+    } while ( 0 )
+
+#define CMD_LAUNCH_CHECK()                                                   \
+    do                                                                       \
+    {                                                                        \
+        hipError_t e_ = hipGetLastError();                                   \
+        if ( e_ != hipSuccess )                                              \
+        {                                                                    \
+            cmd::set_error( "kernel launch failed: %s (%s:%d)",              \
+                            hipGetErrorString( e_ ), __FILE__, __LINE__ );   \
+            return CMD_ERR_HIP;                                              \
+        }                                                                    \
+    } while ( 0 )
+
+namespace cmd
+{
+
+// Owning device buffer.  reserve() never shrinks; keep=true copies the first
+// `used` elements into the new allocation.
+template <class T>
+struct DevBuf
+{
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf( const DevBuf & ) = delete;
+    DevBuf &operator=( const DevBuf & ) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if ( p )
+            (void)hipFree( p );
+        p = nullptr;
+        n = 0;
+    }
+    cmd_status reserve( size_t m, hipStream_t st = nullptr, bool keep = false,
+                        size_t used = 0 )
+    {
+        if ( m <= n )
+            return CMD_OK;
+        T *q = nullptr;
+        hipError_t e = hipMalloc( (void **)&q, ( m ? m : 1 ) * sizeof( T ) );
+        if ( e != hipSuccess )
+        {
+            set_error( "hipMalloc of %zu bytes failed: %s", m * sizeof( T ),
+                       hipGetErrorString( e ) );
+            return CMD_ERR_OOM;
+        }
+        if ( keep && p && used )
+        {
+            e = hipMemcpyAsync( q, p, used * sizeof( T ),
+                                hipMemcpyDeviceToDevice, st );
+            if ( e == hipSuccess )
+                e = hipStreamSynchronize( st );
+            if ( e != hipSuccess )
+            {
+                (void)hipFree( q );
+                set_error( "buffer regrow copy failed: %s",
+                           hipGetErrorString( e ) );
+                return CMD_ERR_HIP;
+            }
+        }
+        release();
+        p = q;
+        n = m;
+        return CMD_OK;
+    }
+    void swap( DevBuf &o )
+    {
+        std::swap( p, o.p );
+        std::swap( n, o.n );
+    }
+};
+
+// Reusable temporary storage for device-wide scans and reductions.
+struct Scratch
+{
+    DevBuf<int64_t> blk;      // per-block partial sums of scans
+    DevBuf<double> dpart;     // per-block partials of FP64 reductions
+    DevBuf<double> dres;      // small result slots
+    DevBuf<int> ires;         // small int result slots
+    std::vector<double> hres; // host staging
+};
+
+// Device-wide exclusive scans (scan.hip).  out has n+1 entries, out[n] is
+// the total.  If total_host is non-null the total is copied back (sync).
+cmd_status scan_exclusive( const int *in, int *out, int n, Scratch &sc,
+                           hipStream_t st, int64_t *total_host );
+cmd_status scan_exclusive( const int *in, int64_t *out, int n, Scratch &sc,
+                           hipStream_t st, int64_t *total_host );
+
+// Fill kernels (graph-safe replacement for hipMemsetAsync).
+cmd_status fill_i32( int *p, int v, size_t n, hipStream_t st );
+cmd_status fill_f64( double *p, double v, size_t n, hipStream_t st );
+
+struct MassTable
+{
+    double m[CMD_MAX_TYPES];
+};
+
+// Linked-cell grid: cell = floor((x - min) * rd) per dimension, with the
+// reference's top-edge clamp and a guard against out-of-range coordinates.
+struct CellGrid
+{
+    double min[3];
+    double rd[3];
+    int n[3];
+    int floor_mode; // 1: floor() (Cabana LinkedCellList); 0: truncation
+};
+
+// Counting sort of n positions into grid cells, stable by index.  Produces
+// perm (sorted slot -> index) and start (ncell+1 cell offsets).
+struct CellSort
+{
+    DevBuf<int> cell_of, count, cursor, start, perm;
+    int ncell = 0;
+};
+cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
+                      Scratch &sc, hipStream_t st );
+
+} // namespace cmd
+
+// The System: device SoA state of one simulated box.
+struct cmd_system
+{
+    int ntypes = 1;
+    double mass[CMD_MAX_TYPES];
+    double dt = 0.005, mvv2e = 1.0, boltz = 1.0;
+    cmd_domain dom{};
+    bool has_domain = false;
+    int N = 0, n_local = 0, n_ghost = 0;
+    int cap = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // positions (x, y, z, type as double); x_alt is the ping-pong partner
+    cmd::DevBuf<double4> x, x_alt;
+    // velocities and forces: three planes of cap doubles each
+    cmd::DevBuf<double> v, v_alt, f, f_alt;
+    cmd::DevBuf<int> id, id_alt;
+
+    // ghosts: per ghost its local root and the bit set of applied shifts
+    // (bit p = phase p: -Lx, +Lx, -Ly, +Ly, -Lz, +Lz); send_idx holds the
+    // reference's per-phase source lists back to back.
+    cmd::DevBuf<int> ghost_root, ghost_code, send_idx;
+    int counts[6] = { 0, 0, 0, 0, 0, 0 };
+    cmd::DevBuf<int> flag, pos;
+
+    cmd::CellSort bins;
+    cmd::Scratch sc;
+
+    cmd::MassTable mass_table() const
+    {
+        cmd::MassTable t;
+        for ( int k = 0; k < CMD_MAX_TYPES; ++k )
+            t.m[k] = mass[k];
+        return t;
+    }
+    cmd_status reserve( int n ); // keeps the first n_local+n_ghost atoms
+};
+
+struct cmd_neigh
+{
+    double neigh_cut = 0.0;
+    int half = 0;
+    int max_neigh_guess = 50;
+    int n_rows = 0;
+    int64_t nnz = 0;
+    int max_row = 0;
+    cmd::DevBuf<int64_t> offsets;
+    cmd::DevBuf<int> idx;
+    cmd::DevBuf<int> counts;
+    cmd::DevBuf<double4> xs; // positions in grid order, w = atom index
+    cmd::CellSort grid_sort;
+    int ng[3] = { 0, 0, 0 };
+    hipStream_t stream = nullptr; // stream of the last build
+};
+
+// LJ parameters passed by value to kernels.
+struct LJParams
+{
+    double lj1[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    double lj2[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    double cutsq[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    float lj1f[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    float lj2f[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    double eshift[CMD_MAX_TYPES * CMD_MAX_TYPES]; // energy at the cutoff
+    int ntypes;
+};
+
+struct cmd_force
+{
+    int kind = 0; // 0 lj
+    int ntypes = 1;
+    double lj1[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    double lj2[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    double cutsq[CMD_MAX_TYPES * CMD_MAX_TYPES];
+    int prec = CMD_PREC_FP64;
+    int lanes = 4;
+    bool tuned = false;
+    cmd::DevBuf<LJParams> dparams; // device copy for multi-type kernels
+};
+
+namespace cmd
+{
+LJParams lj_params( const cmd_force *f );
+
+// Integration constants for the fused force epilogue.
+struct StepCtl
+{
+    double dtf, dtv;      // 0.5*dt/mvv2e and dt
+    double L[3];          // box lengths for the wrap
+    int do_final;         // finish this step's velocities
+    int do_next_initial;  // start the next step (half kick + drift)
+    int wrap_next;        // wrap the drifted positions (next step rebuilds)
+    int thermo;           // accumulate KE, PE and virial partials
+    int write_f;          // store f
+};
+
+// LJ kernels (lj.hip)
+cmd_status lj_full( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
+                    int lanes, const StepCtl &ctl, double4 *x_next,
+                    double *partial, int *nblocks_out );
+cmd_status lj_half( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
+                    int thermo, double *partial, int *nblocks_out );
+cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
+                             const cmd_neigh *N, double *pe, double *w );
+
+// Integrator kernels (core.hip)
+cmd_status nve_final_initial( cmd_system *S, const StepCtl &ctl,
+                              double4 *x_next, double *partial,
+                              int *nblocks_out );
+// Reduce nblocks rows of 4 partials into out[4] (device) deterministically.
+cmd_status reduce_partials( const double *partial, int nblocks, double *out,
+                            hipStream_t st );
+cmd_status ghost_refresh( cmd_system *S );
+cmd_status wrap_local( cmd_system *S );
+cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N );
+} // namespace cmd

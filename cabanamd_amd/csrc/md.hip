@@ -1,0 +1,452 @@
+// md.hip -- the timestep loop: CbnMD::init and CbnMD::run
+// (cabanamd_impl.h:70-255 and :257-432) on one GPU.
+//
+// Step order kept from the reference (cabanamd_impl.h:285-399):
+//   initial_integrate; rebuild (wrap, sort, ghosts, list) every `every`
+//   steps, else ghost refresh; f = 0 + force; ghost force fold for half
+//   lists; final_integrate; thermo every thermo_rate steps.
+// Fusion: for full lists one kernel computes the force of step s, finishes
+// step s (final kick, thermo partials) and starts step s+1 (half kick,
+// drift, and the periodic wrap when s+1 rebuilds), writing positions to the
+// ping-pong buffer.  Every value is produced by the same operations in the
+// same order as the unfused kernels, so both paths give identical bits.
+#include "internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace cmd
+{
+enum
+{
+    PH_FORCE = 0,
+    PH_NEIGH = 1,
+    PH_COMM = 2,
+    PH_INT = 3,
+    PH_OTHER = 4
+};
+} // namespace cmd
+
+struct cmd_md
+{
+    cmd_params p{};
+    cmd_domain d{};
+    cmd_system *S = nullptr;
+    cmd_neigh *N = nullptr;
+    cmd_force *F = nullptr;
+    double neigh_cut = 0.0;
+    int step = 0;
+    bool inited = false;
+    int prec = CMD_PREC_FP64;
+    int lanes = 0;
+    std::vector<double> thermo; // rows of 5: step, T, PE/N, (PE+KE)/N, W
+    cmd::DevBuf<double> partA, partB, trows;
+    double timers[6] = { 0, 0, 0, 0, 0, 0 };
+    bool profile = false;
+    int prof_launches = 0;
+    double prof_ms = 0.0;
+    struct Ev
+    {
+        hipEvent_t a, b;
+        int phase;
+    };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> pool;
+
+    ~cmd_md()
+    {
+        for ( auto &e : events )
+        {
+            pool.push_back( e.a );
+            pool.push_back( e.b );
+        }
+        for ( auto e : pool )
+            (void)hipEventDestroy( e );
+        cmd_force_destroy( F );
+        cmd_neigh_destroy( N );
+        cmd_system_destroy( S );
+    }
+    hipEvent_t get_event()
+    {
+        if ( !pool.empty() )
+        {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate( &e );
+        return e;
+    }
+    // Phase timing by HIP events on the system's stream (profile mode).
+    int begin( int phase )
+    {
+        if ( !profile )
+            return -1;
+        Ev e{ get_event(), get_event(), phase };
+        (void)hipEventRecord( e.a, S->stream );
+        events.push_back( e );
+        return (int)events.size() - 1;
+    }
+    void end( int k )
+    {
+        if ( k >= 0 )
+            (void)hipEventRecord( events[k].b, S->stream );
+    }
+    void collect()
+    {
+        for ( auto &e : events )
+        {
+            float ms = 0.0f;
+            (void)hipEventSynchronize( e.b );
+            (void)hipEventElapsedTime( &ms, e.a, e.b );
+            timers[e.phase] += ms * 1e-3;
+            if ( e.phase == cmd::PH_FORCE )
+            {
+                prof_launches++;
+                prof_ms += ms;
+            }
+            pool.push_back( e.a );
+            pool.push_back( e.b );
+        }
+        events.clear();
+    }
+};
+
+namespace cmd
+{
+
+static cmd_status rebuild( cmd_md *md )
+{
+    cmd_system *S = md->S;
+    int k = md->begin( PH_OTHER );
+    CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut,
+                                 1, 1, 0, 1, nullptr ) );
+    md->end( k );
+    k = md->begin( PH_COMM );
+    CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
+    md->end( k );
+    k = md->begin( PH_NEIGH );
+    CMD_TRY( cmd_neigh_build( md->N, S ) );
+    md->end( k );
+    // ghost force rows are never written by full-list kernels: keep them 0
+    if ( !md->N->half )
+        for ( int c = 0; c < 3; ++c )
+            CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap + S->n_local, 0.0,
+                               S->n_ghost, S->stream ) );
+    return CMD_OK;
+}
+
+static void thermo_row( cmd_md *md, int step, double mv2, double pe, double w )
+{
+    cmd_system *S = md->S;
+    const int dof = 3 * S->N - 3;
+    const double T = mv2 * ( S->mvv2e / ( 1.0 * dof * S->boltz ) );
+    const double KE = mv2 * ( 0.5 * S->mvv2e );
+    md->thermo.push_back( step );
+    md->thermo.push_back( T );
+    md->thermo.push_back( pe / S->N );
+    md->thermo.push_back( ( pe + KE ) / S->N );
+    md->thermo.push_back( w );
+}
+
+static cmd_status run_steps( cmd_md *md, int nsteps )
+{
+    cmd_system *S = md->S;
+    cmd_neigh *N = md->N;
+    cmd_force *F = md->F;
+    const cmd_params &p = md->p;
+    const hipStream_t st = S->stream;
+    const bool half = N->half != 0;
+    const int s0 = md->step + 1, s1 = md->step + nsteps;
+    const int nthermo = p.thermo_rate > 0 ? nsteps / p.thermo_rate + 2 : 0;
+    CMD_TRY( md->trows.reserve( 8 * (size_t)std::max( nthermo, 1 ) ) );
+    std::vector<int> thermo_steps;
+    StepCtl ctl{};
+    ctl.dtf = 0.5 * S->dt / S->mvv2e;
+    ctl.dtv = S->dt;
+    for ( int k = 0; k < 3; ++k )
+        ctl.L[k] = S->dom.L[k];
+
+    auto t0 = std::chrono::steady_clock::now();
+    // initial_integrate of the first step of this call (in place)
+    {
+        int k = md->begin( PH_INT );
+        CMD_TRY( cmd_nve_initial( S ) );
+        if ( s0 % p.every == 0 )
+            CMD_TRY( wrap_local( S ) );
+        md->end( k );
+    }
+    for ( int s = s0; s <= s1; ++s )
+    {
+        const bool rb = ( s % p.every == 0 );
+        if ( rb )
+        {
+            S->n_ghost = 0;
+            CMD_TRY( rebuild( md ) );
+        }
+        else
+        {
+            int k = md->begin( PH_COMM );
+            CMD_TRY( ghost_refresh( S ) );
+            md->end( k );
+        }
+        const bool th = p.thermo_rate > 0 && ( s % p.thermo_rate == 0 );
+        const bool last = ( s == s1 );
+        ctl.do_final = 1;
+        ctl.do_next_initial = last ? 0 : 1;
+        ctl.wrap_next = ( !last && ( ( s + 1 ) % p.every == 0 ) ) ? 1 : 0;
+        ctl.thermo = th ? 1 : 0;
+        ctl.write_f = 1;
+        const int n = S->n_local;
+        const int G = F->lanes ? F->lanes : 4;
+        const int nbA = std::max( 1, ( n + 255 ) / 256 * G );
+        CMD_TRY( md->partA.reserve( 4 * (size_t)nbA + 4 ) );
+        CMD_TRY( md->partB.reserve( 4 * (size_t)( ( n + 255 ) / 256 + 1 ) ) );
+        CMD_TRY( S->x_alt.reserve( S->cap ) );
+        int nb_force = 0, nb_int = 0;
+        if ( !half )
+        {
+            int k = md->begin( PH_FORCE );
+            CMD_TRY( lj_full( F, S, N, G, ctl, S->x_alt.p, md->partA.p,
+                              &nb_force ) );
+            md->end( k );
+        }
+        else
+        {
+            int k = md->begin( PH_FORCE );
+            const int nt = S->n_local + S->n_ghost;
+            for ( int c = 0; c < 3; ++c )
+                CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap, 0.0, nt, st ) );
+            CMD_TRY( lj_half( F, S, N, ctl.thermo, md->partA.p, &nb_force ) );
+            md->end( k );
+            k = md->begin( PH_COMM );
+            CMD_TRY( cmd_comm_update_force( S ) );
+            md->end( k );
+            k = md->begin( PH_INT );
+            CMD_TRY( nve_final_initial( S, ctl, S->x_alt.p, md->partB.p,
+                                        &nb_int ) );
+            md->end( k );
+        }
+        if ( th )
+        {
+            const int r = (int)thermo_steps.size();
+            CMD_TRY( reduce_partials( md->partA.p, nb_force,
+                                      md->trows.p + 8 * r, st ) );
+            if ( half )
+                CMD_TRY( reduce_partials( md->partB.p, nb_int,
+                                          md->trows.p + 8 * r + 4, st ) );
+            else
+                CMD_TRY( fill_f64( md->trows.p + 8 * r + 4, 0.0, 4, st ) );
+            thermo_steps.push_back( s );
+        }
+        if ( !last )
+            S->x.swap( S->x_alt );
+    }
+    CMD_HIP( hipStreamSynchronize( st ) );
+    md->timers[5] +=
+        std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 )
+            .count();
+    md->collect();
+    md->step = s1;
+    if ( !thermo_steps.empty() )
+    {
+        std::vector<double> h( 8 * thermo_steps.size() );
+        CMD_HIP( hipMemcpy( h.data(), md->trows.p, sizeof( double ) * h.size(),
+                            hipMemcpyDeviceToHost ) );
+        for ( size_t r = 0; r < thermo_steps.size(); ++r )
+        {
+            const double *row = &h[8 * r];
+            // full: [mv2, pe, w, 0 | 0...]; half: [0, pe, w, 0 | mv2, 0, 0, 0]
+            thermo_row( md, thermo_steps[r], row[0] + row[4], row[1], row[2] );
+        }
+    }
+    return CMD_OK;
+}
+
+} // namespace cmd
+
+using namespace cmd;
+
+extern "C" {
+
+cmd_status cmd_md_create( const cmd_params *p, cmd_md **out )
+{
+    CMD_REQUIRE( p && out, CMD_ERR_ARG, "NULL argument" );
+    CMD_REQUIRE( p->force_type == 0, CMD_ERR_ARG,
+                 "cmd_md drives pair_style lj/cut; use the NNP entry points "
+                 "for pair_style nnp" );
+    CMD_REQUIRE( p->every > 0, CMD_ERR_ARG, "neigh_modify every must be > 0" );
+    CMD_REQUIRE( p->ntypes >= 1 && p->ntypes <= CMD_MAX_TYPES, CMD_ERR_ARG,
+                 "ntypes %d outside [1,%d]", p->ntypes, CMD_MAX_TYPES );
+    cmd_md *md = new cmd_md;
+    md->p = *p;
+    *out = md;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_destroy( cmd_md *md )
+{
+    delete md;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_set_precision( cmd_md *md, int prec )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( prec == CMD_PREC_FP64 || prec == CMD_PREC_MIXED, CMD_ERR_ARG,
+                 "unknown precision %d", prec );
+    md->prec = prec;
+    if ( md->F )
+        CMD_TRY( cmd_force_set_precision( md->F, prec ) );
+    return CMD_OK;
+}
+
+cmd_status cmd_md_set_lanes( cmd_md *md, int lanes )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    md->lanes = lanes;
+    if ( md->F )
+        CMD_TRY( cmd_force_set_lanes( md->F, lanes ) );
+    return CMD_OK;
+}
+
+cmd_status cmd_md_init( cmd_md *md )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "init called twice" );
+    const cmd_params &p = md->p;
+    md->neigh_cut = p.force_cutoff + p.skin;
+    double low[3], high[3];
+    for ( int k = 0; k < 3; ++k )
+    {
+        low[k] = p.lattice_constant * p.region_lo[k];
+        high[k] = p.lattice_constant * p.region_hi[k];
+    }
+    CMD_TRY( cmd_domain_create( low, high, p.comm_ghost_cutoff, &md->d ) );
+    int n = 0;
+    CMD_TRY( cmd_lattice_create( &p, &md->d, &n, nullptr, nullptr, nullptr,
+                                 nullptr ) );
+    CMD_REQUIRE( n > 0, CMD_ERR_ARG, "the lattice creates no atoms" );
+    std::vector<double> x( 3 * (size_t)n ), v( 3 * (size_t)n );
+    std::vector<int> type( n ), id( n );
+    CMD_TRY( cmd_lattice_create( &p, &md->d, &n, x.data(), v.data(),
+                                 type.data(), id.data() ) );
+    CMD_TRY( cmd_system_create( p.ntypes, &md->S ) );
+    cmd_system *S = md->S;
+    CMD_TRY( cmd_system_set_units( S, p.dt, p.mvv2e, p.boltz ) );
+    CMD_TRY( cmd_system_set_mass( S, p.ntypes, p.mass ) );
+    CMD_TRY( cmd_system_set_domain( S, &md->d ) );
+    CMD_TRY( cmd_system_reserve( S, n + n / 2 + 1024 ) );
+    CMD_TRY( cmd_system_upload( S, n, 0, x.data(), v.data(), nullptr,
+                                type.data(), id.data() ) );
+    CMD_TRY( cmd_force_lj_create( p.ntypes, &md->F ) );
+    for ( int c = 0; c < p.ncoeff; ++c )
+        CMD_TRY( cmd_force_lj_init_coeff( md->F, p.coeff_i[c], p.coeff_j[c],
+                                          p.coeff_eps[c], p.coeff_sigma[c],
+                                          p.coeff_cut[c] ) );
+    CMD_TRY( cmd_force_set_precision( md->F, md->prec ) );
+    CMD_TRY( cmd_force_set_lanes( md->F, md->lanes ) );
+    CMD_TRY( cmd_neigh_create( md->neigh_cut, p.half, p.max_neigh_guess,
+                               &md->N ) );
+    // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
+    CMD_TRY( cmd_comm_exchange( S ) );
+    CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
+                                 md->neigh_cut, 1, 1, 0, 1, nullptr ) );
+    CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
+    CMD_TRY( cmd_neigh_build( md->N, S ) );
+    CMD_TRY( force_compute_plain( md->F, S, md->N ) );
+    if ( p.half )
+        CMD_TRY( cmd_comm_update_force( S ) );
+    if ( p.thermo_rate > 0 )
+    {
+        double T = 0.0, pe = 0.0, w = 0.0, ke = 0.0;
+        CMD_TRY( cmd_thermo_temperature( S, &T ) );
+        CMD_TRY( lj_energy_virial( md->F, S, md->N, &pe, &w ) );
+        CMD_TRY( cmd_thermo_kinetic( S, &ke ) );
+        const double mv2 = ke / ( 0.5 * S->mvv2e );
+        thermo_row( md, 0, mv2, pe, w );
+    }
+    CMD_HIP( hipStreamSynchronize( S->stream ) );
+    md->inited = true;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_run( cmd_md *md, int nsteps )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( md->inited, CMD_ERR_STATE, "run before init" );
+    if ( nsteps <= 0 )
+        return CMD_OK;
+    return run_steps( md, nsteps );
+}
+
+cmd_status cmd_md_system( cmd_md *md, cmd_system **s )
+{
+    CMD_REQUIRE( md && s, CMD_ERR_ARG, "NULL argument" );
+    *s = md->S;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_neigh( cmd_md *md, cmd_neigh **n )
+{
+    CMD_REQUIRE( md && n, CMD_ERR_ARG, "NULL argument" );
+    *n = md->N;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_force( cmd_md *md, cmd_force **f )
+{
+    CMD_REQUIRE( md && f, CMD_ERR_ARG, "NULL argument" );
+    *f = md->F;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_thermo( const cmd_md *md, int max_rows, double *rows,
+                          int *rows_out )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    const int n = (int)( md->thermo.size() / 5 );
+    if ( rows_out )
+        *rows_out = n;
+    if ( rows )
+    {
+        const int m = std::min( n, max_rows );
+        std::memcpy( rows, md->thermo.data(), sizeof( double ) * 5 * m );
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_md_timers( const cmd_md *md, double out[6] )
+{
+    CMD_REQUIRE( md && out, CMD_ERR_ARG, "NULL argument" );
+    for ( int k = 0; k < 6; ++k )
+        out[k] = md->timers[k];
+    return CMD_OK;
+}
+
+cmd_status cmd_md_profile( cmd_md *md, int enable )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    md->profile = enable != 0;
+    md->prof_launches = 0;
+    md->prof_ms = 0.0;
+    for ( int k = 0; k < 6; ++k )
+        md->timers[k] = 0.0;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
+                                double *mean_ms )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    if ( launches )
+        *launches = md->prof_launches;
+    if ( mean_ms )
+        *mean_ms = md->prof_launches ? md->prof_ms / md->prof_launches : 0.0;
+    return CMD_OK;
+}
+
+} // extern "C"

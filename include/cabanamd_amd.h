@@ -1,0 +1,256 @@
+/*
+ * cabanamd_amd.h -- C ABI of the MI355X-native CabanaMD timestep engine.
+ *
+ * One shared library (libcabanamd_amd.so) exports these entry points.  They
+ * replace the reference's C++ module classes on the timestep path; each block
+ * below names the reference interface it stands in for (paths relative to the
+ * CabanaMD source tree).  The C++ classes in cabanamd_amd.hpp wrap this ABI
+ * with the reference's own class and method names.
+ *
+ * Conventions
+ *  - Opaque handles own device-resident state (HBM).  Host arrays crossing
+ *    the ABI are plain pointers: positions, velocities and forces are
+ *    row-major (n,3) doubles, types are 0-based ints, ids are ints.
+ *  - Every call returns cmd_status; CMD_OK is 0.  On error
+ *    cmd_last_error() returns a message (thread-local).  Nothing throws
+ *    across the ABI.  The reference reports errors by throwing
+ *    std::runtime_error from log_err (output.h:37-54).
+ *  - Calls are ordered on the handle's HIP stream (cmd_system_set_stream);
+ *    calls that return host data synchronise that stream.
+ *  - One host thread per handle.
+ */
+#ifndef CABANAMD_AMD_H
+#define CABANAMD_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cmd_status
+{
+    CMD_OK = 0,
+    CMD_ERR_ARG = 1,      /* bad argument */
+    CMD_ERR_HIP = 2,      /* HIP runtime error */
+    CMD_ERR_OOM = 3,      /* device allocation failed */
+    CMD_ERR_STATE = 4,    /* call out of order */
+    CMD_ERR_CAPACITY = 5, /* a fixed capacity was exceeded */
+    CMD_ERR_IO = 6,       /* file or deck error */
+    CMD_ERR_NO_DEVICE = 7 /* no GPU visible */
+} cmd_status;
+
+enum
+{
+    CMD_MAX_TYPES = 8
+};
+
+/* Force-kernel precision.  FP64: pair maths in double (the reference's).
+ * MIXED: separations and cutoff tests in double, pair maths in float,
+ * per-atom accumulation in double. */
+enum
+{
+    CMD_PREC_FP64 = 0,
+    CMD_PREC_MIXED = 1
+};
+
+const char *cmd_last_error( void );
+int cmd_version( void );
+
+/* ---- device --------------------------------------------------------------
+ * Replaces Kokkos::initialize / ScopeGuard (bin/main.cpp:64). */
+cmd_status cmd_device_init( int device );
+cmd_status cmd_device_count( int *count );
+cmd_status cmd_device_synchronize( void );
+
+/* ---- domain --------------------------------------------------------------
+ * System::create_domain (system.h:149-205): a periodic box of 100 mesh
+ * cells per dimension, one rank, halo = ceil(ghost_cutoff / cell) cells. */
+typedef struct cmd_domain
+{
+    double low[3], high[3];
+    double L[3]; /* global_mesh_{x,y,z} */
+    double cell[3];
+    double local_lo[3], local_hi[3], local_mesh[3];
+    double ghost_lo[3], ghost_hi[3];
+    int halo_width;
+} cmd_domain;
+
+cmd_status cmd_domain_create( const double low[3], const double high[3],
+                              double ghost_cutoff, cmd_domain *out );
+
+/* ---- System --------------------------------------------------------------
+ * System<Device,layout> (system.h:65-279, system_types/system_1aosoa.h):
+ * N, N_local, N_ghost, ntypes, mass, dt, mvv2e, boltz, mesh bounds and the
+ * x/v/f/type/id slices, held here as device SoA. */
+typedef struct cmd_system cmd_system;
+
+cmd_status cmd_system_create( int ntypes, cmd_system **out );
+cmd_status cmd_system_destroy( cmd_system *s );
+cmd_status cmd_system_set_stream( cmd_system *s, void *hip_stream );
+cmd_status cmd_system_set_units( cmd_system *s, double dt, double mvv2e,
+                                 double boltz );
+cmd_status cmd_system_set_mass( cmd_system *s, int ntypes,
+                                const double *mass );
+cmd_status cmd_system_set_domain( cmd_system *s, const cmd_domain *d );
+/* Grow device capacity to at least n atoms (System::resize, system_1aosoa.h:63-70). */
+cmd_status cmd_system_reserve( cmd_system *s, int n );
+/* Upload n_local owned atoms followed by n_ghost ghosts.  v, f, id may be
+ * NULL (zero-filled; id defaults to index+1).  N (global) = n_local. */
+cmd_status cmd_system_upload( cmd_system *s, int n_local, int n_ghost,
+                              const double *x, const double *v,
+                              const double *f, const int *type,
+                              const int *id );
+/* Download the first n = n_local + n_ghost atoms (any pointer may be NULL). */
+cmd_status cmd_system_download( cmd_system *s, double *x, double *v,
+                                double *f, int *type, int *id );
+cmd_status cmd_system_counts( const cmd_system *s, int *n_global,
+                              int *n_local, int *n_ghost );
+/* Per-atom device storage in bytes (for admission by HBM footprint). */
+cmd_status cmd_system_footprint( const cmd_system *s, int64_t *bytes );
+
+/* ---- Integrator -----------------------------------------------------------
+ * Integrator<System> (integrator_nve.h:56-110; integrator_nve_impl.h:49-83):
+ * velocity-Verlet over [0, N_local). */
+cmd_status cmd_nve_initial( cmd_system *s );
+cmd_status cmd_nve_final( cmd_system *s );
+
+/* ---- Comm (one rank) -----------------------------------------------------
+ * Comm<System> (comm_mpi_impl.h:191-441): exchange = periodic wrap (and
+ * drop ghosts), exchange_halo = 6-phase ghost build at depth comm_depth,
+ * update_halo = ghost position refresh, update_force = ghost force fold. */
+cmd_status cmd_comm_exchange( cmd_system *s );
+cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth );
+cmd_status cmd_comm_update_halo( cmd_system *s );
+cmd_status cmd_comm_update_force( cmd_system *s );
+/* Ghost bookkeeping in the reference's terms: the concatenated per-phase
+ * source lists (pack_indicies) and per-phase counts. send_idx may be NULL. */
+cmd_status cmd_comm_ghost_lists( cmd_system *s, int *send_idx, int counts[6] );
+
+/* ---- Binning --------------------------------------------------------------
+ * Binning<System>::create_binning (binning_cabana_impl.h:57-113): linked-cell
+ * counting sort of the local atoms and permutation of every per-atom array
+ * into cell order.  perm_out (optional, host, N_local ints) receives the
+ * permutation: new slot k holds old atom perm_out[k]. */
+cmd_status cmd_binning_create( cmd_system *s, double dx, double dy,
+                               double dz, int halo_depth, int do_local,
+                               int do_ghost, int sort, int *perm_out );
+
+/* ---- Neighbor --------------------------------------------------------------
+ * NeighborVerlet<System,Full|Half,...> (neighbor_verlet.h:19-70): CSR Verlet
+ * list over rows [0,N_local), candidates [0,N_local+N_ghost), radius
+ * neigh_cut.  max_neigh_guess keeps the reference's regrow-by-1.1 semantics
+ * for the reported capacity. */
+typedef struct cmd_neigh cmd_neigh;
+
+cmd_status cmd_neigh_create( double neigh_cut, int half, int max_neigh_guess,
+                             cmd_neigh **out );
+cmd_status cmd_neigh_destroy( cmd_neigh *n );
+cmd_status cmd_neigh_build( cmd_neigh *n, cmd_system *s );
+cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
+                           int *max_row, int *max_neigh_guess );
+/* offsets: n_rows+1 int64; idx: nnz ints (either may be NULL). */
+cmd_status cmd_neigh_download( cmd_neigh *n, int64_t *offsets, int *idx );
+
+/* ---- Force -----------------------------------------------------------------
+ * Force<System,Neighbor> (force.h:57-74) with ForceLJ
+ * (force_types/force_lj_cabana_neigh_impl.h:49-377). */
+typedef struct cmd_force cmd_force;
+
+cmd_status cmd_force_lj_create( int ntypes, cmd_force **out );
+cmd_status cmd_force_destroy( cmd_force *f );
+/* pair_coeff i j eps sigma cut, 1-based types (impl.h:62-89). */
+cmd_status cmd_force_lj_init_coeff( cmd_force *f, int i, int j, double eps,
+                                    double sigma, double cut );
+cmd_status cmd_force_set_precision( cmd_force *f, int prec );
+/* Lanes per atom in the full-list kernel (1,2,4,8,16); 0 = autotune. */
+cmd_status cmd_force_set_lanes( cmd_force *f, int lanes );
+cmd_status cmd_force_get_lanes( const cmd_force *f, int *lanes );
+/* f = 0 over local+ghost, then accumulate (cabanamd_impl.h:336-341). */
+cmd_status cmd_force_compute( cmd_force *f, cmd_system *s, cmd_neigh *n );
+/* Shifted pair energy with the reference's list weights (impl.h:261-377). */
+cmd_status cmd_force_energy( cmd_force *f, cmd_system *s, cmd_neigh *n,
+                             double *pe );
+/* Pair virial sum r.F (not in the reference; used for pressure). */
+cmd_status cmd_force_virial( cmd_force *f, cmd_system *s, cmd_neigh *n,
+                             double *w );
+
+/* ---- thermo ----------------------------------------------------------------
+ * Temperature / KinE (property_temperature_impl.h:55-77,
+ * property_kine_impl.h:55-76). */
+cmd_status cmd_thermo_temperature( cmd_system *s, double *T );
+cmd_status cmd_thermo_kinetic( cmd_system *s, double *ke );
+
+/* ---- run parameters and input deck ----------------------------------------
+ * InputFile<System> (inputFile.h:150-275; inputFile_impl.h:125-534). */
+typedef struct cmd_params
+{
+    int units;               /* 0 real, 1 lj, 2 metal (types.h:52-57) */
+    int lattice_style;       /* 0 sc, 1 fcc */
+    double lattice_constant; /* converted: fcc + lj gives (4/rho)^(1/3) */
+    double lattice_offset[3];
+    double region_lo[3], region_hi[3]; /* lattice units */
+    int ntypes;
+    double mass[CMD_MAX_TYPES];
+    double temp_target;
+    int temp_seed;
+    int vel_gaussian; /* 0 loop-geom uniform (reference), 1 gaussian */
+    double force_cutoff, skin;
+    int every;           /* neigh_modify every */
+    int max_neigh_guess; /* neigh_modify one */
+    double comm_ghost_cutoff;
+    int half;            /* newton on */
+    int nsteps, thermo_rate;
+    double dt, mvv2e, boltz;
+    int ncoeff;
+    int coeff_i[16], coeff_j[16];
+    double coeff_eps[16], coeff_sigma[16], coeff_cut[16];
+    int force_type; /* 0 lj/cut, 2 nnp (types.h:80-85) */
+    char nnp_dir[256];
+} cmd_params;
+
+/* Parse a LAMMPS-subset deck (inputFile_impl.h:141-533).  Unknown or
+ * unsupported keywords return CMD_ERR_IO with the reference's message. */
+cmd_status cmd_params_default( cmd_params *p );
+cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
+                                  cmd_params *p );
+
+/* Initial state on the host (inputFile_impl.h:536-865): lattice sites,
+ * ids, types and velocities.  Two-call protocol: x == NULL returns the count. */
+cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d,
+                               int *n, double *x, double *v, int *type,
+                               int *id );
+
+/* ---- timestep loop ---------------------------------------------------------
+ * CbnMD<System,Neighbor>::init / run (cabanamd_impl.h:70-432).  run() fuses
+ * the steps on the device: force + final integrate + next initial integrate
+ * (with wrap before a rebuild) in one kernel, then ghost refresh or the
+ * rebuild sequence. */
+typedef struct cmd_md cmd_md;
+
+cmd_status cmd_md_create( const cmd_params *p, cmd_md **out );
+cmd_status cmd_md_destroy( cmd_md *md );
+cmd_status cmd_md_set_precision( cmd_md *md, int prec );
+cmd_status cmd_md_set_lanes( cmd_md *md, int lanes );
+cmd_status cmd_md_init( cmd_md *md );
+cmd_status cmd_md_run( cmd_md *md, int nsteps );
+cmd_status cmd_md_system( cmd_md *md, cmd_system **s );
+cmd_status cmd_md_neigh( cmd_md *md, cmd_neigh **n );
+cmd_status cmd_md_force( cmd_md *md, cmd_force **f );
+/* Thermo rows (step, T, PE/N, (PE+KE)/N, virial W); returns row count in
+ * *rows_out and copies min(max_rows, count) rows (5 doubles each). */
+cmd_status cmd_md_thermo( const cmd_md *md, int max_rows, double *rows,
+                          int *rows_out );
+/* Phase times in seconds (CbnMD::run timers, cabanamd_impl.h:273-282):
+ * force, neigh, comm, integrate, other, total. */
+cmd_status cmd_md_timers( const cmd_md *md, double out[6] );
+/* Per-launch HIP-event timing of the fused force kernel on the system's
+ * stream: enable before run(); returns launches and mean ms. */
+cmd_status cmd_md_profile( cmd_md *md, int enable );
+cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
+                                double *mean_ms );
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,323 @@
+"""GPU parity of the LJ timestep path against the FP64 oracle.
+
+Every case runs the HIP engine through the C ABI and the oracle on the same
+seeded inputs.  Integer/index results (permutations, ghost lists, neighbour
+lists) must match exactly; FP64 results must match exactly where the
+operation order is the same (integrator, ghost positions, G=1 forces) and
+to the stated relative tolerance otherwise.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _liquid_state(ncells, steps=30, seed=87287, half=False):
+    """An fcc lattice melted for a few oracle steps (positions off-lattice)."""
+    md = O.MD(O.lj_params(ncells=ncells, seed=seed, half=half, thermo=0))
+    md.init()
+    md.run(steps)
+    x, v, f, t, i = md.arrays()
+    n = md.n_local
+    return md, x[:n].copy(), v[:n].copy(), t[:n].copy(), i[:n].copy()
+
+
+def _domain_for(ncells, rho=0.8442):
+    a = (4.0 / rho) ** (1.0 / 3.0)
+    hi = [a * c for c in ncells]
+    p = O.lj_params(ncells=ncells)
+    return hi, p.comm_ghost_cutoff
+
+
+def test_integrator_bit_exact(gpu):
+    M = gpu
+    rng = np.random.default_rng(342343901)
+    n, ng = 1000, 200
+    lo, hi = -5.3 * 2.32, 4.7 * 2.32
+    x = rng.uniform(lo, hi, (n, 3))
+    v = rng.uniform(-1, 1, (n, 3))
+    f = rng.uniform(-1, 1, (n, 3))
+    s = M.System(1)
+    s.set_units(0.005, 1.0, 1.0)
+    s.set_mass([1.0])
+    s.upload(x, v, f, n_ghost=ng)
+    integ = M.Integrator(s)
+    integ.initial_integrate()
+    integ.final_integrate()
+    xg, vg, _, _, _ = s.download()
+    xo, vo = x.copy(), v.copy()
+    t = np.zeros(n, dtype=np.int32)
+    O.nve_initial(xo[:n - ng], vo[:n - ng], f[:n - ng], t, [1.0], 0.005)
+    O.nve_final(vo[:n - ng], f[:n - ng], t, [1.0], 0.005)
+    assert np.array_equal(xg, xo)
+    assert np.array_equal(vg, vo)
+
+
+def test_integrator_reversibility(gpu):
+    """tstIntegrator.hpp:83-137: 100 steps, negate v, 100 steps."""
+    M = gpu
+    rng = np.random.default_rng(342343901)
+    n, ng = 1000, 200
+    x = rng.uniform(-5.3 * 2.32, 4.7 * 2.32, (n, 3))
+    v = rng.uniform(-1, 1, (n, 3))
+    f = rng.uniform(-1, 1, (n, 3))
+    s = M.System(1)
+    s.set_units(0.005, 1.0, 1.0)
+    s.upload(x, v, f, n_ghost=ng)
+    integ = M.Integrator(s)
+    for _ in range(100):
+        integ.initial_integrate()
+        integ.final_integrate()
+    xs, vs, fs, _, _ = s.download()
+    s.upload(xs, -vs, fs, n_ghost=ng)
+    for _ in range(100):
+        integ.initial_integrate()
+        integ.final_integrate()
+    xf = s.download()[0]
+    # EXPECT_FLOAT_EQ: within 4 ULPs of float
+    tol = 4 * np.finfo(np.float32).eps * np.maximum(np.abs(x), 1.0)
+    assert np.all(np.abs(xf - x) <= tol)
+
+
+@pytest.mark.parametrize("ncells", [(6, 6, 6), (10, 9, 8)])
+def test_binning_permutation_exact(gpu, ncells):
+    M = gpu
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    rng = np.random.default_rng(7)
+    order = rng.permutation(len(x))  # scramble so the sort has work to do
+    x, v, t, i = x[order], v[order], t[order], i[order]
+    d = O.create_domain([0, 0, 0], hi, gc)
+    O.wrap(x, d.L)
+    perm_o, cell_o, _ = O.bin_sort(x, d, [2.8] * 3, 1)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    perm = M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
+    assert np.array_equal(perm, perm_o)
+    xs, vs, _, _, ids = s.download()
+    assert np.array_equal(xs, x[perm_o])
+    assert np.array_equal(vs, v[perm_o])
+    assert np.array_equal(ids, i[perm_o])
+    assert np.all(np.diff(cell_o) >= 0)
+
+
+@pytest.mark.parametrize("ncells", [(5, 5, 5), (8, 7, 6)])
+def test_ghosts_exact(gpu, ncells):
+    M = gpu
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    d = O.create_domain([0, 0, 0], hi, gc)
+    O.wrap(x, d.L)
+    xo, to, io, send_o, counts_o = O.ghost_build(x, t, i, d, 2.8)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    comm = M.Comm(s, 2.8)
+    comm.exchange()
+    comm.exchange_halo()
+    send, counts = comm.ghost_lists()
+    assert np.array_equal(counts, counts_o)
+    assert np.array_equal(send, send_o)
+    xg, _, _, tg, ig = s.download()
+    assert np.array_equal(xg, xo)
+    assert np.array_equal(ig, io)
+    # refresh after a drift of the local atoms
+    s.set_units(0.005)
+    integ = M.Integrator(s)
+    integ.initial_integrate()
+    comm.update_halo()
+    xg2 = s.download()[0]
+    xl = xo[:len(x)].copy()
+    vl = v.copy()
+    O.nve_initial(xl, vl, np.zeros_like(xl), t, [1.0], 0.005)
+    xo2 = xo.copy()
+    xo2[:len(x)] = xl
+    O.ghost_refresh(xo2, len(x), d, send_o, counts_o)
+    assert np.array_equal(xg2, xo2)
+    # the ghost force fold is checked with real half-list forces in
+    # test_lj_half_fp64
+
+
+def _tstneighbor_positions(seed=342343901):
+    rc = 2.32
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-5.3 * rc, 4.7 * rc, (1000, 3))
+    return x, rc
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_neighbor_tstneighbor_geometry(gpu, half):
+    """tstNeighbor.hpp:257-320: 1000 atoms, last 200 ghosts, rc 2.32."""
+    M = gpu
+    x, rc = _tstneighbor_positions()
+    s = M.System(1)
+    s.upload(x, n_ghost=200)
+    nl = M.NeighborVerlet(rc, half, 100)
+    nl.create(s)
+    off, idx = nl.get()
+    off_o, idx_o = O.neigh_build(x, 800, rc, half)
+    assert np.array_equal(off, off_o)
+    assert np.array_equal(idx, idx_o)
+    off_b, idx_b = O.neigh_brute(x, 800, rc, half)
+    for r in range(800):
+        a = np.sort(idx[off[r]:off[r + 1]])
+        b = np.sort(idx_b[off_b[r]:off_b[r + 1]])
+        assert np.array_equal(a, b)
+    if half:
+        offf, idxf = O.neigh_brute(x, 800, rc, False)
+        assert len(idx) <= len(idxf)
+        # no pair stored in both directions among local rows
+        pairs = set()
+        for r in range(800):
+            for j in idx[off[r]:off[r + 1]]:
+                assert (int(j), r) not in pairs
+                pairs.add((r, int(j)))
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_neighbor_lattice_exact(gpu, half):
+    M = gpu
+    ncells = (9, 8, 7)
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    d = O.create_domain([0, 0, 0], hi, gc)
+    O.wrap(x, d.L)
+    xo, _, _, _, _ = O.ghost_build(x, t, i, d, 2.8)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    comm = M.Comm(s, 2.8)
+    comm.exchange_halo()
+    nl = M.NeighborVerlet(2.8, half, 50)
+    nl.create(s)
+    off, idx = nl.get()
+    off_o, idx_o = O.neigh_build(xo, len(x), 2.8, half)
+    assert np.array_equal(off, off_o)
+    assert np.array_equal(idx, idx_o)
+    assert nl.max_neigh_guess >= int(np.diff(off).max())
+
+
+def _force_case(M, ncells, half, prec, lanes):
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    d = O.create_domain([0, 0, 0], hi, gc)
+    O.wrap(x, d.L)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    comm = M.Comm(s, 2.8)
+    comm.exchange_halo()
+    nl = M.NeighborVerlet(2.8, half, 50)
+    nl.create(s)
+    F = M.ForceLJ(s, precision=prec, lanes=lanes)
+    F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
+    F.compute(s, nl)
+    if half:
+        comm.update_force()
+    xg, _, fg, tg, _ = s.download()
+    off, idx = nl.get()
+    lj1, lj2, cs = O.lj_coeff(1, [(1, 1, 1.0, 1.0, 2.5)])
+    fo = O.lj_force(xg, tg, len(x), 1, lj1, lj2, cs, off, idx, half)
+    send, counts = comm.ghost_lists()
+    if half:
+        O.ghost_fold(fo, len(x), send, counts)
+    eo = O.lj_energy(xg, tg, len(x), 1, lj1, lj2, cs, off, idx, half)
+    wo = O.lj_virial(xg, tg, len(x), 1, lj1, lj2, cs, off, idx, half)
+    e = F.compute_energy(s, nl)
+    w = F.compute_virial(s, nl)
+    return fg[:len(x)], fo[:len(x)], e, eo, w, wo
+
+
+def test_lj_full_fp64_bit_exact_one_lane(gpu):
+    fg, fo, e, eo, w, wo = _force_case(gpu, (8, 8, 8), False, gpu.PREC_FP64, 1)
+    assert np.array_equal(fg, fo)
+    assert abs(e - eo) <= 1e-12 * abs(eo)
+    assert abs(w - wo) <= 1e-12 * abs(wo)
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16])
+def test_lj_full_fp64_lanes(gpu, lanes):
+    fg, fo, e, eo, w, wo = _force_case(gpu, (8, 8, 8), False, gpu.PREC_FP64,
+                                       lanes)
+    err = np.linalg.norm(fg - fo) / np.linalg.norm(fo)
+    assert err < 1e-13
+    assert np.abs(fg.sum(0)).max() < 1e-9
+
+
+def test_lj_half_fp64(gpu):
+    fg, fo, e, eo, w, wo = _force_case(gpu, (8, 8, 8), True, gpu.PREC_FP64, 0)
+    err = np.linalg.norm(fg - fo) / np.linalg.norm(fo)
+    assert err < 1e-12
+    assert abs(e - eo) <= 1e-12 * abs(eo)
+    assert abs(w - wo) <= 1e-12 * abs(wo)
+
+
+def test_lj_mixed_precision(gpu):
+    fg, fo, e, eo, w, wo = _force_case(gpu, (8, 8, 8), False, gpu.PREC_MIXED,
+                                       0)
+    err = np.linalg.norm(fg - fo) / np.linalg.norm(fo)
+    rms = np.sqrt((fo ** 2).sum(1).mean())
+    maxe = np.abs(fg - fo).max() / rms
+    print(f"mixed: |dF|/|F| = {err:.3e}, max|dF_i|/rms|F| = {maxe:.3e}")
+    assert err < 1e-5
+    assert maxe < 1e-4
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_md_loop_fp64(gpu, half):
+    """CbnMD::init + run on fcc 10^3 cells (4000 atoms), 100 steps."""
+    M = gpu
+    kw = dict(ncells=(10, 10, 10), nsteps=100, thermo=10, half=half)
+    md = M.CbnMD(M.lattice_params(**kw), precision=M.PREC_FP64,
+                 lanes=1 if not half else 0)
+    md.init()
+    md.run(100)
+    ref = O.MD(O.lj_params(**kw))
+    ref.init()
+    ref.run(100)
+    th, tho = md.thermo(), ref.thermo()
+    assert th.shape[0] == tho.shape[0] == 11
+    assert np.allclose(th[:, :4], tho, rtol=1e-10, atol=1e-13)
+    x, v, f, _, _ = md.arrays()
+    xo, vo, fo, _, _ = ref.arrays()
+    n = md.n_local
+    if not half:
+        # same operations in the same order: bit-identical trajectory
+        assert np.array_equal(x[:n], xo[:n])
+        assert np.array_equal(v[:n], vo[:n])
+    else:
+        assert np.abs(x[:n] - xo[:n]).max() < 1e-9
+        assert np.abs(v[:n] - vo[:n]).max() < 1e-9
+
+
+def test_md_loop_split_runs_match_one_run(gpu):
+    M = gpu
+    kw = dict(ncells=(6, 6, 6), nsteps=50, thermo=10)
+    a = M.CbnMD(M.lattice_params(**kw), lanes=1)
+    a.init()
+    a.run(50)
+    b = M.CbnMD(M.lattice_params(**kw), lanes=1)
+    b.init()
+    b.run(17)
+    b.run(33)
+    xa, va = a.arrays()[:2]
+    xb, vb = b.arrays()[:2]
+    n = a.n_local
+    assert np.array_equal(xa[:n], xb[:n])
+    assert np.array_equal(va[:n], vb[:n])
+
+
+def test_md_mixed_energy_drift(gpu):
+    M = gpu
+    kw = dict(ncells=(10, 10, 10), nsteps=200, thermo=20)
+    md = M.CbnMD(M.lattice_params(**kw), precision=M.PREC_MIXED)
+    md.init()
+    md.run(200)
+    th = md.thermo()
+    e = th[:, 3]
+    drift = np.abs(e - e[0]).max() / abs(e[0])
+    print(f"mixed precision drift over 200 steps: {drift:.3e}")
+    assert drift < 1e-3
