@@ -1,0 +1,183 @@
+"""Benchmark: LJ timestep throughput on one MI355X (atom-steps/s).
+
+Workload (the reference's LJ melt at scale): fcc lattice at reduced density
+0.8442, 80x80x80 unit cells = 2,048,000 atoms, periodic cube, eps = sigma =
+mass = 1, cutoff 2.5, skin 0.3, full Verlet list rebuilt every 20 steps,
+velocities from the seeded loop-geom generator at T = 1.44 (zero net
+momentum), dt = 0.005, NVE, thermo every 100 steps.  A "step" is one MD
+timestep of the whole system (integrate, list maintenance, force, thermo).
+
+Prints ONE JSON line.  value = atom-steps/s of the headline variant
+(N * steps / wall time of the timed steps, cabanamd_impl.h:406-407), with
+all state resident in HBM.  The roofline object is for the fused LJ force +
+integrate kernel, timed with HIP events on the engine's stream during the
+timed steps.  cpu_baseline times the FP64 OpenMP oracle on a bounded sample
+of the same workload on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("atom-steps/s/GPU (LJ 2M, NNP) + batched vs plain speedup, "
+          "p50/p99 job latency")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per atom of the fused full-list kernel: the LJ full
+# force per-atom figure (x_i 24 + offset 4 + 78 neighbour indices x 4 +
+# f_i 24 = 364 B) plus the fused integrator traffic (v read 24, v write 24,
+# x write 24 = 72 B).
+LJ_FULL_BYTES_PER_ATOM = 364
+FUSED_NVE_BYTES_PER_ATOM = 72
+ALG_BYTES_PER_ATOM = LJ_FULL_BYTES_PER_ATOM + FUSED_NVE_BYTES_PER_ATOM
+
+
+def _variant(M, ncells, steps, warmup, half, prec, lanes=0):
+    p = M.lattice_params(ncells=ncells, half=half, thermo=100,
+                         nsteps=steps)
+    md = M.CbnMD(p, precision=prec, lanes=lanes)
+    t0 = time.time()
+    md.init()
+    t_init = time.time() - t0
+    n = md.n_local
+    if warmup:
+        md.run(warmup)
+    M._lib.call("cmd_device_synchronize")
+    md.profile(True)
+    t0 = time.perf_counter()
+    md.run(steps)
+    M._lib.call("cmd_device_synchronize")
+    dt = time.perf_counter() - t0
+    launches, mean_ms = md.kernel_stats()
+    th = md.thermo()
+    e = th[:, 3]
+    out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
+               atom_steps_per_s=n * steps / dt, kernel_launches=launches,
+               kernel_mean_ms=mean_ms, init_s=t_init,
+               lanes=md.force.lanes,
+               energy_drift=float(abs(e[-1] - e[0]) / abs(e[0])),
+               timers=md.timers())
+    del md
+    return out
+
+
+def _cpu_baseline(ncells, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", threads))
+    md = O.MD(O.lj_params(ncells=ncells, thermo=100, nsteps=steps))
+    md.init()
+    t0 = time.perf_counter()
+    md.run(steps)
+    dt = time.perf_counter() - t0
+    n = md.n_local
+    return dict(value=n * steps / dt, unit="atom-steps/s", cores=threads,
+                kind="port",
+                sample=(f"FP64 OpenMP oracle (restatement of the reference "
+                        f"algorithm), same workload, {n} atoms, {steps} "
+                        f"steps incl. one list rebuild, {dt:.2f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=80)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--variants", default="mixed,fp64,half")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a 2 GiB streaming copy (k_copy_probe) "
+                         "to calibrate byte counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding of one system is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import cabanamd_amd as M
+    if M.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    nc = (args.cells,) * 3
+    copy_gbs = None
+    if args.calibrate:
+        import ctypes as C
+        g, ms = C.c_double(), C.c_double()
+        M._lib.call("cmd_bandwidth_probe", C.c_int64(1 << 31), 10,
+                    C.byref(g), C.byref(ms))
+        copy_gbs = g.value
+        print(f"# copy probe: 2 GiB read + 2 GiB write, {ms.value:.4f} ms, "
+              f"{copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
+    variants = args.variants.split(",")
+    res = {}
+    for v in variants:
+        half = v == "half"
+        prec = M.PREC_MIXED if v == "mixed" else M.PREC_FP64
+        res[v] = _variant(M, nc, args.steps, args.warmup, half, prec)
+        print(f"# {v}: {res[v]['atom_steps_per_s']:.4e} atom-steps/s, "
+              f"{res[v]['ms_per_step']:.4f} ms/step, kernel "
+              f"{res[v]['kernel_mean_ms']:.4f} ms x {res[v]['kernel_launches']}"
+              f", lanes {res[v]['lanes']}", file=sys.stderr, flush=True)
+    head = variants[0]
+    h = res[head]
+    n = h["n"]
+    achieved = (ALG_BYTES_PER_ATOM * n / (h["kernel_mean_ms"] * 1e-3) / 1e9
+                if h["kernel_mean_ms"] > 0 else None)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lj_full_fused_2M_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(nc, args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": h["atom_steps_per_s"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-pair/f64" if head == "mixed" else "f64",
+        "data": "synthetic (seeded fcc lattice + loop-geom velocities)",
+        "config": {
+            "workload": ("LJ fcc rho 0.8442, 80^3 cells = 2,048,000 atoms, "
+                         "rc 2.5, skin 0.3, full Verlet list rebuilt every 20"
+                         " steps, dt 0.005, NVE, thermo every 100"),
+            "n_atoms": n, "list": "half" if head == "half" else "full",
+            "precision": head, "lanes_per_atom": h["lanes"]},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": traffic,
+            "kernel": "k_lj_full (fused force + NVE), mean of "
+                      f"{h['kernel_launches']} launches = "
+                      f"{h['kernel_mean_ms']:.4f} ms, "
+                      f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},
+        "cpu_baseline": cpu,
+        "copy_probe_gbs": copy_gbs,
+        "variants": {k: {kk: vv for kk, vv in r.items() if kk != "timers"}
+                     for k, r in res.items()},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

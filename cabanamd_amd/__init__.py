@@ -210,11 +210,12 @@ class Binning:
 class NeighborVerlet:
     """NeighborVerlet<System, Full|Half, CSR> (neighbor_verlet.h:19-70)."""
 
-    def __init__(self, neigh_cut, half_neigh, max_neigh_guess=50):
+    def __init__(self, neigh_cut, half_neigh, max_neigh_guess=50, lanes=1):
         h = C.c_void_p()
         call("cmd_neigh_create", neigh_cut, int(half_neigh),
              int(max_neigh_guess), C.byref(h))
         self._h = h
+        call("cmd_neigh_set_lanes", h, int(lanes))
         self.neigh_cut = neigh_cut
         self.half_neigh = bool(half_neigh)
 
@@ -266,13 +267,12 @@ class NeighborVerlet:
 class ForceLJ:
     """ForceLJ (force_types/force_lj_cabana_neigh_impl.h:49-377)."""
 
-    def __init__(self, system, precision=PREC_FP64, lanes=0):
+    def __init__(self, system, precision=PREC_FP64):
         h = C.c_void_p()
         call("cmd_force_lj_create", system.ntypes, C.byref(h))
         self._h = h
         self._own = True
         call("cmd_force_set_precision", h, precision)
-        call("cmd_force_set_lanes", h, lanes)
 
     @classmethod
     def _borrow(cls, handle):

@@ -67,6 +67,7 @@ _PROTOS = {
     "cmd_device_init": ([C.c_int], None),
     "cmd_device_count": ([IP], None),
     "cmd_device_synchronize": ([], None),
+    "cmd_bandwidth_probe": ([C.c_int64, C.c_int, DP, DP], None),
     "cmd_domain_create": ([DP, DP, C.c_double, C.POINTER(Domain)], None),
     "cmd_system_create": ([C.c_int, C.POINTER(VP)], None),
     "cmd_system_destroy": ([VP], None),
@@ -90,6 +91,7 @@ _PROTOS = {
                             C.c_int, C.c_int, C.c_int, IP], None),
     "cmd_neigh_create": ([C.c_double, C.c_int, C.c_int, C.POINTER(VP)], None),
     "cmd_neigh_destroy": ([VP], None),
+    "cmd_neigh_set_lanes": ([VP, C.c_int], None),
     "cmd_neigh_build": ([VP, VP], None),
     "cmd_neigh_info": ([VP, IP, LP, IP, IP], None),
     "cmd_neigh_download": ([VP, LP, IP], None),

@@ -216,17 +216,6 @@ cmd_status scan_exclusive( const int *in, int64_t *out, int n, Scratch &sc,
 }
 
 // ---------------------------------------------------------------- cell sort
-__device__ __forceinline__ int cell_coord( double c, double mn, double rd,
-                                           int n, int floor_mode )
-{
-    double t = ( c - mn ) * rd;
-    int ic = floor_mode ? (int)floor( t ) : (int)t;
-    if ( ic == n )
-        ic = n - 1;
-    ic = ic < 0 ? 0 : ( ic > n - 1 ? n - 1 : ic );
-    return ic;
-}
-
 __device__ __forceinline__ int cell_of_pos( const CellGrid &g, double4 p )
 {
     int a = cell_coord( p.x, g.min[0], g.rd[0], g.n[0], g.floor_mode );
@@ -696,6 +685,16 @@ static cmd_status sum_mv2( cmd_system *S, double *out )
     return CMD_OK;
 }
 
+// Streaming copy used to calibrate byte counters and achievable bandwidth.
+__global__ void k_copy_probe( const double4 *__restrict__ a,
+                              double4 *__restrict__ b, size_t n )
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for ( ; i < n; i += stride )
+        b[i] = a[i];
+}
+
 } // namespace cmd
 
 using namespace cmd;
@@ -729,6 +728,40 @@ cmd_status cmd_device_init( int device )
     CMD_REQUIRE( device >= 0 && device < c, CMD_ERR_ARG,
                  "device %d out of range (%d visible)", device, c );
     CMD_HIP( hipSetDevice( device ) );
+    return CMD_OK;
+}
+
+cmd_status cmd_bandwidth_probe( int64_t bytes, int reps, double *gbs,
+                                double *ms_per_copy )
+{
+    CMD_REQUIRE( bytes >= 32 && reps >= 1, CMD_ERR_ARG, "bad probe size" );
+    const size_t n = (size_t)bytes / sizeof( double4 );
+    DevBuf<double4> a, b;
+    CMD_TRY( a.reserve( n ) );
+    CMD_TRY( b.reserve( n ) );
+    hipStream_t st;
+    CMD_HIP( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+    const int grid = 256 * 8;
+    k_fill<double><<<grid, TPB, 0, st>>>( (double *)a.p, 1.0, 4 * n );
+    k_copy_probe<<<grid, TPB, 0, st>>>( a.p, b.p, n );
+    hipEvent_t e0, e1;
+    CMD_HIP( hipEventCreate( &e0 ) );
+    CMD_HIP( hipEventCreate( &e1 ) );
+    CMD_HIP( hipEventRecord( e0, st ) );
+    for ( int r = 0; r < reps; ++r )
+        k_copy_probe<<<grid, TPB, 0, st>>>( a.p, b.p, n );
+    CMD_HIP( hipEventRecord( e1, st ) );
+    CMD_HIP( hipEventSynchronize( e1 ) );
+    float ms = 0.0f;
+    CMD_HIP( hipEventElapsedTime( &ms, e0, e1 ) );
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    (void)hipStreamDestroy( st );
+    const double per = ms / reps;
+    if ( ms_per_copy )
+        *ms_per_copy = per;
+    if ( gbs )
+        *gbs = 2.0 * n * sizeof( double4 ) / ( per * 1e-3 ) / 1e9;
     return CMD_OK;
 }
 
@@ -942,6 +975,7 @@ cmd_status cmd_system_upload( cmd_system *s, int n_local, int n_ghost,
     s->n_local = n_local;
     s->n_ghost = n_ghost;
     s->N = n_local;
+    s->bin_grid_valid = false;
     for ( int k = 0; k < 6; ++k )
         s->counts[k] = 0;
     return CMD_OK;
@@ -1190,6 +1224,8 @@ cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
         s->f.swap( s->f_alt );
         s->id.swap( s->id_alt );
         s->n_ghost = 0; // ghosts are not part of the permuted range
+        s->bin_grid = g;
+        s->bin_grid_valid = true;
     }
     if ( perm_out && n > 0 )
     {

@@ -154,6 +154,17 @@ struct CellGrid
     int floor_mode; // 1: floor() (Cabana LinkedCellList); 0: truncation
 };
 
+__device__ __forceinline__ int cell_coord( double c, double mn, double rd,
+                                           int n, int floor_mode )
+{
+    const double t = ( c - mn ) * rd;
+    int ic = floor_mode ? (int)floor( t ) : (int)t;
+    if ( ic == n )
+        ic = n - 1;
+    ic = ic < 0 ? 0 : ( ic > n - 1 ? n - 1 : ic );
+    return ic;
+}
+
 // Counting sort of n positions into grid cells, stable by index.  Produces
 // perm (sorted slot -> index) and start (ncell+1 cell offsets).
 struct CellSort
@@ -193,6 +204,9 @@ struct cmd_system
     cmd::DevBuf<int> flag, pos;
 
     cmd::CellSort bins;
+    // grid of the last sorting create_binning (valid until atoms change)
+    cmd::CellGrid bin_grid{};
+    bool bin_grid_valid = false;
     cmd::Scratch sc;
 
     cmd::MassTable mass_table() const
@@ -210,15 +224,20 @@ struct cmd_neigh
     double neigh_cut = 0.0;
     int half = 0;
     int max_neigh_guess = 50;
+    int lanes = 1;       // G: lanes per atom of the slab layout (1, 2, 4)
+    int cap = 0;         // per-atom slab capacity (>= longest row)
     int n_rows = 0;
-    int64_t nnz = 0;
+    int64_t nnz = -1;    // set when the CSR view is made
     int max_row = 0;
-    cmd::DevBuf<int64_t> offsets;
+    cmd::DevBuf<int> slab;   // lane-interleaved rows, see neighbor.hip
+    cmd::DevBuf<int> counts; // per-row lengths
+    bool csr_valid = false;
+    cmd::DevBuf<int64_t> offsets; // CSR view (on demand)
     cmd::DevBuf<int> idx;
-    cmd::DevBuf<int> counts;
     cmd::DevBuf<double4> xs; // positions in grid order, w = atom index
     cmd::CellSort grid_sort;
     int ng[3] = { 0, 0, 0 };
+    bool used_bin_grid = false;
     hipStream_t stream = nullptr; // stream of the last build
 };
 
@@ -265,8 +284,8 @@ struct StepCtl
 
 // LJ kernels (lj.hip)
 cmd_status lj_full( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
-                    int lanes, const StepCtl &ctl, double4 *x_next,
-                    double *partial, int *nblocks_out );
+                    const StepCtl &ctl, double4 *x_next, double *partial,
+                    int *nblocks_out );
 cmd_status lj_half( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
                     int thermo, double *partial, int *nblocks_out );
 cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
@@ -282,4 +301,8 @@ cmd_status reduce_partials( const double *partial, int nblocks, double *out,
 cmd_status ghost_refresh( cmd_system *S );
 cmd_status wrap_local( cmd_system *S );
 cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N );
+cmd_status neigh_build( cmd_neigh *N, cmd_system *S );
+cmd_status neigh_csr( cmd_neigh *N );
+size_t slab_entries( int n_local, int cap, int lanes );
+cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N );
 } // namespace cmd

@@ -201,7 +201,7 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         ctl.thermo = th ? 1 : 0;
         ctl.write_f = 1;
         const int n = S->n_local;
-        const int G = F->lanes ? F->lanes : 4;
+        const int G = N->lanes;
         const int nbA = std::max( 1, ( n + 255 ) / 256 * G );
         CMD_TRY( md->partA.reserve( 4 * (size_t)nbA + 4 ) );
         CMD_TRY( md->partB.reserve( 4 * (size_t)( ( n + 255 ) / 256 + 1 ) ) );
@@ -210,7 +210,7 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         if ( !half )
         {
             int k = md->begin( PH_FORCE );
-            CMD_TRY( lj_full( F, S, N, G, ctl, S->x_alt.p, md->partA.p,
+            CMD_TRY( lj_full( F, S, N, ctl, S->x_alt.p, md->partA.p,
                               &nb_force ) );
             md->end( k );
         }
@@ -351,12 +351,18 @@ cmd_status cmd_md_init( cmd_md *md )
     CMD_TRY( cmd_force_set_lanes( md->F, md->lanes ) );
     CMD_TRY( cmd_neigh_create( md->neigh_cut, p.half, p.max_neigh_guess,
                                &md->N ) );
+    if ( md->lanes > 0 )
+        CMD_TRY( cmd_neigh_set_lanes( md->N, md->lanes ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
     CMD_TRY( cmd_comm_exchange( S ) );
     CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
                                  md->neigh_cut, 1, 1, 0, 1, nullptr ) );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
-    CMD_TRY( cmd_neigh_build( md->N, S ) );
+    if ( md->lanes == 0 && !p.half )
+        CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
+    else
+        CMD_TRY( cmd_neigh_build( md->N, S ) );
+    md->F->lanes = md->N->lanes;
     CMD_TRY( force_compute_plain( md->F, S, md->N ) );
     if ( p.half )
         CMD_TRY( cmd_comm_update_force( S ) );

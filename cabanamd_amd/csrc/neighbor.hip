@@ -1,4 +1,4 @@
-// neighbor.hip -- CSR Verlet list build.
+// neighbor.hip -- Verlet list build.
 //
 // Stands in for NeighborVerlet::create (neighbor_verlet.h:43-62), which
 // builds a Cabana::VerletList over rows [0,N_local) and candidates
@@ -8,14 +8,19 @@
 // contraction; half lists keep j only if x_j is lexicographically greater
 // than x_i (x, then y, then z).
 //
-// Build: (1) bounding box of all atoms, (2) stable counting sort of all
-// atoms into a grid with cells >= rc, (3) one 64-lane wave per cell stages
-// the 27-cell candidate set in LDS in tiles and, for each local atom of the
-// cell, tests 64 candidates per step, counting by ballot/popcount (count
-// pass) or writing by ballot prefix (fill pass); (4) scan of the per-row
-// counts into int64 CSR offsets between the passes.  Rows come out in
-// stencil-cell order (x-cell, y-cell, z-cell) and index order inside a
-// cell, so a build is deterministic.
+// Layout ("slab"): atoms are grouped per wavefront, APW = 64/G atoms per
+// wave with G lanes per atom.  Neighbour k of atom a of wave w is stored at
+//   slab[((w * R + k / G) * 64) + (k % G) * APW + a],   R = ceil(cap / G),
+// so that at every step of the force loop the 64 lanes of a wave read 64
+// consecutive ints.  cap is the per-atom capacity, i.e. the reference's
+// max_neigh_guess of the 2D Verlet layout: a build that finds a longer row
+// grows cap to 1.1x the longest row and runs again (neighbor_verlet.h:58-61).
+// A CSR view (int64 offsets, int32 indices) is derived on demand.
+//
+// Build: bounding box of all atoms, stable counting sort of all atoms into a
+// grid with cells >= rc, then one lane per local atom walks its 27-cell
+// stencil (x-cell, y-cell, z-cell order; index order inside a cell) and
+// appends accepted neighbours.  The row order is therefore deterministic.
 #include "internal.h"
 
 #include <algorithm>
@@ -25,8 +30,6 @@
 namespace cmd
 {
 static constexpr int TPB = 256;
-static constexpr int NB_TILE = 512; // candidates staged per LDS tile
-static constexpr int NB_ATOMS = 64; // row counters kept per chunk of a cell
 
 __global__ void k_bbox( int n, const double4 *__restrict__ x,
                         double *__restrict__ part )
@@ -72,17 +75,30 @@ __global__ void k_bbox( int n, const double4 *__restrict__ x,
     }
 }
 
+// One 64-lane wave: lane t folds partials t, t+64, ... then a butterfly.
 __global__ void k_bbox_final( int nb, const double *__restrict__ part,
                               double *__restrict__ out )
 {
-    if ( threadIdx.x < 6 )
+    const int t = threadIdx.x;
+    double v[6];
+    for ( int k = 0; k < 6; ++k )
+        v[k] = ( k < 3 ) ? DBL_MAX : -DBL_MAX;
+    for ( int b = t; b < nb; b += 64 )
+        for ( int k = 0; k < 6; ++k )
+            v[k] = ( k < 3 ) ? fmin( v[k], part[b * 6 + k] )
+                             : fmax( v[k], part[b * 6 + k] );
+    for ( int k = 0; k < 6; ++k )
     {
-        const int k = threadIdx.x;
-        double a = part[k];
-        for ( int b = 1; b < nb; ++b )
-            a = ( k < 3 ) ? fmin( a, part[b * 6 + k] ) : fmax( a, part[b * 6 + k] );
-        out[k] = a;
+#pragma unroll
+        for ( int o = 32; o > 0; o >>= 1 )
+        {
+            const double u = __shfl_xor( v[k], o, 64 );
+            v[k] = ( k < 3 ) ? fmin( v[k], u ) : fmax( v[k], u );
+        }
     }
+    if ( t == 0 )
+        for ( int k = 0; k < 6; ++k )
+            out[k] = v[k];
 }
 
 __global__ void k_gather_sorted( int n, const int *__restrict__ perm,
@@ -115,110 +131,80 @@ __device__ __forceinline__ bool accept_pair( const double4 &pi, const double4 &p
     return rsq <= rc2;
 }
 
-template <bool FILL>
-__global__ __launch_bounds__( 64 ) void k_neigh_rows(
-    const double4 *__restrict__ xs, const int *__restrict__ start, int ng0,
-    int ng1, int ng2, int n_local, double rc2, int half,
-    int *__restrict__ counts, const int64_t *__restrict__ offsets,
-    int *__restrict__ idx, int *__restrict__ max_row )
+
+// One lane per local atom i; lanes of a wave are consecutive atoms, which
+// after binning sit in neighbouring cells and walk overlapping stencils.
+template <int G>
+__global__ __launch_bounds__( TPB ) void k_neigh_slab(
+    int n_local, const double4 *__restrict__ x, const double4 *__restrict__ xs,
+    const int *__restrict__ start, CellGrid g, double rc2, int half, int cap,
+    int *__restrict__ counts, int *__restrict__ slab, int *__restrict__ max_row )
 {
-    const int c = blockIdx.x;
-    const int cs = start[c], ce = start[c + 1];
-    if ( cs == ce )
-        return;
-    const int lane = threadIdx.x;
-    __shared__ int s_rb[9], s_roff[10];
-    __shared__ double4 s_c[NB_TILE];
-    __shared__ int s_cnt[NB_ATOMS];
-    if ( lane == 0 )
+    constexpr int APW = 64 / G;
+    const int i = blockIdx.x * TPB + (int)threadIdx.x;
+    int cnt = 0;
+    if ( i < n_local )
     {
-        const int a = c / ( ng1 * ng2 ), b = ( c / ng2 ) % ng1, e = c % ng2;
+        const double4 pi = x[i];
+        const int a = cell_coord( pi.x, g.min[0], g.rd[0], g.n[0], g.floor_mode );
+        const int b = cell_coord( pi.y, g.min[1], g.rd[1], g.n[1], g.floor_mode );
+        const int e = cell_coord( pi.z, g.min[2], g.rd[2], g.n[2], g.floor_mode );
         const int e0 = e > 0 ? e - 1 : 0;
-        const int e1 = e < ng2 - 1 ? e + 1 : ng2 - 1;
-        int q = 0, tot = 0;
-        for ( int da = -1; da <= 1; ++da )
-            for ( int db = -1; db <= 1; ++db )
-            {
-                const int aa = a + da, bb = b + db;
-                int rb = 0, re = 0;
-                if ( aa >= 0 && aa < ng0 && bb >= 0 && bb < ng1 )
-                {
-                    const int base = ( aa * ng1 + bb ) * ng2;
-                    rb = start[base + e0];
-                    re = start[base + e1 + 1];
-                }
-                s_rb[q] = rb;
-                s_roff[q] = tot;
-                tot += re - rb;
-                ++q;
-            }
-        s_roff[9] = tot;
-    }
-    __syncthreads();
-    const int ncand = s_roff[9];
-    for ( int a0 = cs; a0 < ce; a0 += NB_ATOMS )
-    {
-        const int na = min( NB_ATOMS, ce - a0 );
-        if ( lane < na )
-            s_cnt[lane] = 0;
-        __syncthreads();
-        for ( int t0 = 0; t0 < ncand; t0 += NB_TILE )
+        const int e1 = e < g.n[2] - 1 ? e + 1 : g.n[2] - 1;
+        const int R = ( cap + G - 1 ) / G;
+        const int w = i / APW, ai = i % APW;
+        int *out = slab + (size_t)w * R * 64 + ai;
+        for ( int aa = a - 1; aa <= a + 1; ++aa )
         {
-            const int nt = min( NB_TILE, ncand - t0 );
-            for ( int q = lane; q < nt; q += 64 )
+            if ( aa < 0 || aa >= g.n[0] )
+                continue;
+            for ( int bb = b - 1; bb <= b + 1; ++bb )
             {
-                const int qq = t0 + q;
-                int r = 0;
-                while ( qq >= s_roff[r + 1] )
-                    ++r;
-                s_c[q] = xs[s_rb[r] + ( qq - s_roff[r] )];
-            }
-            __syncthreads();
-            for ( int t = 0; t < na; ++t )
-            {
-                const double4 pi = xs[a0 + t];
-                const int i = (int)pi.w;
-                if ( i >= n_local )
+                if ( bb < 0 || bb >= g.n[1] )
                     continue;
-                int cnt = s_cnt[t];
-                const int64_t rbase = FILL ? offsets[i] : 0;
-                for ( int q0 = 0; q0 < nt; q0 += 64 )
+                const int base = ( aa * g.n[1] + bb ) * g.n[2];
+                const int s1 = start[base + e1 + 1];
+                for ( int s = start[base + e0]; s < s1; ++s )
                 {
-                    const int q = q0 + lane;
-                    bool acc = false;
-                    int j = 0;
-                    if ( q < nt )
+                    const double4 pj = xs[s];
+                    const int j = (int)pj.w;
+                    if ( accept_pair( pi, pj, i, j, rc2, half ) )
                     {
-                        const double4 pj = s_c[q];
-                        j = (int)pj.w;
-                        acc = accept_pair( pi, pj, i, j, rc2, half );
+                        if ( cnt < cap )
+                            out[( cnt / G ) * 64 + ( cnt % G ) * APW] = j;
+                        ++cnt;
                     }
-                    const uint64_t m = __ballot( acc );
-                    if ( FILL && acc )
-                    {
-                        const int pre = __builtin_amdgcn_mbcnt_hi(
-                            (unsigned)( m >> 32 ),
-                            __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) );
-                        idx[rbase + cnt + pre] = j;
-                    }
-                    cnt += __popcll( m );
                 }
-                if ( lane == 0 )
-                    s_cnt[t] = cnt;
-            }
-            __syncthreads();
-        }
-        if ( !FILL && lane < na )
-        {
-            const int i = (int)xs[a0 + lane].w;
-            if ( i < n_local )
-            {
-                counts[i] = s_cnt[lane];
-                atomicMax( max_row, s_cnt[lane] );
             }
         }
-        __syncthreads();
+        counts[i] = cnt;
     }
+    int m = cnt;
+#pragma unroll
+    for ( int o = 32; o > 0; o >>= 1 )
+        m = max( m, __shfl_xor( m, o, 64 ) );
+    if ( ( threadIdx.x & 63 ) == 0 && m > 0 )
+        atomicMax( max_row, m );
+}
+
+// CSR view: row i of the slab copied to idx[offsets[i] ...].
+template <int G>
+__global__ void k_slab_to_csr( int n_local, int cap, const int *__restrict__ counts,
+                               const int *__restrict__ slab,
+                               const int64_t *__restrict__ offsets,
+                               int *__restrict__ idx )
+{
+    constexpr int APW = 64 / G;
+    const int i = blockIdx.x * TPB + (int)threadIdx.x;
+    if ( i >= n_local )
+        return;
+    const int R = ( cap + G - 1 ) / G;
+    const int w = i / APW, ai = i % APW;
+    const int *row = slab + (size_t)w * R * 64 + ai;
+    const int n = counts[i];
+    const int64_t o = offsets[i];
+    for ( int k = 0; k < n; ++k )
+        idx[o + k] = row[( k / G ) * 64 + ( k % G ) * APW];
 }
 
 // Grid for the candidate search (mirrored by the oracle): cells of at least
@@ -248,19 +234,57 @@ static void neigh_grid( const double lo[3], const double hi[3], double rc,
         rcell[k] = ng[k] / std::max( ext[k], 1e-300 );
 }
 
-static cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
+static cmd_status launch_slab( cmd_neigh *N, cmd_system *S, const CellGrid &g,
+                               int n_local, int cap, int *max_row_dev )
+{
+    const double rc2 = N->neigh_cut * N->neigh_cut;
+    const int nb = std::max( 1, ( n_local + TPB - 1 ) / TPB );
+#define CMD_SLAB( GG )                                                        \
+    k_neigh_slab<GG><<<nb, TPB, 0, S->stream>>>(                              \
+        n_local, S->x.p, N->xs.p, N->grid_sort.start.p, g, rc2, N->half, cap, \
+        N->counts.p, N->slab.p, max_row_dev )
+    switch ( N->lanes )
+    {
+    case 1:
+        CMD_SLAB( 1 );
+        break;
+    case 2:
+        CMD_SLAB( 2 );
+        break;
+    case 4:
+        CMD_SLAB( 4 );
+        break;
+    case 8:
+        CMD_SLAB( 8 );
+        break;
+    default:
+        set_error( "unsupported neighbour layout lanes %d", N->lanes );
+        return CMD_ERR_ARG;
+    }
+#undef CMD_SLAB
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+size_t slab_entries( int n_local, int cap, int lanes )
+{
+    const int apw = 64 / lanes;
+    const size_t waves = (size_t)( n_local + apw - 1 ) / apw;
+    const size_t R = (size_t)( cap + lanes - 1 ) / lanes;
+    return std::max<size_t>( waves * R * 64, 1 );
+}
+
+cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
 {
     hipStream_t st = S->stream;
     N->stream = st;
     const int n_local = S->n_local;
     const int nt = S->n_local + S->n_ghost;
     N->n_rows = n_local;
-    CMD_TRY( N->offsets.reserve( (size_t)n_local + 1 ) );
+    N->csr_valid = false;
     CMD_TRY( N->counts.reserve( std::max( n_local, 1 ) ) );
     if ( nt == 0 || n_local == 0 )
     {
-        CMD_TRY( fill_i32( (int *)N->offsets.p, 0, 2 * ( (size_t)n_local + 1 ),
-                           st ) );
         N->nnz = 0;
         N->max_row = 0;
         return CMD_OK;
@@ -277,53 +301,116 @@ static cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
     CMD_HIP( hipMemcpyAsync( bb, S->sc.dres.p, sizeof( bb ),
                              hipMemcpyDeviceToHost, st ) );
     CMD_HIP( hipStreamSynchronize( st ) );
-    // (2) grid and counting sort of all atoms
-    int ng[3];
-    double rcell[3];
-    neigh_grid( bb, bb + 3, N->neigh_cut, nt, ng, rcell );
+    // (2) grid and counting sort of all atoms.  The grid of the last sorting
+    // create_binning is used when its cells are at least rc wide and it holds
+    // every atom (local and ghost): rows then walk cells in the order the
+    // local atoms are stored.  Otherwise a grid over the bounding box.
     CellGrid g;
-    g.floor_mode = 0;
-    for ( int k = 0; k < 3; ++k )
+    bool use_bin = S->bin_grid_valid;
+    if ( use_bin )
     {
-        g.min[k] = bb[k];
-        g.rd[k] = rcell[k];
-        g.n[k] = ng[k];
-        N->ng[k] = ng[k];
+        const CellGrid &bg = S->bin_grid;
+        for ( int k = 0; k < 3; ++k )
+        {
+            const double lo = bg.min[k], hi = bg.min[k] + bg.n[k] / bg.rd[k];
+            if ( !( 1.0 / bg.rd[k] >= N->neigh_cut ) || !( bb[k] >= lo ) ||
+                 !( bb[3 + k] < hi ) )
+                use_bin = false;
+        }
     }
+    if ( use_bin )
+        g = S->bin_grid;
+    else
+    {
+        int ng[3];
+        double rcell[3];
+        neigh_grid( bb, bb + 3, N->neigh_cut, nt, ng, rcell );
+        g.floor_mode = 0;
+        for ( int k = 0; k < 3; ++k )
+        {
+            g.min[k] = bb[k];
+            g.rd[k] = rcell[k];
+            g.n[k] = ng[k];
+        }
+    }
+    for ( int k = 0; k < 3; ++k )
+        N->ng[k] = g.n[k];
+    N->used_bin_grid = use_bin;
     CMD_TRY( cell_sort( S->x.p, nt, g, N->grid_sort, S->sc, st ) );
     CMD_TRY( N->xs.reserve( nt ) );
     k_gather_sorted<<<( nt + TPB - 1 ) / TPB, TPB, 0, st>>>(
         nt, N->grid_sort.perm.p, S->x.p, N->xs.p );
     CMD_LAUNCH_CHECK();
-    // (3) count pass
-    const int ncell = N->grid_sort.ncell;
-    const double rc2 = N->neigh_cut * N->neigh_cut;
+    // (3) single pass into the slab at the current capacity; regrow and
+    // repeat if a row was longer
     CMD_TRY( S->sc.ires.reserve( 4 ) );
-    CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, st ) );
-    k_neigh_rows<false><<<ncell, 64, 0, st>>>(
-        N->xs.p, N->grid_sort.start.p, ng[0], ng[1], ng[2], n_local, rc2,
-        N->half, N->counts.p, nullptr, nullptr, S->sc.ires.p );
-    CMD_LAUNCH_CHECK();
-    // (4) offsets
+    if ( N->cap < 1 )
+        N->cap = std::max( 1, N->max_neigh_guess );
+    for ( int attempt = 0; attempt < 3; ++attempt )
+    {
+        CMD_TRY( N->slab.reserve( slab_entries( n_local, N->cap, N->lanes ) ) );
+        CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, st ) );
+        CMD_TRY( launch_slab( N, S, g, n_local, N->cap, S->sc.ires.p ) );
+        int mr = 0;
+        CMD_HIP( hipMemcpyAsync( &mr, S->sc.ires.p, sizeof( int ),
+                                 hipMemcpyDeviceToHost, st ) );
+        CMD_HIP( hipStreamSynchronize( st ) );
+        N->max_row = mr;
+        // NeighborVerlet::create: guess = 1.1 * max when max > guess
+        if ( mr > N->max_neigh_guess )
+            N->max_neigh_guess = (int)( mr * 1.1 );
+        if ( mr <= N->cap )
+            break;
+        N->cap = std::max( N->max_neigh_guess, mr );
+    }
+    CMD_REQUIRE( N->max_row <= N->cap, CMD_ERR_CAPACITY,
+                 "neighbour rows kept growing during the build" );
+    N->nnz = -1; // known once the CSR view is made
+    return CMD_OK;
+}
+
+cmd_status neigh_csr( cmd_neigh *N )
+{
+    if ( N->csr_valid )
+        return CMD_OK;
+    const int n = N->n_rows;
+    hipStream_t st = N->stream;
+    CMD_TRY( N->offsets.reserve( (size_t)n + 1 ) );
+    if ( n == 0 )
+    {
+        CMD_TRY( fill_i32( (int *)N->offsets.p, 0, 2, st ) );
+        N->nnz = 0;
+        N->csr_valid = true;
+        return CMD_OK;
+    }
+    Scratch sc;
     int64_t nnz = 0;
-    CMD_TRY( scan_exclusive( N->counts.p, N->offsets.p, n_local, S->sc, st,
-                             &nnz ) );
-    int mr = 0;
-    CMD_HIP( hipMemcpyAsync( &mr, S->sc.ires.p, sizeof( int ),
-                             hipMemcpyDeviceToHost, st ) );
+    CMD_TRY( scan_exclusive( N->counts.p, N->offsets.p, n, sc, st, &nnz ) );
+    CMD_TRY( N->idx.reserve( (size_t)std::max<int64_t>( nnz, 1 ) ) );
+    const int nb = ( n + TPB - 1 ) / TPB;
+    switch ( N->lanes )
+    {
+    case 1:
+        k_slab_to_csr<1><<<nb, TPB, 0, st>>>( n, N->cap, N->counts.p, N->slab.p,
+                                              N->offsets.p, N->idx.p );
+        break;
+    case 2:
+        k_slab_to_csr<2><<<nb, TPB, 0, st>>>( n, N->cap, N->counts.p, N->slab.p,
+                                              N->offsets.p, N->idx.p );
+        break;
+    case 8:
+        k_slab_to_csr<8><<<nb, TPB, 0, st>>>( n, N->cap, N->counts.p, N->slab.p,
+                                              N->offsets.p, N->idx.p );
+        break;
+    default:
+        k_slab_to_csr<4><<<nb, TPB, 0, st>>>( n, N->cap, N->counts.p, N->slab.p,
+                                              N->offsets.p, N->idx.p );
+        break;
+    }
+    CMD_LAUNCH_CHECK();
     CMD_HIP( hipStreamSynchronize( st ) );
     N->nnz = nnz;
-    N->max_row = mr;
-    CMD_TRY( N->idx.reserve( (size_t)std::max<int64_t>( nnz, 1 ) +
-                             (size_t)( nnz / 10 ) ) );
-    // (5) fill pass
-    k_neigh_rows<true><<<ncell, 64, 0, st>>>(
-        N->xs.p, N->grid_sort.start.p, ng[0], ng[1], ng[2], n_local, rc2,
-        N->half, nullptr, N->offsets.p, N->idx.p, nullptr );
-    CMD_LAUNCH_CHECK();
-    // NeighborVerlet::create regrows its capacity guess (neighbor_verlet.h:58-61)
-    if ( N->max_row > N->max_neigh_guess )
-        N->max_neigh_guess = (int)( N->max_row * 1.1 );
+    N->csr_valid = true;
     return CMD_OK;
 }
 
@@ -341,7 +428,7 @@ cmd_status cmd_neigh_create( double neigh_cut, int half, int max_neigh_guess,
     cmd_neigh *n = new cmd_neigh;
     n->neigh_cut = neigh_cut;
     n->half = half ? 1 : 0;
-    n->max_neigh_guess = max_neigh_guess;
+    n->max_neigh_guess = std::max( 1, max_neigh_guess );
     *out = n;
     return CMD_OK;
 }
@@ -349,6 +436,20 @@ cmd_status cmd_neigh_create( double neigh_cut, int half, int max_neigh_guess,
 cmd_status cmd_neigh_destroy( cmd_neigh *n )
 {
     delete n;
+    return CMD_OK;
+}
+
+cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes )
+{
+    CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    CMD_REQUIRE( lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+                 CMD_ERR_ARG, "lanes per atom must be 1, 2, 4 or 8" );
+    if ( lanes != n->lanes )
+    {
+        n->lanes = lanes;
+        n->n_rows = 0; // layout changed: must rebuild
+        n->csr_valid = false;
+    }
     return CMD_OK;
 }
 
@@ -362,10 +463,13 @@ cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
                            int *max_row, int *max_neigh_guess )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    if ( nnz )
+    {
+        CMD_TRY( neigh_csr( const_cast<cmd_neigh *>( n ) ) );
+        *nnz = n->nnz;
+    }
     if ( n_rows )
         *n_rows = n->n_rows;
-    if ( nnz )
-        *nnz = n->nnz;
     if ( max_row )
         *max_row = n->max_row;
     if ( max_neigh_guess )
@@ -376,7 +480,9 @@ cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
 cmd_status cmd_neigh_download( cmd_neigh *n, int64_t *offsets, int *idx )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
-    CMD_REQUIRE( n->offsets.p, CMD_ERR_STATE, "neighbor list was never built" );
+    CMD_REQUIRE( n->stream || n->n_rows == 0, CMD_ERR_STATE,
+                 "neighbor list was never built" );
+    CMD_TRY( neigh_csr( n ) );
     if ( offsets )
         CMD_HIP( hipMemcpyAsync( offsets, n->offsets.p,
                                  sizeof( int64_t ) * ( (size_t)n->n_rows + 1 ),

@@ -62,6 +62,11 @@ int cmd_version( void );
 cmd_status cmd_device_init( int device );
 cmd_status cmd_device_count( int *count );
 cmd_status cmd_device_synchronize( void );
+/* Streaming-copy probe (no reference counterpart): copies `bytes` between two
+ * device buffers `reps` times; reports read+write GB/s and ms per copy.  Used
+ * to calibrate hardware byte counters and the achievable HBM bandwidth. */
+cmd_status cmd_bandwidth_probe( int64_t bytes, int reps, double *gbs,
+                                double *ms_per_copy );
 
 /* ---- domain --------------------------------------------------------------
  * System::create_domain (system.h:149-205): a periodic box of 100 mesh
@@ -146,6 +151,9 @@ typedef struct cmd_neigh cmd_neigh;
 cmd_status cmd_neigh_create( double neigh_cut, int half, int max_neigh_guess,
                              cmd_neigh **out );
 cmd_status cmd_neigh_destroy( cmd_neigh *n );
+/* Lanes per atom of the list's device layout (1, 2 or 4; default 1).  The
+ * force kernels follow the layout of the list they are given. */
+cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes );
 cmd_status cmd_neigh_build( cmd_neigh *n, cmd_system *s );
 cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
                            int *max_row, int *max_neigh_guess );
@@ -163,7 +171,8 @@ cmd_status cmd_force_destroy( cmd_force *f );
 cmd_status cmd_force_lj_init_coeff( cmd_force *f, int i, int j, double eps,
                                     double sigma, double cut );
 cmd_status cmd_force_set_precision( cmd_force *f, int prec );
-/* Lanes per atom in the full-list kernel (1,2,4,8,16); 0 = autotune. */
+/* Lanes per atom the timestep loop lays its list out for (1, 2, 4);
+ * 0 = autotune at init by timing each layout. */
 cmd_status cmd_force_set_lanes( cmd_force *f, int lanes );
 cmd_status cmd_force_get_lanes( const cmd_force *f, int *lanes );
 /* f = 0 over local+ghost, then accumulate (cabanamd_impl.h:336-341). */

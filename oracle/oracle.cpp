@@ -338,9 +338,8 @@ void orc_wrap( int n, double *x, const double L[3] )
  * cell = (max-min)/n, index = floor((x-min)/cell) clamped to n-1, cardinal id
  * (i*ny + j)*nz + k; the permutation is the counting sort by cell id, in
  * index order within a cell (the Serial-backend order). */
-int orc_bin_sort( int n, const double *x, const orc_domain *d,
-                  const double delta_in[3], int halo_depth, int *perm,
-                  int *cell_of )
+void orc_bin_grid( const orc_domain *d, const double delta_in[3],
+                   int halo_depth, double gmin[3], double rd[3], int ng[3] )
 {
     int nbin[3];
     double delta[3];
@@ -352,16 +351,34 @@ int orc_bin_sort( int n, const double *x, const orc_domain *d,
         delta[k] = d->local_mesh[k] / nbin[k];
     }
     const double eps = delta[0] / 1000;
-    double gmin[3], gmax[3], rd[3];
-    int ng[3];
     for ( int k = 0; k < 3; ++k )
     {
         gmin[k] = -delta[k] * halo_depth - eps + d->local_lo[k];
-        gmax[k] = delta[k] * halo_depth + eps + d->local_hi[k];
-        ng[k] = (int)std::floor( ( gmax[k] - gmin[k] ) * ( 1.0 / delta[k] ) );
-        double c = ( gmax[k] - gmin[k] ) / ng[k];
+        const double gmax = delta[k] * halo_depth + eps + d->local_hi[k];
+        ng[k] = (int)std::floor( ( gmax - gmin[k] ) * ( 1.0 / delta[k] ) );
+        double c = ( gmax - gmin[k] ) / ng[k];
         rd[k] = 1.0 / c;
     }
+}
+
+static inline int grid_cell_coord( double c, double mn, double rd, int n,
+                                   int floor_mode )
+{
+    const double t = ( c - mn ) * rd;
+    int ic = floor_mode ? (int)std::floor( t ) : (int)t;
+    if ( ic == n )
+        ic = n - 1;
+    /* out-of-grid coordinates (undefined in the reference) clamp */
+    return std::min( std::max( ic, 0 ), n - 1 );
+}
+
+int orc_bin_sort( int n, const double *x, const orc_domain *d,
+                  const double delta_in[3], int halo_depth, int *perm,
+                  int *cell_of )
+{
+    double gmin[3], rd[3];
+    int ng[3];
+    orc_bin_grid( d, delta_in, halo_depth, gmin, rd, ng );
     const int ncell = ng[0] * ng[1] * ng[2];
     std::vector<int> cid( n );
     std::vector<int> count( ncell + 1, 0 );
@@ -369,14 +386,7 @@ int orc_bin_sort( int n, const double *x, const orc_domain *d,
     {
         int c3[3];
         for ( int k = 0; k < 3; ++k )
-        {
-            int ic = (int)std::floor( ( x[3 * i + k] - gmin[k] ) * rd[k] );
-            if ( ic == ng[k] )
-                ic = ng[k] - 1;
-            /* out-of-grid coordinates (undefined in the reference) clamp */
-            ic = std::min( std::max( ic, 0 ), ng[k] - 1 );
-            c3[k] = ic;
-        }
+            c3[k] = grid_cell_coord( x[3 * i + k], gmin[k], rd[k], ng[k], 1 );
         cid[i] = ( c3[0] * ng[1] + c3[1] ) * ng[2] + c3[2];
         count[cid[i]]++;
     }
@@ -569,65 +579,24 @@ int64_t orc_neigh_brute( int n_local, int n_total, const double *x, double rc,
     return nnz;
 }
 
-/* Cell-list build: grid over the bounding box of all n_total atoms with cell
- * size >= rc, so a 27-cell stencil holds every candidate.  Rows are in
- * stencil order, index order within a cell. */
-int64_t orc_neigh_build( int n_local, int n_total, const double *x, double rc,
-                         int half, int64_t *offsets, int *idx )
+/* Cell-list build on a given grid: rows walk the 27-cell stencil in
+ * (x-cell, y-cell, z-cell) order, atoms of a cell in index order. */
+int64_t orc_neigh_build_grid( int n_local, int n_total, const double *x,
+                              double rc, int half, const double gmin[3],
+                              const double rd[3], const int ng[3],
+                              int floor_mode, int64_t *offsets, int *idx )
 {
     const double rc2 = rc * rc;
-    if ( n_total == 0 )
-    {
-        if ( offsets )
-            offsets[0] = 0;
-        return 0;
-    }
-    double lo[3], hi[3];
-    for ( int k = 0; k < 3; ++k )
-    {
-        lo[k] = std::numeric_limits<double>::max();
-        hi[k] = -std::numeric_limits<double>::max();
-    }
-    for ( int i = 0; i < n_total; ++i )
-        for ( int k = 0; k < 3; ++k )
-        {
-            lo[k] = std::min( lo[k], x[3 * i + k] );
-            hi[k] = std::max( hi[k], x[3 * i + k] );
-        }
-    /* cells >= rc per dimension, at most 1024 per dimension and at most
-     * max(64, 2 n_total) in total (halving the largest dimension) */
-    int ng[3];
-    double rcell[3], ext[3];
-    for ( int k = 0; k < 3; ++k )
-    {
-        ext[k] = hi[k] - lo[k];
-        ng[k] = std::max( 1, (int)std::floor( ext[k] / rc ) );
-        ng[k] = std::min( ng[k], 1024 );
-    }
-    const int64_t limit = std::max<int64_t>( 64, 2 * (int64_t)n_total );
-    while ( (int64_t)ng[0] * ng[1] * ng[2] > limit )
-    {
-        int m = 0;
-        if ( ng[1] > ng[m] )
-            m = 1;
-        if ( ng[2] > ng[m] )
-            m = 2;
-        ng[m] = std::max( 1, ng[m] / 2 );
-    }
-    for ( int k = 0; k < 3; ++k )
-        rcell[k] = ng[k] / std::max( ext[k], 1e-300 );
     const int ncell = ng[0] * ng[1] * ng[2];
     auto cell3 = [&]( int i, int c[3] )
     {
         for ( int k = 0; k < 3; ++k )
-        {
-            int ic = (int)( ( x[3 * i + k] - lo[k] ) * rcell[k] );
-            c[k] = std::min( std::max( ic, 0 ), ng[k] - 1 );
-        }
+            c[k] = grid_cell_coord( x[3 * i + k], gmin[k], rd[k], ng[k],
+                                    floor_mode );
     };
     std::vector<int> cnt( ncell + 1, 0 ), start( ncell + 1, 0 ),
-        sorted( n_total );
-    std::vector<int> cid( n_total );
+        sorted( std::max( n_total, 1 ) );
+    std::vector<int> cid( std::max( n_total, 1 ) );
     for ( int i = 0; i < n_total; ++i )
     {
         int c[3];
@@ -685,6 +654,67 @@ int64_t orc_neigh_build( int n_local, int n_total, const double *x, double rc,
             visit_row( i, idx + offsets[i] );
     }
     return nnz;
+}
+
+static void bbox_of( int n, const double *x, double lo[3], double hi[3] )
+{
+    for ( int k = 0; k < 3; ++k )
+    {
+        lo[k] = std::numeric_limits<double>::max();
+        hi[k] = -std::numeric_limits<double>::max();
+    }
+    for ( int i = 0; i < n; ++i )
+        for ( int k = 0; k < 3; ++k )
+        {
+            lo[k] = std::min( lo[k], x[3 * i + k] );
+            hi[k] = std::max( hi[k], x[3 * i + k] );
+        }
+}
+
+/* Bounding-box grid: cells >= rc per dimension, at most 1024 per dimension
+ * and at most max(64, 2 n_total) in total (halving the largest dimension). */
+static void bbox_grid( const double lo[3], const double hi[3], double rc,
+                       int n_total, double gmin[3], double rd[3], int ng[3] )
+{
+    double ext[3];
+    for ( int k = 0; k < 3; ++k )
+    {
+        ext[k] = hi[k] - lo[k];
+        ng[k] = std::max( 1, (int)std::floor( ext[k] / rc ) );
+        ng[k] = std::min( ng[k], 1024 );
+    }
+    const int64_t limit = std::max<int64_t>( 64, 2 * (int64_t)n_total );
+    while ( (int64_t)ng[0] * ng[1] * ng[2] > limit )
+    {
+        int m = 0;
+        if ( ng[1] > ng[m] )
+            m = 1;
+        if ( ng[2] > ng[m] )
+            m = 2;
+        ng[m] = std::max( 1, ng[m] / 2 );
+    }
+    for ( int k = 0; k < 3; ++k )
+    {
+        gmin[k] = lo[k];
+        rd[k] = ng[k] / std::max( ext[k], 1e-300 );
+    }
+}
+
+int64_t orc_neigh_build( int n_local, int n_total, const double *x, double rc,
+                         int half, int64_t *offsets, int *idx )
+{
+    if ( n_total == 0 )
+    {
+        if ( offsets )
+            offsets[0] = 0;
+        return 0;
+    }
+    double lo[3], hi[3], gmin[3], rd[3];
+    int ng[3];
+    bbox_of( n_total, x, lo, hi );
+    bbox_grid( lo, hi, rc, n_total, gmin, rd, ng );
+    return orc_neigh_build_grid( n_local, n_total, x, rc, half, gmin, rd, ng, 0,
+                                 offsets, idx );
 }
 
 /* ------------------------------------------------------------------------ */
@@ -912,15 +942,33 @@ struct orc_md
             grow( (int)( cap * 1.5 ) + 64 );
         }
     }
+    /* The engine's grid rule: the binning grid when its cells are >= rc and
+     * it holds every atom, otherwise a grid over the bounding box. */
     void neighbors()
     {
         int nt = n_local + n_ghost;
         off.resize( n_local + 1 );
-        int64_t nnz = orc_neigh_build( n_local, nt, x.data(), neigh_cut,
-                                       p.half, off.data(), nullptr );
+        double lo[3], hi[3], gmin[3], rd[3];
+        int ng[3], fm = 1;
+        bbox_of( nt, x.data(), lo, hi );
+        double delta[3] = { neigh_cut, neigh_cut, neigh_cut };
+        orc_bin_grid( &d, delta, 1, gmin, rd, ng );
+        bool use_bin = true;
+        for ( int k = 0; k < 3; ++k )
+            if ( !( 1.0 / rd[k] >= neigh_cut ) || !( lo[k] >= gmin[k] ) ||
+                 !( hi[k] < gmin[k] + ng[k] / rd[k] ) )
+                use_bin = false;
+        if ( !use_bin )
+        {
+            bbox_grid( lo, hi, neigh_cut, nt, gmin, rd, ng );
+            fm = 0;
+        }
+        int64_t nnz = orc_neigh_build_grid( n_local, nt, x.data(), neigh_cut,
+                                            p.half, gmin, rd, ng, fm, off.data(),
+                                            nullptr );
         idx.resize( std::max<int64_t>( nnz, 1 ) );
-        orc_neigh_build( n_local, nt, x.data(), neigh_cut, p.half, off.data(),
-                         idx.data() );
+        orc_neigh_build_grid( n_local, nt, x.data(), neigh_cut, p.half, gmin, rd,
+                              ng, fm, off.data(), idx.data() );
     }
     void sort()
     {

@@ -94,6 +94,10 @@ void orc_wrap( int n, double *x, const double L[3] );
 int orc_bin_sort( int n, const double *x, const orc_domain *d,
                   const double delta_in[3], int halo_depth, int *perm,
                   int *cell_of );
+/* The LinkedCellList grid create_binning sorts with: lower corner, reciprocal
+ * cell size and cell counts (floor-based cell index, top-edge clamp). */
+void orc_bin_grid( const orc_domain *d, const double delta_in[3],
+                   int halo_depth, double gmin[3], double rd[3], int ng[3] );
 /* Apply perm to x (n,3), v (n,3), type, id in place. */
 void orc_permute( int n, const int *perm, double *x, double *v, int *type,
                   int *id );
@@ -117,6 +121,14 @@ void orc_ghost_fold( int n_local, double *f, const int *send_idx,
  * call with idx == NULL to get nnz and offsets, then with idx to fill. */
 int64_t orc_neigh_build( int n_local, int n_total, const double *x, double rc,
                          int half, int64_t *offsets, int *idx );
+/* Same accept rule and row order, candidate search on a given grid
+ * (floor_mode 1: cell = floor((x - min) * rd) with the top-edge clamp, as the
+ * binning grid; 0: truncation).  The grid cells must be >= rc wide and hold
+ * every atom. */
+int64_t orc_neigh_build_grid( int n_local, int n_total, const double *x,
+                              double rc, int half, const double gmin[3],
+                              const double rd[3], const int ng[3],
+                              int floor_mode, int64_t *offsets, int *idx );
 /* N^2 brute force with the same accept rule (tstNeighbor.hpp:76-141). */
 int64_t orc_neigh_brute( int n_local, int n_total, const double *x, double rc,
                          int half, int64_t *offsets, int *idx );

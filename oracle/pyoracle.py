@@ -64,6 +64,7 @@ def lib():
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_neigh_build.restype = C.c_int64
         L.orc_neigh_brute.restype = C.c_int64
+        L.orc_neigh_build_grid.restype = C.c_int64
         L.orc_md_create.restype = C.c_void_p
         L.orc_md_destroy.argtypes = [C.c_void_p]
         L.orc_md_init.argtypes = [C.c_void_p]
@@ -200,6 +201,32 @@ def bin_sort(x, d, delta, halo_depth=1):
     nc = lib().orc_bin_sort(C.c_int(n), dp(x), C.byref(d), _d3(delta),
                             C.c_int(halo_depth), ip(perm), ip(cell))
     return perm, cell, nc
+
+
+def bin_grid(d, delta, halo_depth=1):
+    gmin = np.zeros(3)
+    rd = np.zeros(3)
+    ng = np.zeros(3, dtype=np.int32)
+    lib().orc_bin_grid(C.byref(d), _d3(delta), C.c_int(halo_depth), dp(gmin),
+                       dp(rd), ip(ng))
+    return gmin, rd, ng
+
+
+def neigh_build_grid(x, n_local, rc, half, gmin, rd, ng, floor_mode=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    gmin = np.ascontiguousarray(gmin, dtype=np.float64)
+    rd = np.ascontiguousarray(rd, dtype=np.float64)
+    ng = np.ascontiguousarray(ng, dtype=np.int32)
+    nt = x.shape[0]
+    off = np.zeros(n_local + 1, dtype=np.int64)
+    f = lib().orc_neigh_build_grid
+    nnz = f(C.c_int(n_local), C.c_int(nt), dp(x), C.c_double(rc),
+            C.c_int(int(half)), dp(gmin), dp(rd), ip(ng), C.c_int(floor_mode),
+            lp(off), None)
+    idx = np.zeros(max(nnz, 1), dtype=np.int32)
+    f(C.c_int(n_local), C.c_int(nt), dp(x), C.c_double(rc), C.c_int(int(half)),
+      dp(gmin), dp(rd), ip(ng), C.c_int(floor_mode), lp(off), ip(idx))
+    return off, idx[:nnz]
 
 
 def ghost_build(x_local, type_local, id_local, d, depth, cap=None):
