@@ -976,6 +976,7 @@ cmd_status cmd_system_upload( cmd_system *s, int n_local, int n_ghost,
     s->n_ghost = n_ghost;
     s->N = n_local;
     s->bin_grid_valid = false;
+    s->xg_valid = false;
     for ( int k = 0; k < 6; ++k )
         s->counts[k] = 0;
     return CMD_OK;
@@ -1059,6 +1060,7 @@ cmd_status cmd_nve_initial( cmd_system *s )
     if ( s->n_local == 0 )
         return CMD_OK;
     const double dtf = 0.5 * s->dt / s->mvv2e;
+    s->bin_grid_valid = false; // atoms leave the cells they were sorted into
     k_nve_initial<<<nblocks_for( s->n_local ), TPB, 0, s->stream>>>(
         s->n_local, s->cap, s->x.p, s->v.p, s->f.p, s->mass_table(), dtf,
         s->dt );
@@ -1083,6 +1085,8 @@ cmd_status cmd_comm_exchange( cmd_system *s )
     CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
     CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
     s->n_ghost = 0; // Comm::exchange resizes to N_local (comm_mpi_impl.h:270-271)
+    s->bin_grid_valid = false;
+    s->xg_valid = false;
     return wrap_local( s );
 }
 
@@ -1090,6 +1094,7 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
 {
     CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
     CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
+    s->xg_valid = false;
     const int n_local = s->n_local;
     int n_ghost = 0, send_total = 0;
     for ( int phase = 0; phase < 6; ++phase )
@@ -1140,7 +1145,7 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
 cmd_status cmd_comm_update_halo( cmd_system *s )
 {
     CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
-    return ghost_refresh( s );
+    return s->xg_valid ? tiles_ghost_refresh( s ) : ghost_refresh( s );
 }
 
 cmd_status cmd_comm_update_force( cmd_system *s )
@@ -1226,6 +1231,7 @@ cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
         s->n_ghost = 0; // ghosts are not part of the permuted range
         s->bin_grid = g;
         s->bin_grid_valid = true;
+        s->xg_valid = false;
     }
     if ( perm_out && n > 0 )
     {

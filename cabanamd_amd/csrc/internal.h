@@ -165,6 +165,16 @@ __device__ __forceinline__ int cell_coord( double c, double mn, double rd,
     return ic;
 }
 
+// Tiles of the binning grid: a tile is the run of cells [e0, e1) of one
+// interior column (a, b); ntiles = nc0 * nc1 * tpc.
+struct TileGrid
+{
+    CellGrid g;
+    int nc0, nc1; // interior columns: a in [1, n0-2], b in [1, n1-2]
+    int E, tpc;   // tile height in cells, tiles per column
+    int ntiles;
+};
+
 // Counting sort of n positions into grid cells, stable by index.  Produces
 // perm (sorted slot -> index) and start (ncell+1 cell offsets).
 struct CellSort
@@ -207,6 +217,11 @@ struct cmd_system
     // grid of the last sorting create_binning (valid until atoms change)
     cmd::CellGrid bin_grid{};
     bool bin_grid_valid = false;
+    // ghosts sorted into the binning grid (for tiled lists): gsort.perm maps
+    // a sorted slot to a ghost index; xg holds ghost positions in that order
+    cmd::CellSort gsort;
+    cmd::DevBuf<double4> xg;
+    bool xg_valid = false;
     cmd::Scratch sc;
 
     cmd::MassTable mass_table() const
@@ -239,6 +254,18 @@ struct cmd_neigh
     int ng[3] = { 0, 0, 0 };
     bool used_bin_grid = false;
     hipStream_t stream = nullptr; // stream of the last build
+
+    // Tiled layout (full lists on the binning grid, tiles.hip): rows hold
+    // 16-bit slots into the LDS copy of a tile's stencil neighbourhood.
+    bool allow_tiles = true;
+    bool tiled = false;          // the last build produced the tiled layout
+    int tile_e = 0;              // tile height in cells (0 = choose)
+    cmd::TileGrid tg{};
+    cmd::DevBuf<int> tile_info;  // TI_WORDS ints per tile
+    cmd::DevBuf<int64_t> tile_off; // slab offset of each tile (16-bit units)
+    cmd::DevBuf<unsigned short> tslab;
+    int max_tile_slots = 0;
+    const int *gperm_view = nullptr; // the system's ghost sort at build time
 };
 
 // LJ parameters passed by value to kernels.
@@ -305,4 +332,9 @@ cmd_status neigh_build( cmd_neigh *N, cmd_system *S );
 cmd_status neigh_csr( cmd_neigh *N );
 size_t slab_entries( int n_local, int cap, int lanes );
 cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N );
+// tiles.hip
+cmd_status tiles_prepare_ghosts( cmd_system *S );
+cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built );
+cmd_status tiles_csr( cmd_neigh *N, int64_t *nnz );
+cmd_status tiles_ghost_refresh( cmd_system *S );
 } // namespace cmd

@@ -14,6 +14,7 @@
 // flight per lane.  Block indices are remapped so each of the 8 XCDs walks a
 // contiguous range of cell-sorted atoms (its x_j gathers stay in its L2).
 #include "internal.h"
+#include "tiles_dev.h"
 
 #include <algorithm>
 #include <cmath>
@@ -192,33 +193,160 @@ __device__ __forceinline__ SlabLane slab_lane( int block )
 }
 
 // Row walk of one lane: neighbours k = r*G + s for r < nr at row[r*64]; the
-// body gets (j, x_j).  Four index loads then four gathers are issued before
-// any arithmetic.
-template <int G, class Body>
+// body gets (j, x_j).  Software-pipelined: the index loads of the next four
+// rows are issued before the gathers and arithmetic of the current four, so
+// the index stream's latency overlaps the gathers.  Index loads are
+// non-temporal: the stream is read once and must not evict the reused x_j
+// lines from L2.
+// Position gather: x, y, z only when the type slot is not needed (one atom
+// type), which saves a quarter of the address/data traffic per pair.
+template <bool XYZ_ONLY>
+__device__ __forceinline__ double4 gather_pos( const double4 *__restrict__ x,
+                                               int j )
+{
+    if ( XYZ_ONLY )
+    {
+        const double2 a = *reinterpret_cast<const double2 *>( x + j );
+        const double c = reinterpret_cast<const double *>( x + j )[2];
+        return make_double4( a.x, a.y, c, 0.0 );
+    }
+    return x[j];
+}
+
+template <int G, bool XYZ_ONLY, class Body>
 __device__ __forceinline__ void walk_row( const int *__restrict__ row, int nr,
                                           const double4 *__restrict__ x,
                                           Body body )
 {
-    int r = 0;
-    for ( ; r + 4 <= nr; r += 4 )
+    constexpr int U = 4;
+    int jn[U];
+#pragma unroll
+    for ( int u = 0; u < U; ++u )
+        jn[u] = ( u < nr ) ? __builtin_nontemporal_load( &row[u * 64] ) : -1;
+    for ( int r = 0; r < nr; r += U )
     {
-        const int j0 = row[( r + 0 ) * 64];
-        const int j1 = row[( r + 1 ) * 64];
-        const int j2 = row[( r + 2 ) * 64];
-        const int j3 = row[( r + 3 ) * 64];
-        const double4 p0 = x[j0];
-        const double4 p1 = x[j1];
-        const double4 p2 = x[j2];
-        const double4 p3 = x[j3];
-        body( j0, p0 );
-        body( j1, p1 );
-        body( j2, p2 );
-        body( j3, p3 );
+        int jc[U];
+#pragma unroll
+        for ( int u = 0; u < U; ++u )
+        {
+            jc[u] = jn[u];
+            const int rn = r + U + u;
+            jn[u] = ( rn < nr ) ? __builtin_nontemporal_load( &row[rn * 64] ) : -1;
+        }
+        double4 p[U];
+#pragma unroll
+        for ( int u = 0; u < U; ++u )
+            p[u] = gather_pos<XYZ_ONLY>( x, jc[u] >= 0 ? jc[u] : 0 );
+#pragma unroll
+        for ( int u = 0; u < U; ++u )
+            if ( jc[u] >= 0 )
+                body( jc[u], p[u] );
     }
-    for ( ; r < nr; ++r )
+}
+
+// After the force of atom i is known: store f, finish this step's
+// velocity (final half kick) and start the next one (half kick, drift, wrap)
+// -- integrator_nve.h:91-110 and comm_mpi.h:141-168 -- adding m v^2 to mv2.
+// With G >= 4 lanes per atom lane s handles component s; otherwise lane 0
+// does all three.  Both orders run the same operations on the same values.
+template <int G, bool FUSED>
+__device__ __forceinline__ void finish_atom( bool active, int s, int i,
+                                             const double4 &xi, double fx,
+                                             double fy, double fz, int scap,
+                                             double4 *__restrict__ x_next,
+                                             double *__restrict__ v,
+                                             double *__restrict__ f,
+                                             const MassTable &mt,
+                                             const StepCtl &ctl, double &mv2 )
+{
+    if ( G >= 4 )
     {
-        const int j = row[r * 64];
-        body( j, x[j] );
+        // every lane of the atom holds the totals: lane s finishes component
+        // s (same operations as the one-lane epilogue, so identical bits)
+        const int c = s;
+        if ( active && c < 4 )
+        {
+            const double fc = c == 0 ? fx : ( c == 1 ? fy : fz );
+            if ( c < 3 && ( !FUSED || ctl.write_f ) )
+                __builtin_nontemporal_store( fc, &f[c * scap + i] );
+            if ( FUSED && ctl.do_final )
+            {
+                double *xn = reinterpret_cast<double *>( x_next + i );
+                if ( c < 3 )
+                {
+                    const double m = mt.m[(int)xi.w];
+                    const double dtfm = ctl.dtf / m;
+                    double vc = v[c * scap + i] + dtfm * fc;
+                    mv2 += vc * vc * m;
+                    if ( ctl.do_next_initial )
+                    {
+                        vc = vc + dtfm * fc;
+                        const double xc = c == 0 ? xi.x : ( c == 1 ? xi.y : xi.z );
+                        double pc = xc + ctl.dtv * vc;
+                        if ( ctl.wrap_next )
+                        {
+                            const double q = pc;
+                            if ( q > ctl.L[c] )
+                                pc = q - ctl.L[c];
+                            if ( q < 0 )
+                                pc = q + ctl.L[c];
+                        }
+                        xn[c] = pc;
+                    }
+                    v[c * scap + i] = vc;
+                }
+                else if ( ctl.do_next_initial )
+                    xn[3] = xi.w;
+            }
+        }
+    }
+    else if ( active && s == 0 )
+    {
+        if ( !FUSED || ctl.write_f )
+        {
+            f[i] = fx;
+            f[scap + i] = fy;
+            f[2 * scap + i] = fz;
+        }
+        if ( FUSED && ctl.do_final )
+        {
+            const double m = mt.m[(int)xi.w];
+            const double dtfm = ctl.dtf / m;
+            double vx = v[i] + dtfm * fx;
+            double vy = v[scap + i] + dtfm * fy;
+            double vz = v[2 * scap + i] + dtfm * fz;
+            mv2 += ( vx * vx + vy * vy + vz * vz ) * m;
+            if ( ctl.do_next_initial )
+            {
+                vx = vx + dtfm * fx;
+                vy = vy + dtfm * fy;
+                vz = vz + dtfm * fz;
+                double4 p = xi;
+                p.x = p.x + ctl.dtv * vx;
+                p.y = p.y + ctl.dtv * vy;
+                p.z = p.z + ctl.dtv * vz;
+                if ( ctl.wrap_next )
+                {
+                    const double cx = p.x, cy = p.y, cz = p.z;
+                    if ( cx > ctl.L[0] )
+                        p.x = cx - ctl.L[0];
+                    if ( cx < 0 )
+                        p.x = cx + ctl.L[0];
+                    if ( cy > ctl.L[1] )
+                        p.y = cy - ctl.L[1];
+                    if ( cy < 0 )
+                        p.y = cy + ctl.L[1];
+                    if ( cz > ctl.L[2] )
+                        p.z = cz - ctl.L[2];
+                    if ( cz < 0 )
+                        p.z = cz + ctl.L[2];
+                }
+                x_next[i] = p;
+            }
+            v[i] = vx;
+            v[scap + i] = vy;
+            v[2 * scap + i] = vz;
+        }
     }
 }
 
@@ -260,7 +388,7 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
                                         : lj_pair<PREC, false>( rsq, c, pe, vir );
             accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
         };
-        walk_row<G>( row, nr, x, body );
+        walk_row<G, ONE_TYPE>( row, nr, x, body );
     }
 #pragma unroll
     for ( int o = APW; o < 64; o <<= 1 )
@@ -270,53 +398,93 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
         fz += __shfl_xor( fz, o, 64 );
     }
     double mv2 = 0.0;
-    if ( active && L.s == 0 )
+    finish_atom<G, FUSED>( active, L.s, i, xi, fx, fy, fz, scap, x_next, v, f, mt,
+                           ctl, mv2 );
+    if ( energy )
+        block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
+}
+
+// Tiled full list (tiles.hip): the block stages its tile's stencil
+// neighbourhood in LDS and walks 16-bit slots, four rows per 8-byte load.
+template <int G, int PREC, bool ONE_TYPE, bool FUSED>
+__global__ __launch_bounds__( TPB ) void k_tile_force(
+    TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
+    int lcap, int scap, const double4 *__restrict__ x,
+    const double4 *__restrict__ xg, double4 *__restrict__ x_next,
+    double *__restrict__ v, double *__restrict__ f,
+    const int *__restrict__ counts, const unsigned short *__restrict__ slab,
+    const LJParams *__restrict__ Pd, PairCoef one, MassTable mt, StepCtl ctl,
+    double *__restrict__ partial )
+{
+    extern __shared__ double4 s_pos[];
+    __shared__ TileHead H;
+    constexpr int APW = 64 / G;
+    const int t = xcd_tile( blockIdx.x, gridDim.x );
+    load_head( info, t, H );
+    stage_positions( H, x, xg, s_pos );
+    const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
+    const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
+    const int nw = woff[t + 1] - woff[t];
+    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
+    const int self0 = H.segoff[8] - H.info[TI_LB + 4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int a = lane % APW, sub = lane / APW;
+    const bool energy = FUSED && ctl.thermo;
+    double pe = 0.0, vir = 0.0, mv2 = 0.0;
+    for ( int w = wid; w < nw; w += TPB / 64 )
     {
-        if ( !FUSED || ctl.write_f )
+        const int ta = w * APW + a;
+        const bool active = ta < nat;
+        const int i = tl + ta;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double4 xi = make_double4( 0.0, 0.0, 0.0, 0.0 );
+        if ( active )
         {
-            f[i] = fx;
-            f[scap + i] = fy;
-            f[2 * scap + i] = fz;
-        }
-        if ( FUSED && ctl.do_final )
-        {
-            const double m = mt.m[(int)xi.w];
-            const double dtfm = ctl.dtf / m;
-            double vx = v[i] + dtfm * fx;
-            double vy = v[scap + i] + dtfm * fy;
-            double vz = v[2 * scap + i] + dtfm * fz;
-            mv2 = ( vx * vx + vy * vy + vz * vz ) * m;
-            if ( ctl.do_next_initial )
+            xi = s_pos[self0 + i];
+            const int ti = (int)xi.w;
+            const int n = counts[i];
+            const int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
+            const unsigned short *row = st + ( (size_t)w * R4 * 64 + lane ) * 4;
+            auto body = [&]( int sj )
             {
-                vx = vx + dtfm * fx;
-                vy = vy + dtfm * fy;
-                vz = vz + dtfm * fz;
-                double4 p = xi;
-                p.x = p.x + ctl.dtv * vx;
-                p.y = p.y + ctl.dtv * vy;
-                p.z = p.z + ctl.dtv * vz;
-                if ( ctl.wrap_next )
-                {
-                    const double cx = p.x, cy = p.y, cz = p.z;
-                    if ( cx > ctl.L[0] )
-                        p.x = cx - ctl.L[0];
-                    if ( cx < 0 )
-                        p.x = cx + ctl.L[0];
-                    if ( cy > ctl.L[1] )
-                        p.y = cy - ctl.L[1];
-                    if ( cy < 0 )
-                        p.y = cy + ctl.L[1];
-                    if ( cz > ctl.L[2] )
-                        p.z = cz - ctl.L[2];
-                    if ( cz < 0 )
-                        p.z = cz + ctl.L[2];
-                }
-                x_next[i] = p;
+                const double4 xj = s_pos[sj];
+                const double dx = xi.x - xj.x;
+                const double dy = xi.y - xj.y;
+                const double dz = xi.z - xj.z;
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
+                const double fpair = energy ? lj_pair<PREC, true>( rsq, c, pe, vir )
+                                            : lj_pair<PREC, false>( rsq, c, pe, vir );
+                accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
+            };
+            const unsigned long long *row8 =
+                reinterpret_cast<const unsigned long long *>( row );
+            unsigned long long nxt = 0ull;
+            if ( nr > 0 )
+                nxt = __builtin_nontemporal_load( row8 );
+            for ( int r = 0; r < nr; r += 4 )
+            {
+                const unsigned long long cur = nxt;
+                if ( r + 4 < nr )
+                    nxt = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 1 ) * 64 );
+                body( (int)( cur & 0xffffull ) );
+                if ( r + 1 < nr )
+                    body( (int)( ( cur >> 16 ) & 0xffffull ) );
+                if ( r + 2 < nr )
+                    body( (int)( ( cur >> 32 ) & 0xffffull ) );
+                if ( r + 3 < nr )
+                    body( (int)( cur >> 48 ) );
             }
-            v[i] = vx;
-            v[scap + i] = vy;
-            v[2 * scap + i] = vz;
         }
+#pragma unroll
+        for ( int o = APW; o < 64; o <<= 1 )
+        {
+            fx += __shfl_xor( fx, o, 64 );
+            fy += __shfl_xor( fy, o, 64 );
+            fz += __shfl_xor( fz, o, 64 );
+        }
+        finish_atom<G, FUSED>( active, sub, i, xi, fx, fy, fz, scap, x_next, v,
+                               f, mt, ctl, mv2 );
     }
     if ( energy )
         block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
@@ -368,7 +536,7 @@ __global__ __launch_bounds__( TPB ) void k_lj_half(
             pe += ( ( j < n_local ) ? 1.0 : 0.5 ) * ep;
             vir += wp;
         };
-        walk_row<G>( row, nr, x, body );
+        walk_row<G, ONE_TYPE>( row, nr, x, body );
         unsafeAtomicAdd( &f[i], fx );
         unsafeAtomicAdd( &f[scap + i], fy );
         unsafeAtomicAdd( &f[2 * scap + i], fz );
@@ -400,6 +568,43 @@ __global__ __launch_bounds__( TPB ) void k_lj_energy(
         for ( int r = 0; r < nr; ++r )
         {
             const int j = row[r * 64];
+            const double4 xj = x[j];
+            const double dx = xi.x - xj.x;
+            const double dy = xi.y - xj.y;
+            const double dz = xi.z - xj.z;
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
+            if ( rsq < c.cs )
+            {
+                const double fac = half ? ( ( j < n_local ) ? 1.0 : 0.5 ) : 0.5;
+                const double r2inv = 1.0 / rsq;
+                const double r6inv = r2inv * r2inv * r2inv;
+                pe += fac * r6inv * ( 0.5 * c.lj1 * r6inv - c.lj2 ) / 6.0;
+                pe -= fac * c.esh;
+                const double fpair = ( r6inv * ( c.lj1 * r6inv - c.lj2 ) ) * r2inv;
+                vir += ( half ? 1.0 : 0.5 ) * rsq * fpair;
+            }
+        }
+    }
+    block_sum4( 0.0, pe, vir, 0.0, partial + 4 * blockIdx.x );
+}
+
+// compute_energy over a CSR view (used for tiled lists).
+template <bool ONE_TYPE>
+__global__ __launch_bounds__( TPB ) void k_lj_energy_csr(
+    int n_local, const double4 *__restrict__ x, const int64_t *__restrict__ off,
+    const int *__restrict__ idx, const LJParams *__restrict__ Pd, PairCoef one,
+    int half, double *__restrict__ partial )
+{
+    const int i = blockIdx.x * TPB + (int)threadIdx.x;
+    double pe = 0.0, vir = 0.0;
+    if ( i < n_local )
+    {
+        const double4 xi = x[i];
+        const int ti = (int)xi.w;
+        for ( int64_t k = off[i]; k < off[i + 1]; ++k )
+        {
+            const int j = idx[k];
             const double4 xj = x[j];
             const double dx = xi.x - xj.x;
             const double dy = xi.y - xj.y;
@@ -499,6 +704,66 @@ static cmd_status launch_full_g( int G, int nb, hipStream_t st, int n, int scap,
     return CMD_OK;
 }
 
+template <int G, int PREC, bool ONE, bool FUSED>
+static void launch_tile_t( cmd_neigh *N, cmd_system *S, double4 *xn,
+                           const LJParams *Pd, const PairCoef &c,
+                           const MassTable &mt, const StepCtl &ctl,
+                           double *partial )
+{
+    const size_t lds = (size_t)std::max( N->max_tile_slots, 1 ) * sizeof( double4 );
+    k_tile_force<G, PREC, ONE, FUSED><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, S->x.p,
+        S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt, ctl,
+        partial );
+}
+
+template <int PREC, bool ONE, bool FUSED>
+static cmd_status launch_tile_g( cmd_neigh *N, cmd_system *S, double4 *xn,
+                                 const LJParams *Pd, const PairCoef &c,
+                                 const MassTable &mt, const StepCtl &ctl,
+                                 double *partial )
+{
+    switch ( N->lanes )
+    {
+    case 2:
+        launch_tile_t<2, PREC, ONE, FUSED>( N, S, xn, Pd, c, mt, ctl, partial );
+        break;
+    case 4:
+        launch_tile_t<4, PREC, ONE, FUSED>( N, S, xn, Pd, c, mt, ctl, partial );
+        break;
+    case 8:
+        launch_tile_t<8, PREC, ONE, FUSED>( N, S, xn, Pd, c, mt, ctl, partial );
+        break;
+    default:
+        set_error( "tiled lists need 2, 4 or 8 lanes per atom" );
+        return CMD_ERR_ARG;
+    }
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+static cmd_status launch_tile( int prec, bool one, bool fused,
+                               const cmd_neigh *Nc, cmd_system *S, double4 *xn,
+                               const LJParams *Pd, const PairCoef &c,
+                               const MassTable &mt, const StepCtl &ctl,
+                               double *partial )
+{
+    cmd_neigh *N = const_cast<cmd_neigh *>( Nc );
+    if ( prec == CMD_PREC_FP64 )
+    {
+        if ( one )
+            return fused ? launch_tile_g<CMD_PREC_FP64, true, true>( N, S, xn, Pd, c, mt, ctl, partial )
+                         : launch_tile_g<CMD_PREC_FP64, true, false>( N, S, xn, Pd, c, mt, ctl, partial );
+        return fused ? launch_tile_g<CMD_PREC_FP64, false, true>( N, S, xn, Pd, c, mt, ctl, partial )
+                     : launch_tile_g<CMD_PREC_FP64, false, false>( N, S, xn, Pd, c, mt, ctl, partial );
+    }
+    if ( one )
+        return fused ? launch_tile_g<CMD_PREC_MIXED, true, true>( N, S, xn, Pd, c, mt, ctl, partial )
+                     : launch_tile_g<CMD_PREC_MIXED, true, false>( N, S, xn, Pd, c, mt, ctl, partial );
+    return fused ? launch_tile_g<CMD_PREC_MIXED, false, true>( N, S, xn, Pd, c, mt, ctl, partial )
+                 : launch_tile_g<CMD_PREC_MIXED, false, false>( N, S, xn, Pd, c, mt, ctl, partial );
+}
+
 cmd_status lj_full( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
                     const StepCtl &ctl, double4 *x_next, double *partial,
                     int *nblocks_out )
@@ -521,6 +786,15 @@ cmd_status lj_full( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
         CMD_TRY( upload_params( F, S->stream, &Pd ) );
     const bool fused = ( ctl.do_final || ctl.thermo );
     const MassTable mt = S->mass_table();
+    if ( N->tiled )
+    {
+        CMD_REQUIRE( S->xg_valid, CMD_ERR_STATE,
+                     "tiled list needs the sorted ghosts of its build" );
+        if ( nblocks_out )
+            *nblocks_out = N->tg.ntiles;
+        return launch_tile( F->prec, one, fused, N, S, x_next, Pd, c, mt, ctl,
+                            partial );
+    }
 #define CMD_FULL( PREC, ONE, FUS )                                             \
     return launch_full_g<PREC, ONE, FUS>( G, nb, S->stream, n, S->cap, N->cap, \
                                           S->x.p, x_next, S->v.p, S->f.p,      \
@@ -624,9 +898,9 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
 {
     const int n = S->n_local;
     const int G = N->lanes;
-    const int nb = blocks_for( n, G );
+    int nb = blocks_for( n, G );
     CMD_REQUIRE( N->n_rows == n, CMD_ERR_STATE, "neighbor list is stale" );
-    CMD_TRY( S->sc.dpart.reserve( 4 * (size_t)nb ) );
+    CMD_TRY( S->sc.dpart.reserve( 4 * (size_t)std::max( nb, blocks_for( n, 1 ) ) ) );
     CMD_TRY( S->sc.dres.reserve( 8 ) );
     const LJParams P = lj_params( F );
     const PairCoef c = one_coef( P );
@@ -634,6 +908,19 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
     const bool one = F->ntypes == 1;
     if ( !one )
         CMD_TRY( upload_params( F, S->stream, &Pd ) );
+    if ( N->tiled )
+    {
+        CMD_TRY( neigh_csr( const_cast<cmd_neigh *>( N ) ) );
+        nb = blocks_for( n, 1 );
+        if ( one )
+            k_lj_energy_csr<true><<<nb, TPB, 0, S->stream>>>(
+                n, S->x.p, N->offsets.p, N->idx.p, Pd, c, N->half, S->sc.dpart.p );
+        else
+            k_lj_energy_csr<false><<<nb, TPB, 0, S->stream>>>(
+                n, S->x.p, N->offsets.p, N->idx.p, Pd, c, N->half, S->sc.dpart.p );
+    }
+    else
+    {
 #define CMD_EN( GG )                                                          \
     if ( one )                                                                \
         k_lj_energy<GG, true><<<nb, TPB, 0, S->stream>>>(                     \
@@ -660,6 +947,7 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
         CMD_EN( 8 );
     }
 #undef CMD_EN
+    }
     CMD_LAUNCH_CHECK();
     CMD_TRY( reduce_partials( S->sc.dpart.p, nb, S->sc.dres.p, S->stream ) );
     double h[4];
@@ -673,25 +961,33 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
     return CMD_OK;
 }
 
-// Build the list in each layout (1, 2, 4, 8 lanes per atom), time the force
-// kernel on it and keep the fastest; the list is left built in that layout.
+// Build the list in each layout -- lanes per atom 1, 2, 4, 8 and, for the
+// LDS-tiled layout, tile heights of 4 and 6 cells -- time the force kernel
+// on it and keep the fastest; the list is left built in that layout.
 cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
 {
-    const int cands[4] = { 1, 2, 4, 8 };
+    struct Cand
+    {
+        int G, E;
+    };
+    const Cand cands[] = { { 1, 0 }, { 2, 4 }, { 2, 6 }, { 4, 4 },
+                           { 4, 6 }, { 8, 4 }, { 8, 6 } };
     StepCtl ctl{};
     hipEvent_t e0, e1;
     CMD_HIP( hipEventCreate( &e0 ) );
     CMD_HIP( hipEventCreate( &e1 ) );
     float best = 1e30f;
-    int best_g = 1;
+    Cand bestc = { 1, 0 };
     cmd_status st = CMD_OK;
-    for ( int c = 0; c < 4 && st == CMD_OK; ++c )
+    for ( const Cand &cd : cands )
     {
-        const int G = cands[c];
-        N->lanes = G;
+        N->lanes = cd.G;
+        N->tile_e = cd.E;
         st = neigh_build( N, S );
         if ( st != CMD_OK )
             break;
+        if ( cd.E > 0 && !N->tiled )
+            continue; // the tiled layout does not apply here
         st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
         if ( st != CMD_OK )
             break;
@@ -703,25 +999,22 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
         float ms = 0.0f;
         (void)hipEventElapsedTime( &ms, e0, e1 );
         if ( std::getenv( "CMD_VERBOSE" ) )
-            std::fprintf( stderr, "# autotune lanes=%d: %.4f ms/launch\n", G,
-                          ms / 3.0f );
+            std::fprintf( stderr, "# autotune lanes=%d tile=%d%s: %.4f ms/launch\n",
+                          cd.G, cd.E, N->tiled ? " (lds)" : "", ms / 3.0f );
         if ( ms < best )
         {
             best = ms;
-            best_g = G;
+            bestc = cd;
         }
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
     CMD_TRY( st );
-    F->lanes = best_g;
+    F->lanes = bestc.G;
     F->tuned = true;
-    if ( N->lanes != best_g )
-    {
-        N->lanes = best_g;
-        CMD_TRY( neigh_build( N, S ) );
-    }
-    return CMD_OK;
+    N->lanes = bestc.G;
+    N->tile_e = bestc.E;
+    return neigh_build( N, S );
 }
 
 cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N )

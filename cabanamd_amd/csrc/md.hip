@@ -190,7 +190,7 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         else
         {
             int k = md->begin( PH_COMM );
-            CMD_TRY( ghost_refresh( S ) );
+            CMD_TRY( cmd_comm_update_halo( S ) );
             md->end( k );
         }
         const bool th = p.thermo_rate > 0 && ( s % p.thermo_rate == 0 );
@@ -202,7 +202,8 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         ctl.write_f = 1;
         const int n = S->n_local;
         const int G = N->lanes;
-        const int nbA = std::max( 1, ( n + 255 ) / 256 * G );
+        const int nbA = std::max( N->tiled ? N->tg.ntiles : 1,
+                                  ( n + 255 ) / 256 * G );
         CMD_TRY( md->partA.reserve( 4 * (size_t)nbA + 4 ) );
         CMD_TRY( md->partB.reserve( 4 * (size_t)( ( n + 255 ) / 256 + 1 ) ) );
         CMD_TRY( S->x_alt.reserve( S->cap ) );

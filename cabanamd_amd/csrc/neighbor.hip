@@ -282,6 +282,7 @@ cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
     const int nt = S->n_local + S->n_ghost;
     N->n_rows = n_local;
     N->csr_valid = false;
+    N->tiled = false;
     CMD_TRY( N->counts.reserve( std::max( n_local, 1 ) ) );
     if ( nt == 0 || n_local == 0 )
     {
@@ -318,6 +319,20 @@ cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
                 use_bin = false;
         }
     }
+    if ( use_bin && N->allow_tiles && !N->half && N->lanes >= 2 )
+    {
+        if ( !S->xg_valid )
+            CMD_TRY( tiles_prepare_ghosts( S ) );
+        bool built = false;
+        CMD_TRY( tiles_build( N, S, &built ) );
+        if ( built )
+        {
+            N->used_bin_grid = true;
+            N->nnz = -1;
+            return CMD_OK;
+        }
+    }
+    N->tiled = false;
     if ( use_bin )
         g = S->bin_grid;
     else
@@ -388,6 +403,14 @@ cmd_status neigh_csr( cmd_neigh *N )
     CMD_TRY( scan_exclusive( N->counts.p, N->offsets.p, n, sc, st, &nnz ) );
     CMD_TRY( N->idx.reserve( (size_t)std::max<int64_t>( nnz, 1 ) ) );
     const int nb = ( n + TPB - 1 ) / TPB;
+    if ( N->tiled )
+    {
+        CMD_TRY( tiles_csr( N, nullptr ) );
+        CMD_HIP( hipStreamSynchronize( st ) );
+        N->nnz = nnz;
+        N->csr_valid = true;
+        return CMD_OK;
+    }
     switch ( N->lanes )
     {
     case 1:

@@ -1,0 +1,94 @@
+// tiles_dev.h -- device helpers shared by the tiled list build (tiles.hip)
+// and the tiled force kernel (lj.hip).  See tiles.hip for the layout.
+#pragma once
+
+#include "internal.h"
+
+namespace cmd
+{
+static constexpr int TI_WORDS = 44;
+enum
+{
+    TI_LB = 0,     // [9] first local index of each stencil column run
+    TI_LLEN = 9,   // [9] local run lengths
+    TI_GB = 18,    // [9] first sorted-ghost slot of each column run
+    TI_GLEN = 27,  // [9] ghost run lengths
+    TI_TL = 36,    // first local atom of the tile
+    TI_NAT = 37,   // atoms in the tile
+    TI_TOTAL = 38, // staged positions
+    TI_EA = 39,    // first staged cell (e0 - 1, clamped)
+    TI_NST = 40    // staged cells per column
+};
+
+__device__ __forceinline__ void tile_coords( const TileGrid &T, int t, int &a,
+                                             int &b, int &e0, int &e1 )
+{
+    const int col = t / T.tpc, z = t % T.tpc;
+    a = 1 + col / T.nc1;
+    b = 1 + col % T.nc1;
+    e0 = 1 + z * T.E;
+    e1 = min( e0 + T.E, T.g.n[2] - 1 );
+}
+
+__device__ __forceinline__ int col_base( const TileGrid &T, int a, int b )
+{
+    return ( a * T.g.n[1] + b ) * T.g.n[2];
+}
+
+// Round-robin dispatch puts block b on XCD b % 8; give XCD k the k-th
+// contiguous slice of tiles.
+__device__ __forceinline__ int xcd_tile( int b, int nb )
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, k = b >> 3;
+    return ( xcd < r ) ? xcd * ( q + 1 ) + k : r * ( q + 1 ) + ( xcd - r ) * q + k;
+}
+
+// Shared tile header: info words and the 19 segment offsets
+// (segment 2q = locals of column q, 2q+1 = its ghosts).
+struct TileHead
+{
+    int info[TI_WORDS];
+    int segoff[19];
+};
+
+__device__ inline void load_head( const int *__restrict__ info, int t, TileHead &H )
+{
+    const int *I = info + (size_t)t * TI_WORDS;
+    for ( int k = threadIdx.x; k < TI_WORDS; k += blockDim.x )
+        H.info[k] = I[k];
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        int off = 0;
+        for ( int q = 0; q < 9; ++q )
+        {
+            H.segoff[2 * q] = off;
+            off += H.info[TI_LLEN + q];
+            H.segoff[2 * q + 1] = off;
+            off += H.info[TI_GLEN + q];
+        }
+        H.segoff[18] = off;
+    }
+    __syncthreads();
+}
+
+// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS.
+__device__ inline void stage_positions( const TileHead &H, const double4 *__restrict__ x,
+                                 const double4 *__restrict__ xg,
+                                 double4 *__restrict__ s_pos )
+{
+    const int total = H.segoff[18];
+    int seg = 0;
+    for ( int k = threadIdx.x; k < total; k += blockDim.x )
+    {
+        while ( k >= H.segoff[seg + 1] )
+            ++seg;
+        const int q = seg >> 1, off = k - H.segoff[seg];
+        s_pos[k] = ( seg & 1 ) ? xg[H.info[TI_GB + q] + off]
+                               : x[H.info[TI_LB + q] + off];
+    }
+    __syncthreads();
+}
+
+} // namespace cmd
