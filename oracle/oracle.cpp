@@ -23,6 +23,8 @@
 #include <omp.h>
 #endif
 
+static const int RED_CHUNK = 4096; /* atoms per deterministic partial sum */
+
 namespace
 {
 using clk = std::chrono::steady_clock;
@@ -812,30 +814,40 @@ double orc_lj_energy( int n_local, const double *x, const int *type,
                       const double *cutsq, const int64_t *offsets,
                       const int *idx, int half )
 {
-    double pe = 0.0;
-#pragma omp parallel for schedule( static ) reduction( + : pe )
-    for ( int i = 0; i < n_local; ++i )
+    // fixed chunks summed in order: the same result for any thread count
+    const int nchunk = ( n_local + RED_CHUNK - 1 ) / RED_CHUNK;
+    std::vector<double> part( nchunk, 0.0 );
+#pragma omp parallel for schedule( static )
+    for ( int c = 0; c < nchunk; ++c )
     {
-        for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
-        {
-            const int j = idx[k];
-            const double dx = x[3 * i] - x[3 * j];
-            const double dy = x[3 * i + 1] - x[3 * j + 1];
-            const double dz = x[3 * i + 2] - x[3 * j + 2];
-            const int t = type[i] * ntypes + type[j];
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            if ( rsq < cutsq[t] )
+        double pe = 0.0;
+        const int i1 = std::min( n_local, ( c + 1 ) * RED_CHUNK );
+        for ( int i = c * RED_CHUNK; i < i1; ++i )
+            for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
             {
-                const double fac = half ? ( ( j < n_local ) ? 1.0 : 0.5 ) : 0.5;
-                double r2inv = 1.0 / rsq;
-                double r6inv = r2inv * r2inv * r2inv;
-                pe += fac * r6inv * ( 0.5 * lj1[t] * r6inv - lj2[t] ) / 6.0;
-                double r2invc = 1.0 / cutsq[t];
-                double r6invc = r2invc * r2invc * r2invc;
-                pe -= fac * r6invc * ( 0.5 * lj1[t] * r6invc - lj2[t] ) / 6.0;
+                const int j = idx[k];
+                const double dx = x[3 * i] - x[3 * j];
+                const double dy = x[3 * i + 1] - x[3 * j + 1];
+                const double dz = x[3 * i + 2] - x[3 * j + 2];
+                const int t = type[i] * ntypes + type[j];
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                if ( rsq < cutsq[t] )
+                {
+                    const double fac =
+                        half ? ( ( j < n_local ) ? 1.0 : 0.5 ) : 0.5;
+                    double r2inv = 1.0 / rsq;
+                    double r6inv = r2inv * r2inv * r2inv;
+                    pe += fac * r6inv * ( 0.5 * lj1[t] * r6inv - lj2[t] ) / 6.0;
+                    double r2invc = 1.0 / cutsq[t];
+                    double r6invc = r2invc * r2invc * r2invc;
+                    pe -= fac * r6invc * ( 0.5 * lj1[t] * r6invc - lj2[t] ) / 6.0;
+                }
             }
-        }
+        part[c] = pe;
     }
+    double pe = 0.0;
+    for ( int c = 0; c < nchunk; ++c )
+        pe += part[c];
     return pe;
 }
 
@@ -844,28 +856,36 @@ double orc_lj_virial( int n_local, const double *x, const int *type,
                       const double *cutsq, const int64_t *offsets,
                       const int *idx, int half )
 {
-    double w = 0.0;
     const double fac = half ? 1.0 : 0.5;
-#pragma omp parallel for schedule( static ) reduction( + : w )
-    for ( int i = 0; i < n_local; ++i )
+    const int nchunk = ( n_local + RED_CHUNK - 1 ) / RED_CHUNK;
+    std::vector<double> part( nchunk, 0.0 );
+#pragma omp parallel for schedule( static )
+    for ( int c = 0; c < nchunk; ++c )
     {
-        for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
-        {
-            const int j = idx[k];
-            const double dx = x[3 * i] - x[3 * j];
-            const double dy = x[3 * i + 1] - x[3 * j + 1];
-            const double dz = x[3 * i + 2] - x[3 * j + 2];
-            const int t = type[i] * ntypes + type[j];
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            if ( rsq < cutsq[t] )
+        double w = 0.0;
+        const int i1 = std::min( n_local, ( c + 1 ) * RED_CHUNK );
+        for ( int i = c * RED_CHUNK; i < i1; ++i )
+            for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
             {
-                double r2inv = 1.0 / rsq;
-                double r6inv = r2inv * r2inv * r2inv;
-                double fpair = ( r6inv * ( lj1[t] * r6inv - lj2[t] ) ) * r2inv;
-                w += fac * rsq * fpair;
+                const int j = idx[k];
+                const double dx = x[3 * i] - x[3 * j];
+                const double dy = x[3 * i + 1] - x[3 * j + 1];
+                const double dz = x[3 * i + 2] - x[3 * j + 2];
+                const int t = type[i] * ntypes + type[j];
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                if ( rsq < cutsq[t] )
+                {
+                    double r2inv = 1.0 / rsq;
+                    double r6inv = r2inv * r2inv * r2inv;
+                    double fpair = ( r6inv * ( lj1[t] * r6inv - lj2[t] ) ) * r2inv;
+                    w += fac * rsq * fpair;
+                }
             }
-        }
+        part[c] = w;
     }
+    double w = 0.0;
+    for ( int c = 0; c < nchunk; ++c )
+        w += part[c];
     return w;
 }
 
