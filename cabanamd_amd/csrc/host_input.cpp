@@ -602,13 +602,23 @@ cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
         v[3 * i + 1] -= sy;
         v[3 * i + 2] -= sz;
     }
+    // sum m v^2 with Neumaier compensation (the reference's device reduction
+    // has no fixed order; the compensated sum is within an ulp of exact)
+    double comp = 0.0;
     for ( int i = 0; i < count; ++i )
     {
         const int t = type ? type[i] : 0;
-        mv2 += ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
-                 v[3 * i + 2] * v[3 * i + 2] ) *
-               p->mass[t];
+        const double term = ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                              v[3 * i + 2] * v[3 * i + 2] ) *
+                            p->mass[t];
+        const double sum = mv2 + term;
+        if ( std::fabs( mv2 ) >= std::fabs( term ) )
+            comp += ( mv2 - sum ) + term;
+        else
+            comp += ( term - sum ) + mv2;
+        mv2 = sum;
     }
+    mv2 = mv2 + comp;
     const int dof = 3 * count - 3;
     const double T = mv2 * ( p->mvv2e / ( 1.0 * dof * p->boltz ) );
     for ( int i = 0; i < count; ++i )

@@ -110,7 +110,7 @@ __device__ __forceinline__ double lj_pair( double rsq, const PairCoef &c,
         const float r2inv = __builtin_amdgcn_rcpf( (float)rsq );
         const float r6inv = r2inv * r2inv * r2inv;
         const float fp = ( r6inv * __builtin_fmaf( c.lj1f, r6inv, -c.lj2f ) ) * r2inv;
-        const double fpair = in ? (double)fp : 0.0;
+        const double fpair = (double)( in ? fp : 0.0f );
         if ( ENERGY )
         {
             const float ep =
@@ -406,10 +406,14 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
 
 // Tiled full list (tiles.hip): the block stages its tile's stencil
 // neighbourhood in LDS and walks 16-bit slots, four rows per 8-byte load.
-template <int G, int PREC, bool ONE_TYPE, bool FUSED>
+// The list pads every row of a slab wave up to the wave's longest row with
+// the tile's sentinel slot (a position far outside any cutoff), so all lanes
+// run the same number of iterations with no per-pair bounds test; inactive
+// lanes sit at another far point.  A padded pair adds exact zeros.
+template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY, bool PLANES>
 __global__ __launch_bounds__( TPB ) void k_tile_force(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
-    int lcap, int scap, const double4 *__restrict__ x,
+    int lcap, int scap, int slot_cap, const double4 *__restrict__ x,
     const double4 *__restrict__ xg, double4 *__restrict__ x_next,
     double *__restrict__ v, double *__restrict__ f,
     const int *__restrict__ counts, const unsigned short *__restrict__ slab,
@@ -421,7 +425,13 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     constexpr int APW = 64 / G;
     const int t = xcd_tile( blockIdx.x, gridDim.x );
     load_head( info, t, H );
-    stage_positions( H, x, xg, s_pos );
+    double *s_x = reinterpret_cast<double *>( s_pos );
+    const double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
+    const int *s_t = reinterpret_cast<const int *>( s_x + 3 * slot_cap );
+    if ( PLANES )
+        stage_planes<!ONE_TYPE>( H, x, xg, s_x, slot_cap );
+    else
+        stage_positions( H, x, xg, s_pos );
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
     const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
     const int nw = woff[t + 1] - woff[t];
@@ -429,7 +439,6 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     const int self0 = H.segoff[8] - H.info[TI_LB + 4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int a = lane % APW, sub = lane / APW;
-    const bool energy = FUSED && ctl.thermo;
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
     for ( int w = wid; w < nw; w += TPB / 64 )
     {
@@ -437,44 +446,62 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         const bool active = ta < nat;
         const int i = tl + ta;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        double4 xi = make_double4( 0.0, 0.0, 0.0, 0.0 );
-        if ( active )
+        const double4 xi =
+            active ? ( PLANES ? x[i] : s_pos[self0 + i] )
+                   : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
+        const int ti = (int)xi.w;
+        const int n = active ? counts[i] : 0;
+        int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
+#pragma unroll
+        for ( int o = 32; o > 0; o >>= 1 )
+            nr = max( nr, __shfl_xor( nr, o, 64 ) );
+        nr = __builtin_amdgcn_readfirstlane( nr );
+        const unsigned long long *row8 = reinterpret_cast<const unsigned long long *>(
+            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
+        auto body = [&]( const double4 &xj )
         {
-            xi = s_pos[self0 + i];
-            const int ti = (int)xi.w;
-            const int n = counts[i];
-            const int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
-            const unsigned short *row = st + ( (size_t)w * R4 * 64 + lane ) * 4;
-            auto body = [&]( int sj )
+            const double dx = xi.x - xj.x;
+            const double dy = xi.y - xj.y;
+            const double dz = xi.z - xj.z;
+            const double rsq = ( PREC == CMD_PREC_FP64 )
+                                   ? dx * dx + dy * dy + dz * dz
+                                   : __builtin_fma( dx, dx, __builtin_fma( dy, dy, dz * dz ) );
+            const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
+            const double fpair = lj_pair<PREC, ENERGY>( rsq, c, pe, vir );
+            accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
+        };
+        // four positions per 8-byte word; all four LDS reads are issued
+        // before the arithmetic that consumes them
+        auto fetch = [&]( unsigned long long wd, int q ) -> double4
+        {
+            const int sj = (int)( ( wd >> ( 16 * q ) ) & 0xffffull );
+            if ( PLANES )
+                return make_double4( s_x[sj], s_y[sj], s_z[sj],
+                                     ONE_TYPE ? 0.0 : (double)s_t[sj] );
+            if ( ONE_TYPE )
             {
-                const double4 xj = s_pos[sj];
-                const double dx = xi.x - xj.x;
-                const double dy = xi.y - xj.y;
-                const double dz = xi.z - xj.z;
-                const double rsq = dx * dx + dy * dy + dz * dz;
-                const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
-                const double fpair = energy ? lj_pair<PREC, true>( rsq, c, pe, vir )
-                                            : lj_pair<PREC, false>( rsq, c, pe, vir );
-                accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
-            };
-            const unsigned long long *row8 =
-                reinterpret_cast<const unsigned long long *>( row );
-            unsigned long long nxt = 0ull;
-            if ( nr > 0 )
-                nxt = __builtin_nontemporal_load( row8 );
-            for ( int r = 0; r < nr; r += 4 )
-            {
-                const unsigned long long cur = nxt;
-                if ( r + 4 < nr )
-                    nxt = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 1 ) * 64 );
-                body( (int)( cur & 0xffffull ) );
-                if ( r + 1 < nr )
-                    body( (int)( ( cur >> 16 ) & 0xffffull ) );
-                if ( r + 2 < nr )
-                    body( (int)( ( cur >> 32 ) & 0xffffull ) );
-                if ( r + 3 < nr )
-                    body( (int)( cur >> 48 ) );
+                const double2 a = *reinterpret_cast<const double2 *>( s_pos + sj );
+                const double c = reinterpret_cast<const double *>( s_pos + sj )[2];
+                return make_double4( a.x, a.y, c, 0.0 );
             }
+            return s_pos[sj];
+        };
+        unsigned long long nxt = 0ull;
+        if ( nr > 0 )
+            nxt = __builtin_nontemporal_load( row8 );
+        for ( int r = 0; r < nr; r += 4 )
+        {
+            const unsigned long long cur = nxt;
+            if ( r + 4 < nr )
+                nxt = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 1 ) * 64 );
+            const double4 p0 = fetch( cur, 0 );
+            const double4 p1 = fetch( cur, 1 );
+            const double4 p2 = fetch( cur, 2 );
+            const double4 p3 = fetch( cur, 3 );
+            body( p0 );
+            body( p1 );
+            body( p2 );
+            body( p3 );
         }
 #pragma unroll
         for ( int o = APW; o < 64; o <<= 1 )
@@ -486,7 +513,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         finish_atom<G, FUSED>( active, sub, i, xi, fx, fy, fz, scap, x_next, v,
                                f, mt, ctl, mv2 );
     }
-    if ( energy )
+    if ( FUSED && ctl.thermo )
         block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
 }
 
@@ -704,17 +731,44 @@ static cmd_status launch_full_g( int G, int nb, hipStream_t st, int n, int scap,
     return CMD_OK;
 }
 
+template <int G, int PREC, bool ONE, bool FUSED, bool ENERGY, bool PLANES>
+static void launch_tile_k( cmd_neigh *N, cmd_system *S, double4 *xn,
+                           const LJParams *Pd, const PairCoef &c,
+                           const MassTable &mt, const StepCtl &ctl,
+                           double *partial )
+{
+    // staged positions plus the sentinel slot
+    const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
+    const size_t lds = PLANES ? (size_t)slot_cap * ( 3 * sizeof( double ) + ( ONE ? 0 : sizeof( int ) ) )
+                              : (size_t)slot_cap * sizeof( double4 );
+    k_tile_force<G, PREC, ONE, FUSED, ENERGY, PLANES><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
+        S->x.p, S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt,
+        ctl, partial );
+}
+
 template <int G, int PREC, bool ONE, bool FUSED>
 static void launch_tile_t( cmd_neigh *N, cmd_system *S, double4 *xn,
                            const LJParams *Pd, const PairCoef &c,
                            const MassTable &mt, const StepCtl &ctl,
                            double *partial )
 {
-    const size_t lds = (size_t)std::max( N->max_tile_slots, 1 ) * sizeof( double4 );
-    k_tile_force<G, PREC, ONE, FUSED><<<N->tg.ntiles, TPB, lds, S->stream>>>(
-        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, S->x.p,
-        S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt, ctl,
-        partial );
+    const bool planes = N->tile_planes;
+    if constexpr ( FUSED )
+    {
+        if ( ctl.thermo )
+        {
+            if ( planes )
+                launch_tile_k<G, PREC, ONE, FUSED, true, true>( N, S, xn, Pd, c, mt, ctl, partial );
+            else
+                launch_tile_k<G, PREC, ONE, FUSED, true, false>( N, S, xn, Pd, c, mt, ctl, partial );
+            return;
+        }
+    }
+    if ( planes )
+        launch_tile_k<G, PREC, ONE, FUSED, false, true>( N, S, xn, Pd, c, mt, ctl, partial );
+    else
+        launch_tile_k<G, PREC, ONE, FUSED, false, false>( N, S, xn, Pd, c, mt, ctl, partial );
 }
 
 template <int PREC, bool ONE, bool FUSED>
@@ -978,6 +1032,7 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     CMD_HIP( hipEventCreate( &e1 ) );
     float best = 1e30f;
     Cand bestc = { 1, 0 };
+    bool best_planes = true;
     cmd_status st = CMD_OK;
     for ( const Cand &cd : cands )
     {
@@ -988,24 +1043,34 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
             break;
         if ( cd.E > 0 && !N->tiled )
             continue; // the tiled layout does not apply here
-        st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+        // LDS layout of the tiled force: records, then coordinate planes
+        for ( int pl = 0; pl < ( N->tiled ? 2 : 1 ); ++pl )
+        {
+            N->tile_planes = pl == 1;
+            st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+            if ( st != CMD_OK )
+                break;
+            (void)hipEventRecord( e0, S->stream );
+            for ( int r = 0; r < 3; ++r )
+                (void)lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+            (void)hipEventRecord( e1, S->stream );
+            (void)hipEventSynchronize( e1 );
+            float ms = 0.0f;
+            (void)hipEventElapsedTime( &ms, e0, e1 );
+            if ( std::getenv( "CMD_VERBOSE" ) )
+                std::fprintf( stderr, "# autotune lanes=%d tile=%d%s: %.4f ms/launch\n",
+                              cd.G, cd.E,
+                              N->tiled ? ( N->tile_planes ? " (lds planes)" : " (lds records)" ) : "",
+                              ms / 3.0f );
+            if ( ms < best )
+            {
+                best = ms;
+                bestc = cd;
+                best_planes = N->tile_planes;
+            }
+        }
         if ( st != CMD_OK )
             break;
-        (void)hipEventRecord( e0, S->stream );
-        for ( int r = 0; r < 3; ++r )
-            (void)lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
-        (void)hipEventRecord( e1, S->stream );
-        (void)hipEventSynchronize( e1 );
-        float ms = 0.0f;
-        (void)hipEventElapsedTime( &ms, e0, e1 );
-        if ( std::getenv( "CMD_VERBOSE" ) )
-            std::fprintf( stderr, "# autotune lanes=%d tile=%d%s: %.4f ms/launch\n",
-                          cd.G, cd.E, N->tiled ? " (lds)" : "", ms / 3.0f );
-        if ( ms < best )
-        {
-            best = ms;
-            bestc = cd;
-        }
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
@@ -1014,6 +1079,7 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     F->tuned = true;
     N->lanes = bestc.G;
     N->tile_e = bestc.E;
+    N->tile_planes = best_planes;
     return neigh_build( N, S );
 }
 

@@ -462,6 +462,21 @@ cmd_status cmd_neigh_destroy( cmd_neigh *n )
     return CMD_OK;
 }
 
+cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height, int lds_planes )
+{
+    CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    CMD_REQUIRE( height >= 0 && height <= 14, CMD_ERR_ARG,
+                 "tile height must be 0 (choose) or 1..14 cells" );
+    if ( height != n->tile_e )
+    {
+        n->tile_e = height;
+        n->n_rows = 0; // layout changed: must rebuild
+        n->csr_valid = false;
+    }
+    n->tile_planes = lds_planes != 0;
+    return CMD_OK;
+}
+
 cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );

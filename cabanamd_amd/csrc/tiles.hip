@@ -31,19 +31,20 @@
 namespace cmd
 {
 static constexpr int TPB = 256;
+static constexpr int BTPB = 512;       // list build block
 static constexpr int EMAX = 16;
 static constexpr int MAX_SLOTS = 1792; // 56 KB of double4 per block
 // Per-tile runs, atom range and staged count; waves per tile for the slab.
 // slots[0] gets the largest staged count, slots[1] the number of local atoms
 // the tiles cover (all of them only when every local atom sits in an
-// interior cell of the grid).
+// interior cell of the grid), slots[2] the most slab waves of one tile.
 __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
                              const int *__restrict__ gs, int apw,
                              int *__restrict__ info, int *__restrict__ waves,
                              int *__restrict__ slots )
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = 0, nat = 0;
+    int total = 0, nat = 0, nwv = 0;
     if ( t < T.ntiles )
     {
         int a, b, e0, e1;
@@ -69,7 +70,8 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
         I[TI_EA] = ea;
         I[TI_NST] = eb - ea;
         nat = I[TI_NAT];
-        waves[t] = ( nat + apw - 1 ) / apw;
+        nwv = ( nat + apw - 1 ) / apw;
+        waves[t] = nwv;
     }
     int m = total;
 #pragma unroll
@@ -77,6 +79,7 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
     {
         m = max( m, __shfl_xor( m, o, 64 ) );
         nat += __shfl_xor( nat, o, 64 );
+        nwv = max( nwv, __shfl_xor( nwv, o, 64 ) );
     }
     if ( ( threadIdx.x & 63 ) == 0 )
     {
@@ -84,43 +87,58 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
             atomicMax( &slots[0], m );
         if ( nat > 0 )
             atomicAdd( &slots[1], nat );
+        if ( nwv > 0 )
+            atomicMax( &slots[2], nwv );
     }
 }
 
+template <bool HALF>
 __device__ __forceinline__ bool tile_accept( const double4 &pi, const double4 &pj,
-                                             double rc2, int half )
+                                             double rc2 )
 {
-    if ( half &&
-         !( ( pj.x > pi.x ) ||
-            ( ( pj.x == pi.x ) &&
-              ( ( pj.y > pi.y ) || ( ( pj.y == pi.y ) && ( pj.z > pi.z ) ) ) ) ) )
-        return false;
     const double dx = pi.x - pj.x;
     const double dy = pi.y - pj.y;
     const double dz = pi.z - pj.z;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    return rsq <= rc2;
+    bool ok = rsq <= rc2;
+    if ( HALF )
+        ok = ok && ( ( pj.x > pi.x ) |
+                     ( ( pj.x == pi.x ) & ( ( pj.y > pi.y ) |
+                                            ( ( pj.y == pi.y ) & ( pj.z > pi.z ) ) ) ) );
+    return ok;
 }
 
-// List build: one lane per atom walks its stencil in LDS.
-template <int G>
-__global__ __launch_bounds__( TPB ) void k_tile_build(
+// List build, one wave per cell of the tile.  All atoms of a cell share the
+// same 27-cell stencil, so the wave walks the stencil's candidates once, 64
+// at a time (lane = candidate, positions from LDS planes), and tests each
+// chunk against every atom of the cell in turn; ballot + mbcnt place each
+// accepted candidate in the atom's row, so rows come out in walk order.  The
+// stencil is a list of contiguous LDS ranges (per column: the locals of its
+// three cells, or locals and ghosts cell by cell where ghosts are present),
+// concatenated so that chunks run across range ends.  Rows are assembled in
+// LDS in the slab layout, pre-filled with the sentinel slot, and written out
+// with 16-byte stores.
+static constexpr int MAXR = 9 * 3 * 2; // stencil ranges of one cell
+
+template <int G, bool HALF>
+__global__ __launch_bounds__( BTPB ) void k_tile_build(
     TileGrid T, const int *__restrict__ ls, const int *__restrict__ gs,
     const int *__restrict__ info, const int *__restrict__ woff,
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2,
-    int half, int cap, unsigned short *__restrict__ slab,
+    int cap, int slot_cap, unsigned short *__restrict__ slab,
     int *__restrict__ counts, int *__restrict__ max_row )
 {
-    extern __shared__ double4 s_pos[];
+    extern __shared__ double s_x[];
     __shared__ TileHead H;
     __shared__ int s_lcs[9][EMAX + 4], s_gcs[9][EMAX + 4];
+    __shared__ int s_rs[BTPB / 64][MAXR];     // range starts (LDS slots)
+    __shared__ int s_rp[BTPB / 64][MAXR + 1]; // prefix of range lengths
     constexpr int APW = 64 / G;
-    const int t = blockIdx.x;
+    const int t = xcd_tile( blockIdx.x, gridDim.x );
     load_head( info, t, H );
     int a, b, e0, e1;
     tile_coords( T, t, a, b, e0, e1 );
     const int ea = H.info[TI_EA], nst = H.info[TI_NST];
-    // LDS slot of the first atom of each staged cell, locals and ghosts
     for ( int k = threadIdx.x; k < 9 * ( nst + 1 ); k += blockDim.x )
     {
         const int q = k / ( nst + 1 ), ce = k % ( nst + 1 );
@@ -128,54 +146,163 @@ __global__ __launch_bounds__( TPB ) void k_tile_build(
         s_lcs[q][ce] = H.segoff[2 * q] + ( ls[c] - H.info[TI_LB + q] );
         s_gcs[q][ce] = H.segoff[2 * q + 1] + ( gs[c] - H.info[TI_GB + q] );
     }
-    stage_positions( H, x, xg, s_pos );
+    stage_planes<false>( H, x, xg, s_x, slot_cap );
+    const double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
+    unsigned short *chunk = reinterpret_cast<unsigned short *>( s_x + 3 * slot_cap );
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
-    unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
-    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
-    int mrow = 0;
-    for ( int ta = threadIdx.x; ta < nat; ta += blockDim.x )
+    const int wsz = R4 * 256; // slab entries of one slab wave
+    const int nw = woff[t + 1] - woff[t];
+    const int tl = H.info[TI_TL];
+    const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nwv = blockDim.x >> 6;
+    // rows start as sentinel slots (the tile's slot TOTAL)
     {
-        const int i = tl + ta;
-        const int si = H.segoff[8] + ( i - H.info[TI_LB + 4] );
-        const double4 pi = s_pos[si];
-        const int ei = cell_coord( pi.z, T.g.min[2], T.g.rd[2], T.g.n[2],
-                                   T.g.floor_mode );
-        // stencil cells of atom i inside the staged range (always true when
-        // the atoms have not moved since create_binning; clamped regardless)
-        const int elo = max( max( ei - 1, 0 ) - ea, 0 );
-        const int ehi = min( min( ei + 1, T.g.n[2] - 1 ) - ea, nst - 1 );
-        const int w = ta / APW, aw = ta % APW;
-        int cnt = 0;
-        auto take = [&]( int sj )
+        const unsigned sent = (unsigned)H.segoff[18];
+        const unsigned sent2 = sent | ( sent << 16 );
+        const uint4 fill = make_uint4( sent2, sent2, sent2, sent2 );
+        const int n16 = nw * wsz / 8;
+        for ( int k = threadIdx.x; k < n16; k += blockDim.x )
+            reinterpret_cast<uint4 *>( chunk )[k] = fill;
+    }
+    __syncthreads();
+    int mrow = 0;
+    int *rs = s_rs[wv], *rp = s_rp[wv];
+    for ( int e = e0 + wv; e < e1; e += nwv )
+    {
+        const int ce = e - ea;
+        // stencil cells of this cell inside the staged range
+        const int elo = max( ce - 1, 0 ), ehi = min( ce + 1, nst - 1 );
+        // column q's ranges (lane q < 9), then their offsets in the list
+        int nr_q = 0;
+        bool gh = false;
+        if ( lane < 9 )
         {
-            if ( sj == si )
-                return;
-            if ( tile_accept( pi, s_pos[sj], rc2, half ) )
+            gh = s_gcs[lane][elo] != s_gcs[lane][ehi + 1];
+            nr_q = gh ? 2 * ( ehi - elo + 1 ) : 1;
+        }
+        int off = nr_q;
+#pragma unroll
+        for ( int o = 1; o < 16; o <<= 1 )
+        {
+            const int u = __shfl_up( off, o, 64 );
+            if ( lane >= o )
+                off += u;
+        }
+        const int nr = __builtin_amdgcn_readfirstlane( __shfl( off, 8, 64 ) );
+        off -= nr_q;
+        if ( lane < 9 )
+        {
+            if ( !gh )
             {
-                if ( cnt < cap )
+                rs[off] = s_lcs[lane][elo];
+                rp[off + 1] = s_lcs[lane][ehi + 1] - s_lcs[lane][elo];
+            }
+            else
+            {
+                for ( int c2 = elo; c2 <= ehi; ++c2 )
                 {
-                    const int r = cnt / G, lane = ( cnt % G ) * APW + aw;
-                    st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )] =
-                        (unsigned short)sj;
+                    rs[off] = s_lcs[lane][c2];
+                    rp[off + 1] = s_lcs[lane][c2 + 1] - s_lcs[lane][c2];
+                    rs[off + 1] = s_gcs[lane][c2];
+                    rp[off + 2] = s_gcs[lane][c2 + 1] - s_gcs[lane][c2];
+                    off += 2;
                 }
-                ++cnt;
             }
-        };
-        for ( int q = 0; q < 9; ++q )
-            for ( int ce = elo; ce <= ehi; ++ce )
+        }
+        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+        // prefix of the range lengths (nr <= MAXR < 64: one range per lane)
+        int len = lane < nr ? rp[lane + 1] : 0;
+        int inc = len;
+#pragma unroll
+        for ( int o = 1; o < 64; o <<= 1 )
+        {
+            const int u = __shfl_up( inc, o, 64 );
+            if ( lane >= o )
+                inc += u;
+        }
+        if ( lane < nr )
+            rp[lane + 1] = inc;
+        if ( lane == 0 )
+            rp[0] = 0;
+        const int total = __builtin_amdgcn_readfirstlane( __shfl( inc, nr - 1, 64 ) );
+        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+        // the cell's atoms, up to 64 at a time (lane l holds atom a0 + l)
+        const int cc = col_base( T, a, b ) + e;
+        const int c0 = __builtin_amdgcn_readfirstlane( ls[cc] );
+        const int c1 = __builtin_amdgcn_readfirstlane( ls[cc + 1] );
+        for ( int a0 = c0; a0 < c1; a0 += 64 )
+        {
+            const int na = min( 64, c1 - a0 );
+            int cnt = 0; // row length of atom a0 + lane
+            for ( int v0 = 0; v0 < total; v0 += 64 )
             {
-                for ( int sj = s_lcs[q][ce]; sj < s_lcs[q][ce + 1]; ++sj )
-                    take( sj );
-                for ( int sj = s_gcs[q][ce]; sj < s_gcs[q][ce + 1]; ++sj )
-                    take( sj );
+                const int v = v0 + lane;
+                const bool valid = v < total;
+                // range holding candidate v: binary search over the prefix
+                int lo = 0, hi = nr;
+                while ( hi - lo > 1 )
+                {
+                    const int mid = ( lo + hi ) >> 1;
+                    if ( rp[mid] <= v )
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                const int sj = valid ? rs[lo] + ( v - rp[lo] ) : si0 + a0;
+                const double pjx = s_x[sj], pjy = s_y[sj], pjz = s_z[sj];
+                for ( int l = 0; l < na; ++l )
+                {
+                    const int si = si0 + a0 + l;
+                    const double pix = s_x[si], piy = s_y[si], piz = s_z[si];
+                    const double dx = pix - pjx;
+                    const double dy = piy - pjy;
+                    const double dz = piz - pjz;
+                    const double rsq = dx * dx + dy * dy + dz * dz;
+                    bool acc = valid & ( sj != si ) & ( rsq <= rc2 );
+                    if ( HALF )
+                        acc = acc & ( ( pjx > pix ) |
+                                      ( ( pjx == pix ) &
+                                        ( ( pjy > piy ) | ( ( pjy == piy ) & ( pjz > piz ) ) ) ) );
+                    const unsigned long long m = __ballot( acc );
+                    const int cl = __builtin_amdgcn_readlane( cnt, l );
+                    const unsigned k =
+                        (unsigned)cl + __builtin_amdgcn_mbcnt_hi(
+                                           (unsigned)( m >> 32 ),
+                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) );
+                    if ( acc && k < (unsigned)cap )
+                    {
+                        const unsigned ta = (unsigned)( a0 + l - tl );
+                        const unsigned r = k / G;
+                        const unsigned L = ( k % G ) * APW + ta % APW;
+                        chunk[( ( ( ta / APW ) * R4 + ( r >> 2 ) ) * 64 + L ) * 4 + ( r & 3 )] =
+                            (unsigned short)sj;
+                    }
+                    if ( lane == l )
+                        cnt = cl + (int)__popcll( m );
+                }
             }
-        counts[i] = cnt;
-        mrow = max( mrow, cnt );
+            if ( lane < na )
+            {
+                counts[a0 + lane] = cnt;
+                mrow = max( mrow, cnt );
+            }
+        }
+        // the range tables are rewritten for the next cell
+        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+    }
+    __syncthreads();
+    {
+        const int n16 = nw * wsz / 8;
+        const uint4 *src = reinterpret_cast<const uint4 *>( chunk );
+        uint4 *dst = reinterpret_cast<uint4 *>( slab + (size_t)woff[t] * wsz );
+        for ( int k = threadIdx.x; k < n16; k += blockDim.x )
+            dst[k] = src[k];
     }
 #pragma unroll
     for ( int o = 32; o > 0; o >>= 1 )
         mrow = max( mrow, __shfl_xor( mrow, o, 64 ) );
-    if ( ( threadIdx.x & 63 ) == 0 && mrow > 0 )
+    if ( lane == 0 && mrow > 0 )
         atomicMax( max_row, mrow );
 }
 
@@ -288,8 +415,7 @@ cmd_status tiles_ghost_refresh( cmd_system *S )
     return CMD_OK;
 }
 
-static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E,
-                                 int *max_total, int *covered )
+static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[3] )
 {
     TileGrid &T = N->tg;
     T.g = S->bin_grid;
@@ -305,7 +431,7 @@ static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E,
     DevBuf<int> &waves = N->grid_sort.cursor; // scratch: waves per tile
     CMD_TRY( waves.reserve( T.ntiles + 1 ) );
     CMD_TRY( N->grid_sort.start.reserve( T.ntiles + 1 ) ); // scan of waves
-    CMD_TRY( fill_i32( S->sc.ires.p, 0, 2, S->stream ) );
+    CMD_TRY( fill_i32( S->sc.ires.p, 0, 3, S->stream ) );
     const int apw = 64 / N->lanes;
     k_tile_info<<<( T.ntiles + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
         T, S->bins.start.p, S->gsort.start.p, apw, N->tile_info.p, waves.p,
@@ -313,12 +439,9 @@ static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E,
     CMD_LAUNCH_CHECK();
     CMD_TRY( scan_exclusive( waves.p, N->grid_sort.start.p, T.ntiles, S->sc,
                              S->stream, nullptr ) );
-    int h[2] = { 0, 0 };
-    CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, sizeof( h ),
+    CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, 3 * sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
-    *max_total = h[0];
-    *covered = h[1];
     return CMD_OK;
 }
 
@@ -333,14 +456,39 @@ static size_t tile_slab_entries( cmd_neigh *N, hipStream_t st )
     return std::max<size_t>( (size_t)nw * R4 * 256, 1 );
 }
 
+// LDS limits of one list-build block: the hard per-block limit, and the
+// size that leaves room for four blocks (32 waves) per CU, which the row
+// chunk is sized to when the staged positions allow it.
+static void build_lds_limits( size_t *hard, size_t *target )
+{
+    int dev = 0, lim = 0;
+    if ( hipGetDevice( &dev ) != hipSuccess ||
+         hipDeviceGetAttribute( &lim, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                dev ) != hipSuccess ||
+         lim <= 0 )
+        lim = 65536;
+    const size_t statics = 4096; // tile header and cell tables
+    *hard = (size_t)lim > statics ? (size_t)lim - statics : 0;
+    *target = (size_t)lim >= 2 * 65536 ? (size_t)lim / 4 - statics : *hard / 2;
+}
+
 template <int G>
-static void launch_build( cmd_neigh *N, cmd_system *S, size_t lds, int *max_row )
+static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, size_t lds,
+                          int *max_row )
 {
     const double rc2 = N->neigh_cut * N->neigh_cut;
-    k_tile_build<G><<<N->tg.ntiles, TPB, lds, S->stream>>>(
-        N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
-        N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->half, N->cap, N->tslab.p,
-        N->counts.p, max_row );
+    // one wave per cell of the tile
+    const int threads = 64 * std::min( std::max( N->tg.E, 1 ), BTPB / 64 );
+    if ( N->half )
+        k_tile_build<G, true><<<N->tg.ntiles, threads, lds, S->stream>>>(
+            N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
+            N->tslab.p, N->counts.p, max_row );
+    else
+        k_tile_build<G, false><<<N->tg.ntiles, threads, lds, S->stream>>>(
+            N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
+            N->tslab.p, N->counts.p, max_row );
 }
 
 cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
@@ -348,42 +496,52 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
     *built = false;
     if ( !S->bin_grid_valid || !S->xg_valid || N->lanes < 2 )
         return CMD_OK;
-    // tile height: the requested one, or the tallest whose neighbourhood
-    // fits the LDS budget
-    int E = N->tile_e > 0 ? N->tile_e : 6;
-    int max_total = 0, covered = 0;
-    for ( ;; )
-    {
-        CMD_TRY( tile_info_for( N, S, std::min( E, EMAX - 2 ), &max_total,
-                                &covered ) );
-        // local atoms in the grid's padding cells (not wrapped into the box)
-        // belong to no tile: use the global layout
-        if ( covered != S->n_local )
-            return CMD_OK;
-        if ( max_total <= MAX_SLOTS )
-            break;
-        if ( E == 1 )
-            return CMD_OK; // too dense for LDS tiles: global path
-        E = std::max( 1, E * 2 / 3 );
-    }
-    N->max_tile_slots = max_total;
-    const size_t lds = (size_t)std::max( max_total, 1 ) * sizeof( double4 );
+    const int G = N->lanes;
     if ( N->cap < 1 )
         N->cap = std::max( 1, N->max_neigh_guess );
-    for ( int attempt = 0; attempt < 3; ++attempt )
+    size_t hard = 0, target = 0;
+    build_lds_limits( &hard, &target );
+    (void)target;
+    // tile height: the requested one, or the tallest whose neighbourhood and
+    // rows fit in LDS
+    int E = N->tile_e > 0 ? N->tile_e : 4;
+    int h[3] = { 0, 0, 0 };
+    for ( int attempt = 0; attempt < 4; ++attempt )
     {
+        size_t lds = 0;
+        for ( ;; )
+        {
+            CMD_TRY( tile_info_for( N, S, std::min( E, EMAX - 2 ), h ) );
+            // local atoms in the grid's padding cells (not wrapped into the
+            // box) belong to no tile: use the global layout
+            if ( h[1] != S->n_local )
+                return CMD_OK;
+            const size_t wbytes =
+                (size_t)( ( ( N->cap + G - 1 ) / G + 3 ) / 4 ) * 256 * sizeof( unsigned short );
+            // coordinate planes of the staged atoms and the sentinel slot,
+            // then the rows of every slab wave of the tile
+            lds = (size_t)( h[0] + 1 ) * 3 * sizeof( double ) +
+                  (size_t)std::max( h[2], 1 ) * wbytes;
+            if ( h[0] <= MAX_SLOTS && lds <= hard )
+                break;
+            if ( E == 1 )
+                return CMD_OK; // too dense for LDS tiles: global path
+            E = std::max( 1, E * 2 / 3 );
+        }
+        N->max_tile_slots = h[0];
         CMD_TRY( N->tslab.reserve( tile_slab_entries( N, S->stream ) ) );
         CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, S->stream ) );
-        switch ( N->lanes )
+        const int slot_cap = h[0] + 1;
+        switch ( G )
         {
         case 2:
-            launch_build<2>( N, S, lds, S->sc.ires.p );
+            launch_build<2>( N, S, slot_cap, lds, S->sc.ires.p );
             break;
         case 4:
-            launch_build<4>( N, S, lds, S->sc.ires.p );
+            launch_build<4>( N, S, slot_cap, lds, S->sc.ires.p );
             break;
         default:
-            launch_build<8>( N, S, lds, S->sc.ires.p );
+            launch_build<8>( N, S, slot_cap, lds, S->sc.ires.p );
             break;
         }
         CMD_LAUNCH_CHECK();
@@ -401,6 +559,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             N->gperm_view = S->gsort.perm.p;
             return CMD_OK;
         }
+        // rows overflowed: larger rows, and the LDS fit rechecked
         N->cap = std::max( N->max_neigh_guess, mr );
     }
     set_error( "neighbour rows kept growing during the tiled build" );

@@ -73,20 +73,93 @@ __device__ inline void load_head( const int *__restrict__ info, int t, TileHead 
     __syncthreads();
 }
 
-// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS.
+// Position of a tile's sentinel slot (slot TOTAL, just past the staged
+// atoms): list rows are padded with it, and its distance to any atom is far
+// beyond every cutoff, so a padded pair contributes exact zeros.
+static constexpr double TILE_FAR = 1.0e30;
+
+// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS,
+// followed by the sentinel slot.
 __device__ inline void stage_positions( const TileHead &H, const double4 *__restrict__ x,
                                  const double4 *__restrict__ xg,
                                  double4 *__restrict__ s_pos )
 {
     const int total = H.segoff[18];
+    constexpr int B = 4; // slots per thread whose loads are in flight together
     int seg = 0;
-    for ( int k = threadIdx.x; k < total; k += blockDim.x )
+    for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
     {
-        while ( k >= H.segoff[seg + 1] )
-            ++seg;
-        const int q = seg >> 1, off = k - H.segoff[seg];
-        s_pos[k] = ( seg & 1 ) ? xg[H.info[TI_GB + q] + off]
-                               : x[H.info[TI_LB + q] + off];
+        double4 p[B];
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+        {
+            // past the end: reload the last slot (in bounds)
+            const int k = min( k0 + u * (int)blockDim.x, total - 1 );
+            while ( k >= H.segoff[seg + 1] )
+                ++seg;
+            const int q = seg >> 1, off = k - H.segoff[seg];
+            const double4 *__restrict__ base = ( seg & 1 ) ? xg : x;
+            const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
+            p[u] = base[idx];
+        }
+        // stores past the end go to the sentinel slot, rewritten below
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+            s_pos[min( k0 + u * (int)blockDim.x, total )] = p[u];
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+        s_pos[total] = make_double4( TILE_FAR, TILE_FAR, TILE_FAR, 0.0 );
+    __syncthreads();
+}
+
+// The same staging as separate coordinate planes (x, y, z doubles, and the
+// type as an int when more than one type is present), `cap` slots each: a
+// gather then reads three 8-byte words, which spread over the LDS banks
+// better than one 32-byte record.
+template <bool WITH_TYPE>
+__device__ inline void stage_planes( const TileHead &H, const double4 *__restrict__ x,
+                                     const double4 *__restrict__ xg,
+                                     double *__restrict__ s_x, int cap )
+{
+    double *s_y = s_x + cap, *s_z = s_x + 2 * cap;
+    int *s_t = reinterpret_cast<int *>( s_x + 3 * cap );
+    const int total = H.segoff[18];
+    constexpr int B = 4;
+    int seg = 0;
+    for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
+    {
+        double4 p[B];
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+        {
+            const int k = min( k0 + u * (int)blockDim.x, total - 1 );
+            while ( k >= H.segoff[seg + 1] )
+                ++seg;
+            const int q = seg >> 1, off = k - H.segoff[seg];
+            const double4 *__restrict__ base = ( seg & 1 ) ? xg : x;
+            const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
+            p[u] = base[idx];
+        }
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+        {
+            const int k = min( k0 + u * (int)blockDim.x, total );
+            s_x[k] = p[u].x;
+            s_y[k] = p[u].y;
+            s_z[k] = p[u].z;
+            if ( WITH_TYPE )
+                s_t[k] = (int)p[u].w;
+        }
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        s_x[total] = TILE_FAR;
+        s_y[total] = TILE_FAR;
+        s_z[total] = TILE_FAR;
+        if ( WITH_TYPE )
+            s_t[total] = 0;
     }
     __syncthreads();
 }

@@ -25,6 +25,25 @@
 
 static const int RED_CHUNK = 4096; /* atoms per deterministic partial sum */
 
+/* Neumaier compensated sum: the energy sums run over ~10^6 pair terms with
+ * a systematic shift term, where a plain running sum drifts by ~1e-12
+ * relative; compensation keeps the checker within a few ulp of the exact
+ * sum, whatever the summation order of the code it checks. */
+struct CompSum
+{
+    double s = 0.0, c = 0.0;
+    void add( double v )
+    {
+        const double t = s + v;
+        if ( std::fabs( s ) >= std::fabs( v ) )
+            c += ( s - t ) + v;
+        else
+            c += ( v - t ) + s;
+        s = t;
+    }
+    double value() const { return s + c; }
+};
+
 namespace
 {
 using clk = std::chrono::steady_clock;
@@ -820,7 +839,7 @@ double orc_lj_energy( int n_local, const double *x, const int *type,
 #pragma omp parallel for schedule( static )
     for ( int c = 0; c < nchunk; ++c )
     {
-        double pe = 0.0;
+        CompSum pe;
         const int i1 = std::min( n_local, ( c + 1 ) * RED_CHUNK );
         for ( int i = c * RED_CHUNK; i < i1; ++i )
             for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
@@ -837,18 +856,18 @@ double orc_lj_energy( int n_local, const double *x, const int *type,
                         half ? ( ( j < n_local ) ? 1.0 : 0.5 ) : 0.5;
                     double r2inv = 1.0 / rsq;
                     double r6inv = r2inv * r2inv * r2inv;
-                    pe += fac * r6inv * ( 0.5 * lj1[t] * r6inv - lj2[t] ) / 6.0;
+                    pe.add( fac * r6inv * ( 0.5 * lj1[t] * r6inv - lj2[t] ) / 6.0 );
                     double r2invc = 1.0 / cutsq[t];
                     double r6invc = r2invc * r2invc * r2invc;
-                    pe -= fac * r6invc * ( 0.5 * lj1[t] * r6invc - lj2[t] ) / 6.0;
+                    pe.add( -( fac * r6invc * ( 0.5 * lj1[t] * r6invc - lj2[t] ) / 6.0 ) );
                 }
             }
-        part[c] = pe;
+        part[c] = pe.value();
     }
-    double pe = 0.0;
+    CompSum pe;
     for ( int c = 0; c < nchunk; ++c )
-        pe += part[c];
-    return pe;
+        pe.add( part[c] );
+    return pe.value();
 }
 
 double orc_lj_virial( int n_local, const double *x, const int *type,
@@ -862,7 +881,7 @@ double orc_lj_virial( int n_local, const double *x, const int *type,
 #pragma omp parallel for schedule( static )
     for ( int c = 0; c < nchunk; ++c )
     {
-        double w = 0.0;
+        CompSum w;
         const int i1 = std::min( n_local, ( c + 1 ) * RED_CHUNK );
         for ( int i = c * RED_CHUNK; i < i1; ++i )
             for ( int64_t k = offsets[i]; k < offsets[i + 1]; ++k )
@@ -878,43 +897,44 @@ double orc_lj_virial( int n_local, const double *x, const int *type,
                     double r2inv = 1.0 / rsq;
                     double r6inv = r2inv * r2inv * r2inv;
                     double fpair = ( r6inv * ( lj1[t] * r6inv - lj2[t] ) ) * r2inv;
-                    w += fac * rsq * fpair;
+                    w.add( fac * rsq * fpair );
                 }
             }
-        part[c] = w;
+        part[c] = w.value();
     }
-    double w = 0.0;
+    CompSum w;
     for ( int c = 0; c < nchunk; ++c )
-        w += part[c];
-    return w;
+        w.add( part[c] );
+    return w.value();
 }
 
 /* ------------------------------------------------------------------------ */
 /* Thermo: property_temperature.h:71-77 + impl :55-77; property_kine_impl.h:55-76
- * (sequential sums here; the reference's reduction order is unspecified). */
+ * (compensated sequential sums here; the reference's reduction order is
+ * unspecified). */
 double orc_temperature( int n_local, int n_global, const double *v,
                         const int *type, const double *mass, double mvv2e,
                         double boltz )
 {
-    double T = 0.0;
+    CompSum T;
     for ( int i = 0; i < n_local; ++i )
-        T += ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
-               v[3 * i + 2] * v[3 * i + 2] ) *
-             mass[type[i]];
+        T.add( ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                 v[3 * i + 2] * v[3 * i + 2] ) *
+               mass[type[i]] );
     const int dof = 3 * n_global - 3;
     const double factor = mvv2e / ( 1.0 * dof * boltz );
-    return T * factor;
+    return T.value() * factor;
 }
 
 double orc_kinetic_energy( int n_local, const double *v, const int *type,
                            const double *mass, double mvv2e )
 {
-    double ke = 0.0;
+    CompSum ke;
     for ( int i = 0; i < n_local; ++i )
-        ke += ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
-                v[3 * i + 2] * v[3 * i + 2] ) *
-              mass[type[i]];
-    return ke * ( 0.5 * mvv2e );
+        ke.add( ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                  v[3 * i + 2] * v[3 * i + 2] ) *
+                mass[type[i]] );
+    return ke.value() * ( 0.5 * mvv2e );
 }
 
 /* ------------------------------------------------------------------------ */

@@ -230,10 +230,11 @@ def test_neighbor_on_binning_grid_exact(gpu, half, lanes):
     assert np.array_equal(idx, idx_o)
 
 
+@pytest.mark.parametrize("planes", [True, False])
 @pytest.mark.parametrize("wrapped", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("lanes", [2, 4, 8])
-def test_lj_tiled_force(gpu, prec, lanes, wrapped):
+def test_lj_tiled_force(gpu, prec, lanes, wrapped, planes):
     """Full list built on the binning grid with 2+ lanes per atom uses the
     LDS-tiled layout when every local atom lies in an interior cell; atoms
     left unwrapped in the padding cells make the build use the global layout.
@@ -250,7 +251,7 @@ def test_lj_tiled_force(gpu, prec, lanes, wrapped):
     M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
     comm = M.Comm(s, 2.8)
     comm.exchange_halo()
-    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes)
+    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes, lds_planes=planes)
     nl.create(s)
     F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
     F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
@@ -396,3 +397,53 @@ def test_md_mixed_energy_drift(gpu):
     drift = np.abs(e - e[0]).max() / abs(e[0])
     print(f"mixed precision drift over 200 steps: {drift:.3e}")
     assert drift < 1e-3
+
+
+MULTI = [["pair_coeff", "1", "1", "1.0", "1.0", "2.5"],
+         ["pair_coeff", "1", "2", "0.8", "1.1", "2.4"],
+         ["pair_coeff", "2", "2", "1.3", "0.9", "2.2"]]
+
+
+@pytest.mark.parametrize("path", ["global1", "tiled-planes", "tiled-records",
+                                  "half"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lj_two_types(gpu, prec, path):
+    """Two atom types with their own eps, sigma and cutoff per pair
+    (force_lj_cabana_neigh_impl.h:62-89 symmetric tables): forces and energy
+    against the oracle on the global and the tiled layouts."""
+    M = gpu
+    ncells = (9, 9, 8)
+    _, x, v, t, i = _liquid_state(ncells)
+    t = (i % 2).astype(np.int32)  # alternate types by atom id
+    hi, gc = _domain_for(ncells)
+    O.wrap(x, O.create_domain([0, 0, 0], hi, gc).L)
+    s = M.System(2)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.set_mass([1.0, 1.7])
+    s.upload(x, v, None, t, i)
+    M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
+    comm = M.Comm(s, 2.8)
+    comm.exchange_halo()
+    half = path == "half"
+    lanes = 1 if path == "global1" else 2
+    nl = M.NeighborVerlet(2.8, half, 50, lanes=lanes,
+                          lds_planes=(path != "tiled-records"))
+    nl.create(s)
+    F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
+    F.init_coeff(MULTI)
+    F.compute(s, nl)
+    if half:
+        comm.update_force()
+    xg, _, fg, tg, _ = s.download()
+    off, idx = nl.get()
+    n = len(x)
+    lj1, lj2, cs = O.lj_coeff(2, [(int(c[1]), int(c[2]), float(c[3]),
+                                   float(c[4]), float(c[5])) for c in MULTI])
+    fo = O.lj_force(xg, tg, n, 2, lj1, lj2, cs, off, idx, half)
+    if half:
+        send, counts = comm.ghost_lists()
+        O.ghost_fold(fo, n, send, counts)
+    err = np.linalg.norm(fg[:n] - fo[:n]) / np.linalg.norm(fo[:n])
+    assert err < (1e-12 if prec == "fp64" else 1e-5), err
+    eo = O.lj_energy(xg, tg, n, 2, lj1, lj2, cs, off, idx, half)
+    assert abs(F.compute_energy(s, nl) - eo) <= 1e-10 * abs(eo)
