@@ -35,7 +35,27 @@ FUSED_NVE_BYTES_PER_ATOM = 72
 ALG_BYTES_PER_ATOM = LJ_FULL_BYTES_PER_ATOM + FUSED_NVE_BYTES_PER_ATOM
 
 
-def _variant(M, ncells, steps, warmup, half, prec, lanes=0):
+def _force_error(md):
+    """Forces of the engine on the current (liquid) state against the FP64
+    oracle's full-list LJ on the same positions: ||dF||_2 / ||F||_2 and
+    max_i |dF_i| / RMS |F|.  (At step 0 the atoms sit on the lattice, where
+    forces cancel, so the check runs after the timed steps.)"""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    md.compute_forces()
+    x, v, f, t, i = md.arrays()
+    n = md.n_local
+    off, idx = O.neigh_build(x, n, 2.8, half=False)
+    lj1, lj2, cs = O.lj_coeff(1, [(1, 1, 1.0, 1.0, 2.5)])
+    fo = O.lj_force(x, t, n, 1, lj1, lj2, cs, off, idx, False)[:n]
+    d = f[:n] - fo
+    rms = float(np.sqrt((fo ** 2).sum(axis=1).mean()))
+    return dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(fo)),
+                max_over_rms=float(np.sqrt((d ** 2).sum(axis=1)).max() / rms))
+
+
+def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     p = M.lattice_params(ncells=ncells, half=half, thermo=100,
                          nsteps=steps)
     md = M.CbnMD(p, precision=prec, lanes=lanes)
@@ -52,6 +72,7 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0):
     M._lib.call("cmd_device_synchronize")
     dt = time.perf_counter() - t0
     launches, mean_ms = md.kernel_stats()
+    ferr = _force_error(md) if accuracy else None
     th = md.thermo()
     e = th[:, 3]
     out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
@@ -59,6 +80,7 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0):
                kernel_mean_ms=mean_ms, init_s=t_init,
                lanes=md.force.lanes,
                energy_drift=float(abs(e[-1] - e[0]) / abs(e[0])),
+               force_error_vs_fp64_oracle=ferr,
                timers=md.timers())
     del md
     return out
@@ -123,7 +145,8 @@ def main():
     for v in variants:
         half = v == "half"
         prec = M.PREC_MIXED if v == "mixed" else M.PREC_FP64
-        res[v] = _variant(M, nc, args.steps, args.warmup, half, prec)
+        res[v] = _variant(M, nc, args.steps, args.warmup, half, prec,
+                          accuracy=not args.no_cpu)
         print(f"# {v}: {res[v]['atom_steps_per_s']:.4e} atom-steps/s, "
               f"{res[v]['ms_per_step']:.4f} ms/step, kernel "
               f"{res[v]['kernel_mean_ms']:.4f} ms x {res[v]['kernel_launches']}"
@@ -167,7 +190,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
-            "kernel": "k_lj_full (fused force + NVE), mean of "
+            "kernel": "k_tile_force (LDS-tiled full-list LJ fused with "
+                      "the NVE half-kicks, drift and wrap), mean of "
                       f"{h['kernel_launches']} launches = "
                       f"{h['kernel_mean_ms']:.4f} ms, "
                       f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},

@@ -465,6 +465,15 @@ class CbnMD:
         call("cmd_md_run", self._h,
              int(self.params.nsteps if nsteps is None else nsteps))
 
+    def compute_forces(self):
+        """Force::compute on the current positions with the current list
+        (and the ghost-force fold for a half list), as at a rebuild-free
+        step: leaves the per-atom forces readable by arrays()."""
+        call("cmd_force_compute", self.force.handle, self._sys,
+             self.neighbor.handle)
+        if self.params.half:
+            call("cmd_comm_update_force", self._sys)
+
     def arrays(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         call("cmd_system_counts", self._sys, C.byref(a), C.byref(b),

@@ -199,7 +199,10 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         ctl.do_next_initial = last ? 0 : 1;
         ctl.wrap_next = ( !last && ( ( s + 1 ) % p.every == 0 ) ) ? 1 : 0;
         ctl.thermo = th ? 1 : 0;
-        ctl.write_f = 1;
+        // the fused kernel keeps this step's forces in registers; they are
+        // stored only at the end of the call, for the next call's first
+        // half-kick and for downloads
+        ctl.write_f = half || last ? 1 : 0;
         const int n = S->n_local;
         const int G = N->lanes;
         const int nbA = std::max( N->tiled ? N->tg.ntiles : 1,

@@ -251,35 +251,57 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
                 }
                 const int sj = valid ? rs[lo] + ( v - rp[lo] ) : si0 + a0;
                 const double pjx = s_x[sj], pjy = s_y[sj], pjz = s_z[sj];
-                for ( int l = 0; l < na; ++l )
+                // two atoms per pass: independent distance chains in flight
+                for ( int l = 0; l < na; l += 2 )
                 {
-                    const int si = si0 + a0 + l;
-                    const double pix = s_x[si], piy = s_y[si], piz = s_z[si];
-                    const double dx = pix - pjx;
-                    const double dy = piy - pjy;
-                    const double dz = piz - pjz;
-                    const double rsq = dx * dx + dy * dy + dz * dz;
-                    bool acc = valid & ( sj != si ) & ( rsq <= rc2 );
+                    const bool two = l + 1 < na;
+                    const int l2 = two ? l + 1 : l;
+                    const int s1 = si0 + a0 + l, s2 = si0 + a0 + l2;
+                    const double ax = s_x[s1], ay = s_y[s1], az = s_z[s1];
+                    const double bx = s_x[s2], by = s_y[s2], bz = s_z[s2];
+                    const double dx1 = ax - pjx, dy1 = ay - pjy, dz1 = az - pjz;
+                    const double dx2 = bx - pjx, dy2 = by - pjy, dz2 = bz - pjz;
+                    const double r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+                    const double r2 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
+                    bool acc1 = valid & ( sj != s1 ) & ( r1 <= rc2 );
+                    bool acc2 = two & valid & ( sj != s2 ) & ( r2 <= rc2 );
                     if ( HALF )
-                        acc = acc & ( ( pjx > pix ) |
-                                      ( ( pjx == pix ) &
-                                        ( ( pjy > piy ) | ( ( pjy == piy ) & ( pjz > piz ) ) ) ) );
-                    const unsigned long long m = __ballot( acc );
-                    const int cl = __builtin_amdgcn_readlane( cnt, l );
-                    const unsigned k =
-                        (unsigned)cl + __builtin_amdgcn_mbcnt_hi(
-                                           (unsigned)( m >> 32 ),
-                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) );
-                    if ( acc && k < (unsigned)cap )
                     {
-                        const unsigned ta = (unsigned)( a0 + l - tl );
+                        acc1 = acc1 & ( ( pjx > ax ) |
+                                        ( ( pjx == ax ) &
+                                          ( ( pjy > ay ) | ( ( pjy == ay ) & ( pjz > az ) ) ) ) );
+                        acc2 = acc2 & ( ( pjx > bx ) |
+                                        ( ( pjx == bx ) &
+                                          ( ( pjy > by ) | ( ( pjy == by ) & ( pjz > bz ) ) ) ) );
+                    }
+                    const unsigned long long m1 = __ballot( acc1 );
+                    const unsigned long long m2 = __ballot( acc2 );
+                    const int c1 = __builtin_amdgcn_readlane( cnt, l );
+                    const int c2 = __builtin_amdgcn_readlane( cnt, l2 );
+                    const unsigned k1 =
+                        (unsigned)c1 + __builtin_amdgcn_mbcnt_hi(
+                                           (unsigned)( m1 >> 32 ),
+                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m1, 0u ) );
+                    const unsigned k2 =
+                        (unsigned)c2 + __builtin_amdgcn_mbcnt_hi(
+                                           (unsigned)( m2 >> 32 ),
+                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m2, 0u ) );
+                    auto put = [&]( int ll, unsigned k )
+                    {
+                        const unsigned ta = (unsigned)( a0 + ll - tl );
                         const unsigned r = k / G;
                         const unsigned L = ( k % G ) * APW + ta % APW;
                         chunk[( ( ( ta / APW ) * R4 + ( r >> 2 ) ) * 64 + L ) * 4 + ( r & 3 )] =
                             (unsigned short)sj;
-                    }
+                    };
+                    if ( acc1 && k1 < (unsigned)cap )
+                        put( l, k1 );
+                    if ( acc2 && k2 < (unsigned)cap )
+                        put( l2, k2 );
                     if ( lane == l )
-                        cnt = cl + (int)__popcll( m );
+                        cnt = c1 + (int)__popcll( m1 );
+                    if ( two && lane == l2 )
+                        cnt = c2 + (int)__popcll( m2 );
                 }
             }
             if ( lane < na )
