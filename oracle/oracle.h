@@ -178,6 +178,30 @@ int orc_md_thermo( const orc_md *md, int max_rows, double *rows );
 void orc_md_timers( const orc_md *md, double out[6] );
 int64_t orc_md_nnz( const orc_md *md );
 
+/* ---- NNP (force_nnp_cabana_neigh_impl.h:41-130 + n2p2 definitions) -------
+ * See nnp.cpp for the symmetry-function, scaling and network definitions.
+ * Elements are 0-based; e1 = e2 = -1 on a type-3 function accepts any pair. */
+typedef struct orc_nnp orc_nnp;
+orc_nnp *orc_nnp_create( int nelem, int nlayers, const int *nodes_hidden,
+                         const char *activations );
+void orc_nnp_destroy( orc_nnp *m );
+void orc_nnp_set_scaling_range( orc_nnp *m, int scale, double smin, double smax );
+int orc_nnp_add_sf( orc_nnp *m, int elem, int type, int e1, int e2, double eta,
+                    double rs, double lambda, double zeta, double rc,
+                    double gmin, double gmax );
+void orc_nnp_set_layer( orc_nnp *m, int elem, int layer, int n_out, int n_in,
+                        const double *W, const double *b );
+void orc_nnp_set_atom_energy( orc_nnp *m, int elem, double e );
+int orc_nnp_nsf( const orc_nnp *m, int elem );
+void orc_nnp_symmetry_functions( const orc_nnp *m, int n_local, const double *x,
+                                 const int *type, const int64_t *offsets,
+                                 const int *idx, int ld, double *G );
+double orc_nnp_compute( const orc_nnp *m, int n_local, int n_total,
+                        const double *x, const int *type,
+                        const int64_t *offsets, const int *idx, int ld,
+                        double *G, double *dEdG, double *Ei, double *f );
+double orc_nnp_reference_energy( const orc_nnp *m, int n_local, double sum_ei );
+
 #ifdef __cplusplus
 }
 #endif

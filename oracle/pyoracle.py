@@ -383,3 +383,77 @@ class MD:
         lib().orc_md_timers(self.h, dp(t))
         return dict(zip(["force", "neigh", "comm", "int", "other", "total"],
                         t))
+
+
+# ---- NNP (oracle/nnp.cpp) -----------------------------------------------
+class NNP:
+    """FP64 Behler-Parrinello model of the oracle (nnp.cpp)."""
+
+    def __init__(self, nelem, hidden, activations, scale=True, smin=0.0,
+                 smax=1.0):
+        L = lib()
+        L.orc_nnp_create.restype = C.c_void_p
+        L.orc_nnp_compute.restype = C.c_double
+        L.orc_nnp_reference_energy.restype = C.c_double
+        hid = (C.c_int * max(len(hidden), 1))(*hidden)
+        self.h = C.c_void_p(L.orc_nnp_create(
+            C.c_int(nelem), C.c_int(len(hidden) + 1), hid,
+            activations.encode()))
+        self.nelem = nelem
+        self.hidden = list(hidden)
+        L.orc_nnp_set_scaling_range(self.h, C.c_int(int(scale)),
+                                    C.c_double(smin), C.c_double(smax))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_nnp_destroy(self.h)
+            self.h = None
+
+    def add_sf(self, elem, typ, e1, e2, eta, rs, lam, zeta, rc, gmin=0.0,
+               gmax=1.0):
+        return lib().orc_nnp_add_sf(
+            self.h, C.c_int(elem), C.c_int(typ), C.c_int(e1), C.c_int(e2),
+            C.c_double(eta), C.c_double(rs), C.c_double(lam),
+            C.c_double(zeta), C.c_double(rc), C.c_double(gmin),
+            C.c_double(gmax))
+
+    def set_layer(self, elem, layer, W, b):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        lib().orc_nnp_set_layer(self.h, C.c_int(elem), C.c_int(layer),
+                                C.c_int(W.shape[0]), C.c_int(W.shape[1]),
+                                dp(W), dp(b))
+
+    def set_atom_energy(self, elem, e):
+        lib().orc_nnp_set_atom_energy(self.h, C.c_int(elem), C.c_double(e))
+
+    def nsf(self, elem):
+        return lib().orc_nnp_nsf(self.h, C.c_int(elem))
+
+    def symmetry_functions(self, x, type_, n_local, off, idx, ld=None):
+        ld = ld or max(self.nsf(e) for e in range(self.nelem))
+        G = np.zeros((n_local, ld))
+        lib().orc_nnp_symmetry_functions(
+            self.h, C.c_int(n_local), dp(np.ascontiguousarray(x)),
+            ip(np.ascontiguousarray(type_, dtype=np.int32)), lp(off),
+            ip(idx), C.c_int(ld), dp(G))
+        return G
+
+    def compute(self, x, type_, n_local, off, idx, ld=None):
+        """(sum E_i, E_i, G, dE/dG, f over all atoms incl. ghosts)."""
+        ld = ld or max(self.nsf(e) for e in range(self.nelem))
+        nt = x.shape[0]
+        G = np.zeros((n_local, ld))
+        dG = np.zeros((n_local, ld))
+        Ei = np.zeros(n_local)
+        f = np.zeros((nt, 3))
+        e = lib().orc_nnp_compute(
+            self.h, C.c_int(n_local), C.c_int(nt),
+            dp(np.ascontiguousarray(x)),
+            ip(np.ascontiguousarray(type_, dtype=np.int32)), lp(off),
+            ip(idx), C.c_int(ld), dp(G), dp(dG), dp(Ei), dp(f))
+        return e, Ei, G, dG, f
+
+    def reference_energy(self, n_local, sum_ei):
+        return lib().orc_nnp_reference_energy(self.h, C.c_int(n_local),
+                                              C.c_double(sum_ei))
