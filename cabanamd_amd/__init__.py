@@ -266,6 +266,173 @@ class NeighborVerlet:
                 else "Neighbor:HIPVerletFull")
 
 
+class NNPModel:
+    """The model ForceNNP::init_coeff loads (force_nnp_cabana_neigh_impl.h:
+    41-66): symmetry functions, scaling ranges and per-element networks."""
+
+    def __init__(self, handle, own=True, owner=None):
+        self._h = handle
+        self._own = own
+        self._owner = owner  # keeps the object that owns a borrowed model
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self, "_own", True):
+            lib().cmd_nnp_model_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def _borrowed(cls, handle, owner):
+        return cls(handle, own=False, owner=owner)
+
+    @classmethod
+    def create(cls, nelem, hidden, activations):
+        h = C.c_void_p()
+        hid = (C.c_int * max(len(hidden), 1))(*hidden)
+        call("cmd_nnp_model_create", nelem, len(hidden), hid,
+             activations.encode(), C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def load_n2p2(cls, path, elements, atomic_numbers):
+        h = C.c_void_p()
+        names = (C.c_char_p * len(elements))(*[e.encode() for e in elements])
+        z = (C.c_int * len(elements))(*atomic_numbers)
+        call("cmd_nnp_model_load_n2p2", str(path).encode(), len(elements),
+             names, z, C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def synthetic(cls, seed=87287, rc=6.0):
+        h = C.c_void_p()
+        call("cmd_nnp_model_synthetic", C.c_uint64(seed), rc, C.byref(h))
+        return cls(h)
+
+    def set_scaling(self, scale=True, smin=0.0, smax=1.0):
+        call("cmd_nnp_model_set_scaling", self._h, int(scale), smin, smax)
+
+    def add_sf(self, elem, typ, e1, e2, eta, rs, lam, zeta, rc, gmin=0.0,
+               gmax=1.0):
+        call("cmd_nnp_model_add_sf", self._h, elem, typ, e1, e2, eta, rs,
+             lam, zeta, rc, gmin, gmax)
+
+    def set_layer(self, elem, layer, W, b):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        call("cmd_nnp_model_set_layer", self._h, elem, layer, W.shape[0],
+             W.shape[1], dp(W), dp(b))
+
+    def set_atom_energy(self, elem, e):
+        call("cmd_nnp_model_set_atom_energy", self._h, elem, e)
+
+    def describe(self, nelem):
+        """The model as a plain dict (the layout of tests/nnp_models.py)."""
+        out = dict(nelem=nelem, sf=[], gmin=[], gmax=[], W=[], b=[],
+                   atom_energy=[])
+        for e in range(nelem):
+            ns, nh = C.c_int(), C.c_int()
+            hid = (C.c_int * 8)()
+            ae = C.c_double()
+            call("cmd_nnp_model_info", self._h, e, C.byref(ns), C.byref(nh),
+                 hid, C.byref(ae))
+            hidden = list(hid[:nh.value])
+            out["hidden"] = hidden
+            sf, gmin, gmax = [], [], []
+            prm = (C.c_double * 10)()
+            for k in range(ns.value):
+                call("cmd_nnp_model_get_sf", self._h, e, k, prm)
+                v = list(prm)
+                sf.append((int(v[0]), int(v[1]), int(v[2]), v[3], v[4], v[5],
+                           v[6], v[7]))
+                gmin.append(v[8])
+                gmax.append(v[9])
+            sizes = [ns.value] + hidden + [1]
+            Ws, bs = [], []
+            for l in range(len(sizes) - 1):
+                W = np.zeros((sizes[l + 1], sizes[l]))
+                b = np.zeros(sizes[l + 1])
+                call("cmd_nnp_model_get_layer", self._h, e, l, dp(W), dp(b))
+                Ws.append(W)
+                bs.append(b)
+            out["sf"].append(sf)
+            out["gmin"].append(np.array(gmin))
+            out["gmax"].append(np.array(gmax))
+            out["W"].append(Ws)
+            out["b"].append(bs)
+            out["atom_energy"].append(ae.value)
+        return out
+
+
+class ForceNNP:
+    """ForceNNP (force_types/force_nnp_cabana_neigh_impl.h:41-130)."""
+
+    def __init__(self, model, precision=PREC_MIXED):
+        h = C.c_void_p()
+        call("cmd_force_nnp_create", model.handle, C.byref(h))
+        self._h = h
+        call("cmd_force_set_precision", h, precision)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cmd_force_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, prec):
+        call("cmd_force_set_precision", self._h, prec)
+
+    def fit_scaling(self, system, neighbor):
+        call("cmd_force_nnp_fit_scaling", self._h, system.handle,
+             neighbor.handle)
+
+    def model(self):
+        m = C.c_void_p()
+        call("cmd_force_nnp_model", self._h, C.byref(m))
+        # a view into this force's model: it keeps the force alive
+        return NNPModel._borrowed(m, self)
+
+    def compute(self, system, neighbor):
+        call("cmd_force_compute", self._h, system.handle, neighbor.handle)
+
+    def compute_energy(self, system, neighbor):
+        e = C.c_double()
+        call("cmd_force_energy", self._h, system.handle, neighbor.handle,
+             C.byref(e))
+        return e.value
+
+    def atomic(self, system, ld):
+        n = system.N_local
+        G = np.zeros((n, ld), dtype=np.float32)
+        D = np.zeros((n, ld), dtype=np.float32)
+        E = np.zeros(n, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        call("cmd_force_nnp_atomic", self._h, system.handle, ld,
+             G.ctypes.data_as(fp), D.ctypes.data_as(fp), E.ctypes.data_as(fp))
+        return G, D, E
+
+    def name(self):
+        return "Force:NNPHIP"
+
+
+def nnp_lattice(cells, spacing=2.5, jitter=0.1, frac1=1.0 / 3.0, seed=87287):
+    """Jittered simple-cubic sites with a seeded 2:1 element split."""
+    n = C.c_int()
+    call("cmd_nnp_lattice_create", cells[0], cells[1], cells[2], spacing,
+         jitter, frac1, C.c_uint64(seed), C.byref(n), None, None, None)
+    x = np.zeros((n.value, 3))
+    t = np.zeros(n.value, dtype=np.int32)
+    i = np.zeros(n.value, dtype=np.int32)
+    call("cmd_nnp_lattice_create", cells[0], cells[1], cells[2], spacing,
+         jitter, frac1, C.c_uint64(seed), C.byref(n), dp(x), ip(t), ip(i))
+    return x, t, i
+
+
 class ForceLJ:
     """ForceLJ (force_types/force_lj_cabana_neigh_impl.h:49-377)."""
 
@@ -393,6 +560,32 @@ class InputFile:
         return x, v, t, i
 
 
+def nnp_params(ntypes=2, cutoff=6.0, skin=0.3, dt=0.0005, nsteps=200,
+               thermo=20, every=20, mass=(1.0, 1.0), max_neigh_guess=100):
+    """Run parameters for an NNP system whose atoms come from
+    CbnMD.set_initial_state and whose model from CbnMD.set_nnp_model
+    (pair_style nnp, pair_coeff * * cutoff; reduced units)."""
+    p = Params()
+    call("cmd_params_default", C.byref(p))
+    p.units = 1
+    p.ntypes = ntypes
+    for k in range(ntypes):
+        p.mass[k] = mass[k]
+    p.force_type = 2
+    p.force_cutoff = cutoff
+    p.skin = skin
+    p.every = every
+    p.max_neigh_guess = max_neigh_guess
+    p.comm_ghost_cutoff = cutoff + skin
+    p.half = 0
+    p.nsteps = nsteps
+    p.thermo_rate = thermo
+    p.dt = dt
+    p.mvv2e = 1.0
+    p.boltz = 1.0
+    return p
+
+
 def lattice_params(ncells=(20, 20, 20), rho=0.8442, temp=1.44, seed=87287,
                    mass=1.0, cutoff=2.5, skin=0.3, every=20, half=False,
                    nsteps=100, thermo=10, dt=0.005, gaussian=False,
@@ -447,6 +640,22 @@ class CbnMD:
         if getattr(self, "_h", None):
             lib().cmd_md_destroy(self._h)
             self._h = None
+
+    def set_nnp_model(self, model):
+        """Use this NNP model instead of the deck's pair_style nnp dir."""
+        call("cmd_md_set_nnp_model", self._h, model.handle)
+
+    def set_initial_state(self, x, type, lo, hi, v=None, id=None):
+        """Start from these atoms in the box [lo, hi) instead of the deck's
+        lattice and velocities."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        t = np.ascontiguousarray(type, dtype=np.int32)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        ii = None if id is None else np.ascontiguousarray(id, dtype=np.int32)
+        call("cmd_md_set_initial_state", self._h, n, dp(x),
+             None if vv is None else dp(vv), ip(t),
+             None if ii is None else ip(ii), d3(lo), d3(hi))
 
     def init(self):
         call("cmd_md_init", self._h)

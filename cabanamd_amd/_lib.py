@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcabanamd_amd.so")
 
 DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
 IP = C.POINTER(C.c_int)
 LP = C.POINTER(C.c_int64)
 VP = C.c_void_p
@@ -106,6 +107,30 @@ _PROTOS = {
     "cmd_force_compute": ([VP, VP, VP], None),
     "cmd_force_energy": ([VP, VP, VP, DP], None),
     "cmd_force_virial": ([VP, VP, VP, DP], None),
+    "cmd_nnp_model_create": ([C.c_int, C.c_int, IP, C.c_char_p,
+                              C.POINTER(VP)], None),
+    "cmd_nnp_model_destroy": ([VP], None),
+    "cmd_nnp_model_set_scaling": ([VP, C.c_int, C.c_double, C.c_double], None),
+    "cmd_nnp_model_add_sf": ([VP, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_double], None),
+    "cmd_nnp_model_set_layer": ([VP, C.c_int, C.c_int, C.c_int, C.c_int, DP,
+                                 DP], None),
+    "cmd_nnp_model_set_atom_energy": ([VP, C.c_int, C.c_double], None),
+    "cmd_nnp_model_load_n2p2": ([C.c_char_p, C.c_int,
+                                 C.POINTER(C.c_char_p), IP, C.POINTER(VP)],
+                                None),
+    "cmd_nnp_model_synthetic": ([C.c_uint64, C.c_double, C.POINTER(VP)], None),
+    "cmd_nnp_model_info": ([VP, C.c_int, IP, IP, IP, DP], None),
+    "cmd_nnp_model_get_sf": ([VP, C.c_int, C.c_int, DP], None),
+    "cmd_nnp_model_get_layer": ([VP, C.c_int, C.c_int, DP, DP], None),
+    "cmd_force_nnp_create": ([VP, C.POINTER(VP)], None),
+    "cmd_force_nnp_fit_scaling": ([VP, VP, VP], None),
+    "cmd_force_nnp_model": ([VP, C.POINTER(VP)], None),
+    "cmd_force_nnp_atomic": ([VP, VP, C.c_int, FP, FP, FP], None),
+    "cmd_nnp_lattice_create": ([C.c_int, C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.c_double, C.c_uint64, IP, DP,
+                                IP, IP], None),
     "cmd_thermo_temperature": ([VP, DP], None),
     "cmd_thermo_kinetic": ([VP, DP], None),
     "cmd_params_default": ([C.POINTER(Params)], None),
@@ -116,6 +141,8 @@ _PROTOS = {
     "cmd_md_destroy": ([VP], None),
     "cmd_md_set_precision": ([VP, C.c_int], None),
     "cmd_md_set_lanes": ([VP, C.c_int], None),
+    "cmd_md_set_nnp_model": ([VP, VP], None),
+    "cmd_md_set_initial_state": ([VP, C.c_int, DP, DP, IP, IP, DP, DP], None),
     "cmd_md_init": ([VP], None),
     "cmd_md_run": ([VP, C.c_int], None),
     "cmd_md_system": ([VP, C.POINTER(VP)], None),

@@ -281,9 +281,12 @@ struct LJParams
     int ntypes;
 };
 
+struct cmd_nnp;
+
 struct cmd_force
 {
-    int kind = 0; // 0 lj
+    int kind = 0; // 0 lj, 2 nnp
+    cmd_nnp *nnp = nullptr; // NNP model, plan and per-atom buffers (nnp.hip)
     int ntypes = 1;
     double lj1[CMD_MAX_TYPES * CMD_MAX_TYPES];
     double lj2[CMD_MAX_TYPES * CMD_MAX_TYPES];
@@ -318,6 +321,17 @@ cmd_status lj_half( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
                     int thermo, double *partial, int *nblocks_out );
 cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
                              const cmd_neigh *N, double *pe, double *w );
+
+// NNP force (nnp.hip)
+cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N );
+cmd_status nnp_reference_energy( cmd_nnp *M, cmd_system *S, double *pe );
+cmd_status nnp_energy_sum( cmd_nnp *M, cmd_system *S, double *sum );
+cmd_status nnp_destroy( cmd_nnp *M );
+cmd_status nnp_set_precision( cmd_nnp *M, int prec );
+cmd_status nnp_model_copy( const cmd_nnp_model *m, cmd_nnp_model **out );
+// The model of a pair_style nnp deck: the n2p2 directory, elements from its
+// input.nn 'elements' line, weights files by atomic number.
+cmd_status nnp_load_deck_model( const char *dir, cmd_nnp_model **out );
 
 // Integrator kernels (core.hip)
 cmd_status nve_final_initial( cmd_system *S, const StepCtl &ctl,

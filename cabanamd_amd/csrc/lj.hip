@@ -1119,6 +1119,8 @@ cmd_status cmd_force_lj_create( int ntypes, cmd_force **out )
 
 cmd_status cmd_force_destroy( cmd_force *f )
 {
+    if ( f && f->nnp )
+        nnp_destroy( f->nnp );
     delete f;
     return CMD_OK;
 }
@@ -1147,6 +1149,8 @@ cmd_status cmd_force_set_precision( cmd_force *f, int prec )
     CMD_REQUIRE( prec == CMD_PREC_FP64 || prec == CMD_PREC_MIXED, CMD_ERR_ARG,
                  "unknown precision %d", prec );
     f->prec = prec;
+    if ( f->kind == 2 )
+        return nnp_set_precision( f->nnp, prec );
     return CMD_OK;
 }
 
@@ -1174,6 +1178,8 @@ cmd_status cmd_force_compute( cmd_force *f, cmd_system *s, cmd_neigh *n )
     CMD_REQUIRE( n->n_rows == s->n_local, CMD_ERR_STATE,
                  "neighbor list has %d rows but the system has %d local atoms",
                  n->n_rows, s->n_local );
+    if ( f->kind == 2 )
+        return nnp_compute( f->nnp, s, n );
     return force_compute_plain( f, s, n );
 }
 
@@ -1181,6 +1187,8 @@ cmd_status cmd_force_energy( cmd_force *f, cmd_system *s, cmd_neigh *n,
                              double *pe )
 {
     CMD_REQUIRE( f && s && n && pe, CMD_ERR_ARG, "NULL argument" );
+    if ( f->kind == 2 )
+        return nnp_reference_energy( f->nnp, s, pe );
     return lj_energy_virial( f, s, n, pe, nullptr );
 }
 
@@ -1188,6 +1196,7 @@ cmd_status cmd_force_virial( cmd_force *f, cmd_system *s, cmd_neigh *n,
                              double *w )
 {
     CMD_REQUIRE( f && s && n && w, CMD_ERR_ARG, "NULL argument" );
+    CMD_REQUIRE( f->kind == 0, CMD_ERR_ARG, "the virial is computed for LJ only" );
     return lj_energy_virial( f, s, n, nullptr, w );
 }
 

@@ -42,6 +42,11 @@ struct cmd_md
     int prec = CMD_PREC_FP64;
     int lanes = 0;
     std::vector<double> thermo; // rows of 5: step, T, PE/N, (PE+KE)/N, W
+    cmd_nnp_model *nnp_model = nullptr; // injected NNP model (owned)
+    bool custom_state = false;          // initial state given by the caller
+    std::vector<double> init_x, init_v;
+    std::vector<int> init_type, init_id;
+    double init_lo[3] = { 0, 0, 0 }, init_hi[3] = { 0, 0, 0 };
     cmd::DevBuf<double> partA, partB, trows;
     double timers[6] = { 0, 0, 0, 0, 0, 0 };
     bool profile = false;
@@ -67,6 +72,7 @@ struct cmd_md
         cmd_force_destroy( F );
         cmd_neigh_destroy( N );
         cmd_system_destroy( S );
+        cmd_nnp_model_destroy( nnp_model );
     }
     hipEvent_t get_event()
     {
@@ -152,8 +158,71 @@ static void thermo_row( cmd_md *md, int step, double mv2, double pe, double w )
     md->thermo.push_back( w );
 }
 
+static cmd_status thermo_now( cmd_md *md, int step )
+{
+    cmd_system *S = md->S;
+    double T = 0.0, pe = 0.0, w = 0.0, ke = 0.0;
+    CMD_TRY( cmd_thermo_temperature( S, &T ) );
+    if ( md->F->kind == 2 )
+        CMD_TRY( cmd_force_energy( md->F, S, md->N, &pe ) );
+    else
+        CMD_TRY( lj_energy_virial( md->F, S, md->N, &pe, &w ) );
+    CMD_TRY( cmd_thermo_kinetic( S, &ke ) );
+    thermo_row( md, step, ke / ( 0.5 * S->mvv2e ), pe, w );
+    return CMD_OK;
+}
+
+// The reference's step order without fusion (cabanamd_impl.h:285-399), used
+// for the NNP force, whose cost dwarfs the integrator's.
+static cmd_status run_steps_plain( cmd_md *md, int nsteps )
+{
+    cmd_system *S = md->S;
+    const cmd_params &p = md->p;
+    const int s0 = md->step + 1, s1 = md->step + nsteps;
+    auto t0 = std::chrono::steady_clock::now();
+    for ( int s = s0; s <= s1; ++s )
+    {
+        int k = md->begin( PH_INT );
+        CMD_TRY( cmd_nve_initial( S ) );
+        md->end( k );
+        if ( s % p.every == 0 )
+        {
+            k = md->begin( PH_COMM );
+            CMD_TRY( cmd_comm_exchange( S ) );
+            md->end( k );
+            CMD_TRY( rebuild( md ) );
+        }
+        else
+        {
+            k = md->begin( PH_COMM );
+            CMD_TRY( cmd_comm_update_halo( S ) );
+            md->end( k );
+        }
+        k = md->begin( PH_FORCE );
+        CMD_TRY( cmd_force_compute( md->F, S, md->N ) );
+        md->end( k );
+        k = md->begin( PH_COMM );
+        CMD_TRY( cmd_comm_update_force( S ) );
+        md->end( k );
+        k = md->begin( PH_INT );
+        CMD_TRY( cmd_nve_final( S ) );
+        md->end( k );
+        if ( p.thermo_rate > 0 && s % p.thermo_rate == 0 )
+            CMD_TRY( thermo_now( md, s ) );
+    }
+    CMD_HIP( hipStreamSynchronize( S->stream ) );
+    md->timers[5] +=
+        std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 )
+            .count();
+    md->collect();
+    md->step = s1;
+    return CMD_OK;
+}
+
 static cmd_status run_steps( cmd_md *md, int nsteps )
 {
+    if ( md->F->kind == 2 )
+        return run_steps_plain( md, nsteps );
     cmd_system *S = md->S;
     cmd_neigh *N = md->N;
     cmd_force *F = md->F;
@@ -279,9 +348,8 @@ extern "C" {
 cmd_status cmd_md_create( const cmd_params *p, cmd_md **out )
 {
     CMD_REQUIRE( p && out, CMD_ERR_ARG, "NULL argument" );
-    CMD_REQUIRE( p->force_type == 0, CMD_ERR_ARG,
-                 "cmd_md drives pair_style lj/cut; use the NNP entry points "
-                 "for pair_style nnp" );
+    CMD_REQUIRE( p->force_type == 0 || p->force_type == 2, CMD_ERR_ARG,
+                 "pair_style must be lj/cut or nnp" );
     CMD_REQUIRE( p->every > 0, CMD_ERR_ARG, "neigh_modify every must be > 0" );
     CMD_REQUIRE( p->ntypes >= 1 && p->ntypes <= CMD_MAX_TYPES, CMD_ERR_ARG,
                  "ntypes %d outside [1,%d]", p->ntypes, CMD_MAX_TYPES );
@@ -317,6 +385,40 @@ cmd_status cmd_md_set_lanes( cmd_md *md, int lanes )
     return CMD_OK;
 }
 
+cmd_status cmd_md_set_nnp_model( cmd_md *md, const cmd_nnp_model *m )
+{
+    CMD_REQUIRE( md && m, CMD_ERR_ARG, "NULL argument" );
+    CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "the model is set before init" );
+    cmd_nnp_model_destroy( md->nnp_model );
+    md->nnp_model = nullptr;
+    return nnp_model_copy( m, &md->nnp_model );
+}
+
+cmd_status cmd_md_set_initial_state( cmd_md *md, int n, const double *x,
+                                     const double *v, const int *type,
+                                     const int *id, const double lo[3],
+                                     const double hi[3] )
+{
+    CMD_REQUIRE( md && x && type && lo && hi && n > 0, CMD_ERR_ARG,
+                 "bad initial state" );
+    CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "the state is set before init" );
+    md->custom_state = true;
+    md->init_x.assign( x, x + 3 * (size_t)n );
+    md->init_v.assign( 3 * (size_t)n, 0.0 );
+    if ( v )
+        md->init_v.assign( v, v + 3 * (size_t)n );
+    md->init_type.assign( type, type + n );
+    md->init_id.resize( n );
+    for ( int i = 0; i < n; ++i )
+        md->init_id[i] = id ? id[i] : i + 1;
+    for ( int k = 0; k < 3; ++k )
+    {
+        md->init_lo[k] = lo[k];
+        md->init_hi[k] = hi[k];
+    }
+    return CMD_OK;
+}
+
 cmd_status cmd_md_init( cmd_md *md )
 {
     CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
@@ -329,15 +431,36 @@ cmd_status cmd_md_init( cmd_md *md )
         low[k] = p.lattice_constant * p.region_lo[k];
         high[k] = p.lattice_constant * p.region_hi[k];
     }
+    if ( md->custom_state )
+        for ( int k = 0; k < 3; ++k )
+        {
+            low[k] = md->init_lo[k];
+            high[k] = md->init_hi[k];
+        }
     CMD_TRY( cmd_domain_create( low, high, p.comm_ghost_cutoff, &md->d ) );
     int n = 0;
-    CMD_TRY( cmd_lattice_create( &p, &md->d, &n, nullptr, nullptr, nullptr,
-                                 nullptr ) );
-    CMD_REQUIRE( n > 0, CMD_ERR_ARG, "the lattice creates no atoms" );
-    std::vector<double> x( 3 * (size_t)n ), v( 3 * (size_t)n );
-    std::vector<int> type( n ), id( n );
-    CMD_TRY( cmd_lattice_create( &p, &md->d, &n, x.data(), v.data(),
-                                 type.data(), id.data() ) );
+    std::vector<double> x, v;
+    std::vector<int> type, id;
+    if ( md->custom_state )
+    {
+        n = (int)md->init_type.size();
+        x = md->init_x;
+        v = md->init_v;
+        type = md->init_type;
+        id = md->init_id;
+    }
+    else
+    {
+        CMD_TRY( cmd_lattice_create( &p, &md->d, &n, nullptr, nullptr, nullptr,
+                                     nullptr ) );
+        CMD_REQUIRE( n > 0, CMD_ERR_ARG, "the lattice creates no atoms" );
+        x.resize( 3 * (size_t)n );
+        v.resize( 3 * (size_t)n );
+        type.resize( n );
+        id.resize( n );
+        CMD_TRY( cmd_lattice_create( &p, &md->d, &n, x.data(), v.data(),
+                                     type.data(), id.data() ) );
+    }
     CMD_TRY( cmd_system_create( p.ntypes, &md->S ) );
     cmd_system *S = md->S;
     CMD_TRY( cmd_system_set_units( S, p.dt, p.mvv2e, p.boltz ) );
@@ -346,39 +469,46 @@ cmd_status cmd_md_init( cmd_md *md )
     CMD_TRY( cmd_system_reserve( S, n + n / 2 + 1024 ) );
     CMD_TRY( cmd_system_upload( S, n, 0, x.data(), v.data(), nullptr,
                                 type.data(), id.data() ) );
-    CMD_TRY( cmd_force_lj_create( p.ntypes, &md->F ) );
-    for ( int c = 0; c < p.ncoeff; ++c )
-        CMD_TRY( cmd_force_lj_init_coeff( md->F, p.coeff_i[c], p.coeff_j[c],
-                                          p.coeff_eps[c], p.coeff_sigma[c],
-                                          p.coeff_cut[c] ) );
+    const bool nnp = p.force_type == 2;
+    if ( nnp )
+    {
+        // ForceNNP::init_coeff (force_nnp_cabana_neigh_impl.h:41-66): the
+        // model directory of pair_style nnp, unless a model was given
+        if ( !md->nnp_model )
+            CMD_TRY( nnp_load_deck_model( p.nnp_dir, &md->nnp_model ) );
+        CMD_TRY( cmd_force_nnp_create( md->nnp_model, &md->F ) );
+    }
+    else
+    {
+        CMD_TRY( cmd_force_lj_create( p.ntypes, &md->F ) );
+        for ( int c = 0; c < p.ncoeff; ++c )
+            CMD_TRY( cmd_force_lj_init_coeff( md->F, p.coeff_i[c], p.coeff_j[c],
+                                              p.coeff_eps[c], p.coeff_sigma[c],
+                                              p.coeff_cut[c] ) );
+        CMD_TRY( cmd_force_set_lanes( md->F, md->lanes ) );
+    }
     CMD_TRY( cmd_force_set_precision( md->F, md->prec ) );
-    CMD_TRY( cmd_force_set_lanes( md->F, md->lanes ) );
     CMD_TRY( cmd_neigh_create( md->neigh_cut, p.half, p.max_neigh_guess,
                                &md->N ) );
-    if ( md->lanes > 0 )
+    if ( nnp )
+        md->N->allow_tiles = false; // the NNP kernels read the CSR view
+    else if ( md->lanes > 0 )
         CMD_TRY( cmd_neigh_set_lanes( md->N, md->lanes ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
     CMD_TRY( cmd_comm_exchange( S ) );
     CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
                                  md->neigh_cut, 1, 1, 0, 1, nullptr ) );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
-    if ( md->lanes == 0 && !p.half )
+    if ( md->lanes == 0 && !p.half && !nnp )
         CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
     else
         CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->F->lanes = md->N->lanes;
-    CMD_TRY( force_compute_plain( md->F, S, md->N ) );
-    if ( p.half )
+    CMD_TRY( cmd_force_compute( md->F, S, md->N ) );
+    if ( p.half || nnp )
         CMD_TRY( cmd_comm_update_force( S ) );
     if ( p.thermo_rate > 0 )
-    {
-        double T = 0.0, pe = 0.0, w = 0.0, ke = 0.0;
-        CMD_TRY( cmd_thermo_temperature( S, &T ) );
-        CMD_TRY( lj_energy_virial( md->F, S, md->N, &pe, &w ) );
-        CMD_TRY( cmd_thermo_kinetic( S, &ke ) );
-        const double mv2 = ke / ( 0.5 * S->mvv2e );
-        thermo_row( md, 0, mv2, pe, w );
-    }
+        CMD_TRY( thermo_now( md, 0 ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
     md->inited = true;
     return CMD_OK;

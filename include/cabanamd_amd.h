@@ -188,6 +188,69 @@ cmd_status cmd_force_energy( cmd_force *f, cmd_system *s, cmd_neigh *n,
 cmd_status cmd_force_virial( cmd_force *f, cmd_system *s, cmd_neigh *n,
                              double *w );
 
+/* ---- NNP force --------------------------------------------------------------
+ * ForceNNP (force_types/force_nnp_cabana_neigh_impl.h:41-130).  The model is
+ * what init_coeff loads through n2p2 (:41-66): symmetry functions (types 2
+ * and 3, cosine cutoff), min-max scaling and per-element networks.  Forces
+ * are written to local and ghost rows; fold them with cmd_comm_update_force,
+ * as the reference does after an NNP force (cabanamd_impl.h:346-353).
+ * cmd_force_set_precision: CMD_PREC_MIXED runs the network layers as BF16
+ * MFMA GEMMs with FP32 accumulation (default), CMD_PREC_FP64 runs them in
+ * FP32 on the vector units; symmetry functions and forces are FP32 either
+ * way, forces are accumulated in FP64. */
+typedef struct cmd_nnp_model cmd_nnp_model;
+
+cmd_status cmd_nnp_model_create( int nelem, int n_hidden, const int *hidden,
+                                 const char *activations, cmd_nnp_model **out );
+cmd_status cmd_nnp_model_destroy( cmd_nnp_model *m );
+cmd_status cmd_nnp_model_set_scaling( cmd_nnp_model *m, int scale, double smin,
+                                      double smax );
+/* type 2: e1 = neighbour element (-1 any); type 3: e1 = e2 = -1 (any pair). */
+cmd_status cmd_nnp_model_add_sf( cmd_nnp_model *m, int elem, int type, int e1,
+                                 int e2, double eta, double rs, double lambda,
+                                 double zeta, double rc, double gmin,
+                                 double gmax );
+/* W is [n_out][n_in] row-major. */
+cmd_status cmd_nnp_model_set_layer( cmd_nnp_model *m, int elem, int layer,
+                                    int n_out, int n_in, const double *W,
+                                    const double *b );
+cmd_status cmd_nnp_model_set_atom_energy( cmd_nnp_model *m, int elem,
+                                          double e );
+/* n2p2 model directory: input.nn, scaling.data, weights.%03d.data (by
+ * atomic number), as ForceNNP::init_coeff loads it (:48-65). */
+cmd_status cmd_nnp_model_load_n2p2( const char *dir, int nelem,
+                                    const char *const *names,
+                                    const int *atomic_numbers,
+                                    cmd_nnp_model **out );
+/* Seeded two-element benchmark model (no reference counterpart): 30 radial
+ * + 40 angular functions per element, network 70-25-25-1. */
+cmd_status cmd_nnp_model_synthetic( uint64_t seed, double rc,
+                                    cmd_nnp_model **out );
+cmd_status cmd_nnp_model_info( const cmd_nnp_model *m, int elem, int *nsf,
+                               int *n_hidden, int *hidden,
+                               double *atom_energy );
+/* params: type, e1, e2, eta, rs, lambda, zeta, rc, gmin, gmax */
+cmd_status cmd_nnp_model_get_sf( const cmd_nnp_model *m, int elem, int k,
+                                 double params[10] );
+cmd_status cmd_nnp_model_get_layer( const cmd_nnp_model *m, int elem,
+                                    int layer, double *W, double *b );
+cmd_status cmd_force_nnp_create( const cmd_nnp_model *m, cmd_force **out );
+/* Set every scaling range to the min/max over the current frame's atoms. */
+cmd_status cmd_force_nnp_fit_scaling( cmd_force *f, cmd_system *s,
+                                      cmd_neigh *n );
+cmd_status cmd_force_nnp_model( const cmd_force *f, const cmd_nnp_model **m );
+/* Per-atom G, dE/dG (ld floats per atom) and atomic energies of the last
+ * compute (System_NNP's G, dEdG and E, system_nnp_1aosoa.h:19-57). */
+cmd_status cmd_force_nnp_atomic( cmd_force *f, cmd_system *s, int ld,
+                                 float *G, float *dEdG, float *Ei );
+/* Jittered simple-cubic lattice for the NNP benchmark (no reference
+ * counterpart): nx*ny*nz sites at spacing s, each coordinate displaced by
+ * U(-jitter, jitter)*s, element 1 on round(n * frac1) sites chosen by a
+ * seeded shuffle, element 0 elsewhere.  Two-call protocol (x == NULL). */
+cmd_status cmd_nnp_lattice_create( int nx, int ny, int nz, double s,
+                                   double jitter, double frac1, uint64_t seed,
+                                   int *n, double *x, int *type, int *id );
+
 /* ---- thermo ----------------------------------------------------------------
  * Temperature / KinE (property_temperature_impl.h:55-77,
  * property_kine_impl.h:55-76). */
@@ -245,6 +308,14 @@ cmd_status cmd_md_create( const cmd_params *p, cmd_md **out );
 cmd_status cmd_md_destroy( cmd_md *md );
 cmd_status cmd_md_set_precision( cmd_md *md, int prec );
 cmd_status cmd_md_set_lanes( cmd_md *md, int lanes );
+/* Before init (no reference counterpart): an NNP model to use instead of
+ * loading pair_style nnp's directory, and an initial state (atoms and box)
+ * to use instead of the deck's lattice and velocities. */
+cmd_status cmd_md_set_nnp_model( cmd_md *md, const cmd_nnp_model *m );
+cmd_status cmd_md_set_initial_state( cmd_md *md, int n, const double *x,
+                                     const double *v, const int *type,
+                                     const int *id, const double lo[3],
+                                     const double hi[3] );
 cmd_status cmd_md_init( cmd_md *md );
 cmd_status cmd_md_run( cmd_md *md, int nsteps );
 cmd_status cmd_md_system( cmd_md *md, cmd_system **s );
