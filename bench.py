@@ -108,6 +108,91 @@ def _cpu_baseline(ncells, steps):
                         f"steps incl. one list rebuild, {dt:.2f} s"))
 
 
+NNP_SIZES = {3000: (10, 15, 20), 24000: (20, 30, 40), 192000: (40, 60, 80)}
+
+
+def _nnp_case(M, n, steps, warmup, prec, accuracy=False):
+    """The NNP benchmark configuration: two elements on a jittered simple
+    cubic lattice (spacing 2.5, jitter 0.1 s, 2:1 ratio, seeded), 30 radial +
+    40 angular symmetry functions per element, cutoff 6.0, network
+    70-25-25-1 with seeded weights and min-max scaling fitted to the initial
+    frame; T = 0.05, dt = 0.0005, list rebuilt every 20 steps."""
+    import numpy as np
+    cells = NNP_SIZES[n]
+    x, t, ids = M.nnp_lattice(cells, spacing=2.5, jitter=0.1, seed=87287)
+    L = [2.5 * c for c in cells]
+    s = M.System(2)
+    s.create_domain([0, 0, 0], L, 6.3)
+    s.upload(x, None, None, t, ids)
+    comm = M.Comm(s, 6.3)
+    comm.exchange_halo()
+    nl = M.NeighborVerlet(6.3, False, 100, lanes=1)
+    nl.create(s)
+    F = M.ForceNNP(M.NNPModel.synthetic(seed=87287, rc=6.0), M.PREC_FP64)
+    F.fit_scaling(s, nl)
+    model = F.model()
+    rng = np.random.default_rng(87287)
+    v = rng.normal(0, np.sqrt(0.05), x.shape)
+    v -= v.mean(axis=0)
+    p = M.nnp_params(nsteps=steps, thermo=max(steps // 10, 1))
+    md = M.CbnMD(p, precision=prec)
+    md.set_nnp_model(model)
+    md.set_initial_state(x, t, [0, 0, 0], L, v=v, id=ids)
+    t0 = time.time()
+    md.init()
+    t_init = time.time() - t0
+    if warmup:
+        md.run(warmup)
+    M._lib.call("cmd_device_synchronize")
+    t0 = time.perf_counter()
+    md.run(steps)
+    M._lib.call("cmd_device_synchronize")
+    dt = time.perf_counter() - t0
+    th = md.thermo()
+    pe = th[:, 2] * n / 27.211384021355236
+    ke = (th[:, 3] - th[:, 2]) * n
+    e = pe + ke
+    out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
+               atom_steps_per_s=n * steps / dt, init_s=t_init,
+               energy_drift=float(np.abs(e - e[0]).max() / np.abs(e).max()),
+               layers="bf16-mfma" if prec == M.PREC_MIXED else "fp32")
+    if accuracy:
+        out["force_error_vs_fp64_oracle"] = _nnp_force_error(M, md, model)
+    del md
+    return out
+
+
+def _nnp_force_error(M, md, model):
+    """Engine forces on the final state against the FP64 NNP oracle."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pyoracle as O
+    from nnp_models import to_oracle
+    md.compute_forces()
+    x, v, f, t, i = md.arrays()
+    n = md.n_local
+    off, idx = md.neighbor.get()
+    m = model.describe(2)
+    m.update(act="ttl", smin=0.0, smax=1.0, scale=True)
+    Mo = to_oracle(O, m)
+    _, _, _, _, fo = Mo.compute(x, t, n, off, idx, ld=96)
+    send = np.zeros(1, np.int32)
+    import ctypes as C
+    counts = np.zeros(6, np.int32)
+    M._lib.call("cmd_comm_ghost_lists", md._sys, None,
+                counts.ctypes.data_as(C.POINTER(C.c_int)))
+    send = np.zeros(max(int(counts.sum()), 1), np.int32)
+    M._lib.call("cmd_comm_ghost_lists", md._sys,
+                send.ctypes.data_as(C.POINTER(C.c_int)),
+                counts.ctypes.data_as(C.POINTER(C.c_int)))
+    O.ghost_fold(fo, n, send[:counts.sum()], counts)
+    d = f[:n] - fo[:n]
+    rms = float(np.sqrt((fo[:n] ** 2).sum(axis=1).mean()))
+    return dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(fo[:n])),
+                max_over_rms=float(np.sqrt((d ** 2).sum(axis=1)).max() / rms))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +202,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--variants", default="mixed,fp64,half")
+    ap.add_argument("--nnp", default="3000,24000,192000",
+                    help="NNP system sizes to time ('' to skip)")
+    ap.add_argument("--nnp-steps", type=int, default=200)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 2 GiB streaming copy (k_copy_probe) "
                          "to calibrate byte counters")
@@ -161,6 +249,15 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+    nnp = {}
+    for sz in [int(v) for v in args.nnp.split(",") if v]:
+        for prec in (M.PREC_MIXED, M.PREC_FP64):
+            r = _nnp_case(M, sz, args.nnp_steps, 5, prec,
+                          accuracy=(not args.no_cpu and sz == 3000))
+            nnp[f"{sz}-{r['layers']}"] = r
+            print(f"# nnp {sz} {r['layers']}: {r['atom_steps_per_s']:.4e} "
+                  f"atom-steps/s, {r['ms_per_step']:.3f} ms/step, drift "
+                  f"{r['energy_drift']:.2e}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(nc, args.cpu_steps)
@@ -196,6 +293,7 @@ def main():
                       f"{h['kernel_mean_ms']:.4f} ms, "
                       f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},
         "cpu_baseline": cpu,
+        "nnp": nnp,
         "copy_probe_gbs": copy_gbs,
         "variants": {k: {kk: vv for kk, vv in r.items() if kk != "timers"}
                      for k, r in res.items()},

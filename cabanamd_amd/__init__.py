@@ -680,7 +680,7 @@ class CbnMD:
         step: leaves the per-atom forces readable by arrays()."""
         call("cmd_force_compute", self.force.handle, self._sys,
              self.neighbor.handle)
-        if self.params.half:
+        if self.params.half or self.params.force_type == 2:
             call("cmd_comm_update_force", self._sys)
 
     def arrays(self):

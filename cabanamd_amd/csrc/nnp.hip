@@ -767,6 +767,7 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     __shared__ NbLds lds[NNP_TPB / 64];
     __shared__ float sF[NNP_TPB / 64][NN_MAX][3];
     __shared__ float sC[NNP_TPB / 64][NSF_LD];
+    __shared__ float4 sCg[NNP_TPB / 64][NA_MAX / 4]; // c of angular (g, p) in grid order
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * ( NNP_TPB / 64 ) + wv;
     if ( i >= n_local )
@@ -821,13 +822,18 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
         fiz += vz;
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-    // angular
-    float cp[AG_MAX][AP_MAX];
-#pragma unroll
-    for ( int g = 0; g < AG_MAX; ++g )
-#pragma unroll
-        for ( int pp = 0; pp < AP_MAX; ++pp )
-            cp[g][pp] = ( g < Q.nag && pp < Q.nap ) ? c[Q.a_out[g][pp]] : 0.f;
+    // angular: coefficients in grid order, read from LDS per pair (keeps
+    // 64 registers free for occupancy)
+    {
+        float *cg = reinterpret_cast<float *>( sCg[wv] );
+        for ( int k = lane; k < NA_MAX; k += 64 )
+        {
+            const int g = k / AP_MAX, pp = k % AP_MAX;
+            cg[k] = ( g < Q.nag && pp < Q.nap ) ? c[Q.a_out[g][pp]] : 0.f;
+        }
+    }
+    __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+    const float4 *cg4 = sCg[wv];
     const int np = n * ( n - 1 ) / 2;
     for ( int p = lane; p < np; p += 64 )
     {
@@ -879,10 +885,16 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
                 const float Eg = __expf( -Q.a_eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
                 float sp = 0.f, sd = 0.f;
 #pragma unroll
-                for ( int pp = 0; pp < AP_MAX; ++pp )
+                for ( int q4 = 0; q4 < AP_MAX / 4; ++q4 )
                 {
-                    sp = fmaf( cp[g][pp], Pw[pp], sp );
-                    sd = fmaf( cp[g][pp], Dw[pp], sd );
+                    const float4 cc = cg4[g * ( AP_MAX / 4 ) + q4];
+                    const float cv[4] = { cc.x, cc.y, cc.z, cc.w };
+#pragma unroll
+                    for ( int u = 0; u < 4; ++u )
+                    {
+                        sp = fmaf( cv[u], Pw[q4 * 4 + u], sp );
+                        sd = fmaf( cv[u], Dw[q4 * 4 + u], sd );
+                    }
                 }
                 const float w = sp * Eg; // sum_p c P E
                 Gs += w;
@@ -942,15 +954,31 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     }
 }
 
+// Atoms grouped by element for the per-element network GEMMs; one atomic
+// per element and wave (the order inside a group does not matter).
 __global__ void k_nnp_group( int n_local, const double4 *__restrict__ x,
                              int cap, int *__restrict__ list, int *__restrict__ count )
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n_local )
-        return;
-    const int e = (int)x[i].w;
-    const int slot = atomicAdd( &count[e], 1 );
-    list[(size_t)e * cap + slot] = i;
+    const int lane = threadIdx.x & 63;
+    const int e = i < n_local ? (int)x[i].w : -1;
+    for ( int el = 0; el < NNP_MAXEL; ++el )
+    {
+        const unsigned long long m = __ballot( e == el );
+        if ( !m )
+            continue;
+        const int leader = __ffsll( (long long)m ) - 1;
+        int base = 0;
+        if ( lane == leader )
+            base = atomicAdd( &count[el], (int)__popcll( m ) );
+        base = __shfl( base, leader, 64 );
+        if ( e == el )
+        {
+            const int r = (int)__builtin_amdgcn_mbcnt_hi(
+                (unsigned)( m >> 32 ), __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) );
+            list[(size_t)el * cap + base + r] = i;
+        }
+    }
 }
 
 __global__ void k_nnp_esum( int n, const float *__restrict__ Ei,
