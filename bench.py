@@ -193,6 +193,80 @@ def _nnp_force_error(M, md, model):
                 max_over_rms=float(np.sqrt((d ** 2).sum(axis=1)).max() / rms))
 
 
+def _batch_specs(nsys, seed):
+    """The batched-ensemble workload: per system k, N_k log-uniform in
+    [500, 20000], fcc nx*ny*nz cells with 4 nx ny nz ~ N_k (near-cubic),
+    density U[0.70, 0.90], temperature U[0.8, 2.0], seed + 1000003 k."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nsys):
+        nk = float(np.exp(rng.uniform(np.log(500.0), np.log(20000.0))))
+        rho = float(rng.uniform(0.70, 0.90))
+        temp = float(rng.uniform(0.8, 2.0))
+        nc = max(1, round(nk / 4.0))
+        nx = max(1, round(nc ** (1.0 / 3.0)))
+        ny = max(1, round((nc / nx) ** 0.5))
+        nz = max(1, round(nc / (nx * ny)))
+        out.append(((nx, ny, nz), rho, temp, seed + 1000003 * k))
+    return out
+
+
+def _batch_case(M, nsys, steps, prec, plain_every, seed=87287):
+    """All systems in one batch (one launch per kernel per step for the
+    whole ensemble) against the plain loop: the single-system engine run
+    on each system in turn (timed on every plain_every-th system)."""
+    import numpy as np
+    specs = _batch_specs(nsys, seed)
+    p = M.lattice_params(thermo=100, every=20)
+    B = M.Batch(p, prec, lanes=0)
+    states, handles = [], []
+    for spec in specs:
+        st = M.fcc_system(*spec)
+        states.append(st)
+        x, v, t, i, lo, hi = st
+        handles.append(B.add(x, t, lo, hi, v=v, id=i))
+    t0 = time.perf_counter()
+    B.run(0)  # join: ghosts, lists, lane autotune, step-0 forces
+    t_join = time.perf_counter() - t0
+    natoms = B.counts()["atoms"]
+    M._lib.call("cmd_device_synchronize")
+    t0 = time.perf_counter()
+    B.run(steps)
+    M._lib.call("cmd_device_synchronize")
+    dt = time.perf_counter() - t0
+    drift = []
+    for h in handles:
+        e = B.thermo(h)[:, 3]
+        drift.append(float(np.abs(e - e[0]).max() / abs(e[0])))
+    plain_as, plain_t, nplain = 0.0, 0.0, 0
+    for k in range(0, nsys, plain_every):
+        x, v, t, i, lo, hi = states[k]
+        pk = M.lattice_params(thermo=100, every=20, nsteps=steps)
+        md = M.CbnMD(pk, precision=prec, lanes=0)
+        md.set_initial_state(x, t, lo, hi, v=v, id=i)
+        md.init()
+        M._lib.call("cmd_device_synchronize")
+        t1 = time.perf_counter()
+        md.run(steps)
+        M._lib.call("cmd_device_synchronize")
+        plain_t += time.perf_counter() - t1
+        plain_as += len(x) * steps
+        nplain += 1
+        del md
+    batched = natoms * steps / dt
+    plain = plain_as / plain_t
+    return dict(systems=nsys, atoms=int(natoms), steps=steps,
+                seconds=dt, ms_per_step=1e3 * dt / steps,
+                batched_atom_steps_per_s=batched,
+                plain_atom_steps_per_s=plain, speedup=batched / plain,
+                plain_sample=f"{nplain} of {nsys} systems (every "
+                             f"{plain_every}th), {int(plain_as)} atom-steps",
+                lanes=B.counts()["lanes"], join_s=t_join,
+                drift_median=float(np.median(drift)),
+                drift_max=float(np.max(drift)), timers=B.timers())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +279,11 @@ def main():
     ap.add_argument("--nnp", default="3000,24000,192000",
                     help="NNP system sizes to time ('' to skip)")
     ap.add_argument("--nnp-steps", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=512,
+                    help="systems in the batched-ensemble leg (0 to skip)")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-plain-every", type=int, default=4,
+                    help="time the plain per-system loop on every k-th system")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 2 GiB streaming copy (k_copy_probe) "
                          "to calibrate byte counters")
@@ -258,6 +337,15 @@ def main():
             print(f"# nnp {sz} {r['layers']}: {r['atom_steps_per_s']:.4e} "
                   f"atom-steps/s, {r['ms_per_step']:.3f} ms/step, drift "
                   f"{r['energy_drift']:.2e}", file=sys.stderr, flush=True)
+    batch = None
+    if args.batch > 0:
+        batch = _batch_case(M, args.batch, args.batch_steps, M.PREC_MIXED,
+                            args.batch_plain_every)
+        print(f"# batch {batch['systems']} systems / {batch['atoms']} atoms: "
+              f"{batch['batched_atom_steps_per_s']:.4e} atom-steps/s batched,"
+              f" {batch['plain_atom_steps_per_s']:.4e} plain, speedup "
+              f"{batch['speedup']:.2f}, drift max {batch['drift_max']:.2e}",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(nc, args.cpu_steps)
@@ -294,6 +382,7 @@ def main():
                       f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},
         "cpu_baseline": cpu,
         "nnp": nnp,
+        "batch": batch,
         "copy_probe_gbs": copy_gbs,
         "variants": {k: {kk: vv for kk, vv in r.items() if kk != "timers"}
                      for k, r in res.items()},

@@ -725,3 +725,103 @@ class CbnMD:
         n, ms = C.c_int(), C.c_double()
         call("cmd_md_kernel_stats", self._h, C.byref(n), C.byref(ms))
         return n.value, ms.value
+
+
+class Batch:
+    """Many independent LJ systems stepped together (no reference
+    counterpart; each system follows CbnMD::run, cabanamd_impl.h:285-399).
+    params: shared run parameters (pair_coeff, cutoff, skin, every, dt,
+    thermo_rate); systems join with add() and leave with remove() at the
+    first run() that starts on a rebuild step."""
+
+    def __init__(self, params, precision=PREC_FP64, lanes=0):
+        self.params = params
+        h = C.c_void_p()
+        call("cmd_batch_create", C.byref(params), C.byref(h))
+        self._h = h
+        call("cmd_batch_set_precision", h, precision)
+        call("cmd_batch_set_lanes", h, lanes)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cmd_batch_destroy(self._h)
+            self._h = None
+
+    def add(self, x, type, lo, hi, v=None, id=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        t = np.ascontiguousarray(type, dtype=np.int32)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        ii = None if id is None else np.ascontiguousarray(id, dtype=np.int32)
+        h = C.c_int()
+        call("cmd_batch_add", self._h, n, dp(x),
+             None if vv is None else dp(vv), ip(t),
+             None if ii is None else ip(ii), d3(lo), d3(hi), C.byref(h))
+        return h.value
+
+    def remove(self, handle):
+        call("cmd_batch_remove", self._h, int(handle))
+
+    def run(self, nsteps):
+        call("cmd_batch_run", self._h, int(nsteps))
+
+    def counts(self):
+        a, c, d = C.c_int(), C.c_int(), C.c_int()
+        b = C.c_int64()
+        call("cmd_batch_counts", self._h, C.byref(a), C.byref(b), C.byref(c),
+             C.byref(d))
+        return dict(systems=a.value, atoms=b.value, step=c.value,
+                    lanes=d.value)
+
+    def info(self, handle):
+        n, s, st = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_batch_system_info", self._h, int(handle), C.byref(n),
+             C.byref(s), C.byref(st))
+        return dict(n=n.value, step=s.value,
+                    state=("unknown", "waiting", "running", "leaving")[st.value])
+
+    def download(self, handle, by_id=True):
+        """x, v, f, type, id of one system (sorted by id when by_id)."""
+        n = self.info(handle)["n"]
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        t = np.zeros(n, dtype=np.int32)
+        i = np.zeros(n, dtype=np.int32)
+        call("cmd_batch_download", self._h, int(handle), dp(x), dp(v), dp(f),
+             ip(t), ip(i))
+        if by_id:
+            o = np.argsort(i, kind="stable")
+            x, v, f, t, i = x[o], v[o], f[o], t[o], i[o]
+        return x, v, f, t, i
+
+    def thermo(self, handle):
+        """Rows (local step, T, PE/N, (PE+KE)/N)."""
+        r = C.c_int()
+        call("cmd_batch_thermo", self._h, int(handle), 0, None, C.byref(r))
+        out = np.zeros((r.value, 4))
+        call("cmd_batch_thermo", self._h, int(handle), r.value, dp(out),
+             C.byref(r))
+        return out
+
+    def timers(self):
+        t = np.zeros(4)
+        call("cmd_batch_timers", self._h, dp(t))
+        return dict(rebuild=t[0], steps=t[1], thermo=t[2], join=t[3])
+
+
+def fcc_system(ncells, rho, temp, seed, mass=1.0, gaussian=True):
+    """An fcc LJ box of 4*nx*ny*nz atoms at reduced density rho with
+    velocities at temperature temp (the lattice and velocity code of
+    InputFile::create_lattice, inputFile_impl.h:536-865).  Returns x, v,
+    type, id, lo, hi."""
+    p = lattice_params(ncells=ncells, rho=rho, temp=temp, seed=seed,
+                       mass=mass, gaussian=gaussian)
+    a = p.lattice_constant
+    hi = [a * c for c in ncells]
+    d = create_domain([0.0, 0.0, 0.0], hi, p.force_cutoff + p.skin)
+    inp = InputFile()
+    inp.params = p
+    x, v, t, i = inp.create_lattice(d)
+    return x, v, t, i, [0.0, 0.0, 0.0], hi
+

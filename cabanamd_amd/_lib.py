@@ -152,6 +152,18 @@ _PROTOS = {
     "cmd_md_timers": ([VP, DP], None),
     "cmd_md_profile": ([VP, C.c_int], None),
     "cmd_md_kernel_stats": ([VP, IP, DP], None),
+    "cmd_batch_create": ([C.POINTER(Params), C.POINTER(VP)], None),
+    "cmd_batch_destroy": ([VP], None),
+    "cmd_batch_set_precision": ([VP, C.c_int], None),
+    "cmd_batch_set_lanes": ([VP, C.c_int], None),
+    "cmd_batch_add": ([VP, C.c_int, DP, DP, IP, IP, DP, DP, IP], None),
+    "cmd_batch_remove": ([VP, C.c_int], None),
+    "cmd_batch_run": ([VP, C.c_int], None),
+    "cmd_batch_counts": ([VP, IP, LP, IP, IP], None),
+    "cmd_batch_system_info": ([VP, C.c_int, IP, IP, IP], None),
+    "cmd_batch_download": ([VP, C.c_int, DP, DP, DP, IP, IP], None),
+    "cmd_batch_thermo": ([VP, C.c_int, C.c_int, DP, IP], None),
+    "cmd_batch_timers": ([VP, DP], None),
 }
 
 _lib = None

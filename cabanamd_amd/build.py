@@ -20,7 +20,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcabanamd_amd.so")
 EXE = os.path.join(HERE, "cabanamd")
 
-HIP_SOURCES = ["core.hip", "lj.hip", "neighbor.hip", "tiles.hip", "md.hip", "nnp.hip"]
+HIP_SOURCES = ["core.hip", "lj.hip", "neighbor.hip", "tiles.hip", "md.hip", "nnp.hip",
+               "batch.hip"]
 CXX_SOURCES = ["host_input.cpp"]
 ARCH = os.environ.get("CMD_OFFLOAD_ARCH", "gfx950")
 

@@ -303,6 +303,48 @@ cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
     return CMD_OK;
 }
 
+__global__ void k_cell_count_ids( int n, const int *__restrict__ cell_of,
+                                  int *__restrict__ count )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n )
+        return;
+    atomicAdd( &count[cell_of[i]], 1 );
+}
+
+cmd_status cell_sort_ids( int n, int ncell, CellSort &cs, Scratch &sc,
+                          hipStream_t st )
+{
+    CMD_REQUIRE( ncell > 0, CMD_ERR_ARG, "cell grid is empty" );
+    CMD_REQUIRE( (int64_t)cs.cell_of.n >= n, CMD_ERR_STATE,
+                 "cell ids were not computed" );
+    cs.ncell = ncell;
+    CMD_TRY( cs.count.reserve( ncell ) );
+    CMD_TRY( cs.cursor.reserve( ncell ) );
+    CMD_TRY( cs.start.reserve( ncell + 1 ) );
+    CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
+    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
+    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    if ( n > 0 )
+    {
+        k_cell_count_ids<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
+                                                           cs.count.p );
+        CMD_LAUNCH_CHECK();
+    }
+    CMD_TRY( scan_exclusive( cs.count.p, cs.start.p, ncell, sc, st, nullptr ) );
+    if ( n > 0 )
+    {
+        k_cell_fill<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
+                                                      cs.start.p, cs.cursor.p,
+                                                      cs.perm.p );
+        CMD_LAUNCH_CHECK();
+        k_cell_order<<<nblocks_for( ncell ), TPB, 0, st>>>( ncell, cs.start.p,
+                                                           cs.perm.p );
+        CMD_LAUNCH_CHECK();
+    }
+    return CMD_OK;
+}
+
 // ---------------------------------------------------------------- NVE
 // integrator_nve.h:91-101 (TagInitial) and :103-110 (TagFinal).
 __global__ void k_nve_initial( int n, int cap, double4 *__restrict__ x,
@@ -650,6 +692,26 @@ __global__ void k_permute( int n, int cap, const int *__restrict__ perm,
     fo[cap + k] = f[cap + s];
     fo[2 * cap + k] = f[2 * cap + s];
     ido[k] = id[s];
+}
+
+cmd_status permute_local( cmd_system *s, const int *perm )
+{
+    const int n = s->n_local;
+    if ( n == 0 )
+        return CMD_OK;
+    CMD_TRY( s->x_alt.reserve( s->cap ) );
+    CMD_TRY( s->id_alt.reserve( s->cap ) );
+    CMD_TRY( s->v_alt.reserve( 3 * (size_t)s->cap ) );
+    CMD_TRY( s->f_alt.reserve( 3 * (size_t)s->cap ) );
+    k_permute<<<nblocks_for( n ), TPB, 0, s->stream>>>(
+        n, s->cap, perm, s->x.p, s->x_alt.p, s->v.p, s->v_alt.p, s->f.p,
+        s->f_alt.p, s->id.p, s->id_alt.p );
+    CMD_LAUNCH_CHECK();
+    s->x.swap( s->x_alt );
+    s->v.swap( s->v_alt );
+    s->f.swap( s->f_alt );
+    s->id.swap( s->id_alt );
+    return CMD_OK;
 }
 
 // ---------------------------------------------------------------- thermo

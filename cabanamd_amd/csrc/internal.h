@@ -184,6 +184,9 @@ struct CellSort
 };
 cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
                       Scratch &sc, hipStream_t st );
+// The same sort for cell ids the caller has already written to cs.cell_of.
+cmd_status cell_sort_ids( int n, int ncell, CellSort &cs, Scratch &sc,
+                          hipStream_t st );
 
 } // namespace cmd
 
@@ -341,6 +344,8 @@ cmd_status nve_final_initial( cmd_system *S, const StepCtl &ctl,
 cmd_status reduce_partials( const double *partial, int nblocks, double *out,
                             hipStream_t st );
 cmd_status ghost_refresh( cmd_system *S );
+// System::permute of the owned atoms: slot k takes atom perm[k] (x, v, f, id).
+cmd_status permute_local( cmd_system *S, const int *perm );
 cmd_status wrap_local( cmd_system *S );
 cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N );
 cmd_status neigh_build( cmd_neigh *N, cmd_system *S );

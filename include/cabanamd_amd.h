@@ -334,6 +334,51 @@ cmd_status cmd_md_profile( cmd_md *md, int enable );
 cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
                                 double *mean_ms );
 
+/* ---- batched ensemble -------------------------------------------------------
+ * No reference counterpart: CabanaMD steps one system per process.  A batch
+ * steps many independent periodic LJ systems of uneven size with the same
+ * launches (owned atoms of all systems in one flat range, ghosts after them),
+ * each following CbnMD::run's step order (cabanamd_impl.h:285-399) with the
+ * batch's shared parameters (cmd_params: pair_coeff, cutoff, skin, every,
+ * dt, units, masses, thermo_rate).  Full lists only.  A system's trajectory
+ * does not depend on the others in the batch.  Systems added or removed
+ * join or leave at the start of the next cmd_batch_run that begins on a
+ * rebuild step (step % every == 0). */
+typedef struct cmd_batch cmd_batch;
+
+cmd_status cmd_batch_create( const cmd_params *p, cmd_batch **out );
+cmd_status cmd_batch_destroy( cmd_batch *b );
+cmd_status cmd_batch_set_precision( cmd_batch *b, int prec );
+/* Lanes per atom of the list layout (1, 2, 4, 8); 0 = time each layout on
+ * the first systems and keep the fastest.  Fixed once systems run. */
+cmd_status cmd_batch_set_lanes( cmd_batch *b, int lanes );
+/* Queue a system: n atoms (x, v as n x 3; v may be NULL), types 0-based,
+ * ids (NULL = 1..n) and its box [lo, hi), each side longer than twice the
+ * ghost depth (cutoff + skin).  *handle names it in later calls. */
+cmd_status cmd_batch_add( cmd_batch *b, int n, const double *x, const double *v,
+                          const int *type, const int *id, const double lo[3],
+                          const double hi[3], int *handle );
+cmd_status cmd_batch_remove( cmd_batch *b, int handle );
+/* Advance every running system by nsteps; thermo rows every thermo_rate
+ * steps of the batch and at the last step. */
+cmd_status cmd_batch_run( cmd_batch *b, int nsteps );
+cmd_status cmd_batch_counts( const cmd_batch *b, int *nsys, int64_t *natoms,
+                             int *step, int *lanes );
+/* state: 0 unknown, 1 waiting to join, 2 running, 3 leaving; local_step is
+ * the number of steps the system has run. */
+cmd_status cmd_batch_system_info( const cmd_batch *b, int handle, int *n,
+                                  int *local_step, int *state );
+/* The system's atoms in their current (cell-sorted) order; any pointer may
+ * be NULL.  f holds the forces of the last step. */
+cmd_status cmd_batch_download( cmd_batch *b, int handle, double *x, double *v,
+                               double *f, int *type, int *id );
+/* Rows (local step, T, PE/N, (PE+KE)/N); a row at join (step 0), then as
+ * cmd_batch_run takes them. */
+cmd_status cmd_batch_thermo( const cmd_batch *b, int handle, int max_rows,
+                             double *rows, int *rows_out );
+/* Host seconds in rebuilds, step loops, thermo and joins/leaves. */
+cmd_status cmd_batch_timers( const cmd_batch *b, double out[4] );
+
 #ifdef __cplusplus
 }
 #endif
