@@ -734,13 +734,16 @@ class Batch:
     thermo_rate); systems join with add() and leave with remove() at the
     first run() that starts on a rebuild step."""
 
-    def __init__(self, params, precision=PREC_FP64, lanes=0):
+    def __init__(self, params, precision=PREC_FP64, lanes=0, nnp_model=None):
         self.params = params
         h = C.c_void_p()
         call("cmd_batch_create", C.byref(params), C.byref(h))
         self._h = h
         call("cmd_batch_set_precision", h, precision)
         call("cmd_batch_set_lanes", h, lanes)
+        self._model = nnp_model
+        if nnp_model is not None:
+            call("cmd_batch_set_nnp_model", h, nnp_model.handle)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -824,4 +827,102 @@ def fcc_system(ncells, rho, temp, seed, mass=1.0, gaussian=True):
     inp.params = p
     x, v, t, i = inp.create_lattice(d)
     return x, v, t, i, [0.0, 0.0, 0.0], hi
+
+
+def nnp_system(cells, spacing=2.5, jitter=0.1, temp=0.05, seed=87287,
+               mass=1.0):
+    """A two-element NNP box: jittered simple-cubic lattice (one third
+    element 1) with Gaussian velocities at temperature temp and zero net
+    momentum.  Returns x, v, type, id, lo, hi."""
+    x, t, i = nnp_lattice(cells, spacing=spacing, jitter=jitter, seed=seed)
+    rng = np.random.default_rng(seed + 17)
+    v = rng.normal(0.0, np.sqrt(temp / mass), x.shape)
+    v -= v.mean(axis=0)
+    hi = [spacing * c for c in cells]
+    x = np.mod(x, np.array(hi))
+    return x, v, t, i, [0.0, 0.0, 0.0], hi
+
+
+class Server:
+    """MD request server with continuous batching (no reference
+    counterpart): LJ jobs and, given nnp params and a model, NNP jobs, each
+    kind in one device-resident batch; jobs are admitted in arrival order by
+    device footprint and slot, join at rebuild boundaries and leave with
+    their results when they finish."""
+
+    LJ, NNP = 0, 2
+    QUEUED, RUNNING, DONE = 1, 2, 3
+
+    def __init__(self, lj_params, nnp_params=None, nnp_model=None,
+                 hbm_budget=64 << 30, max_jobs=64, precision=PREC_MIXED,
+                 lanes=0):
+        h = C.c_void_p()
+        call("cmd_server_create", C.byref(lj_params),
+             None if nnp_params is None else C.byref(nnp_params),
+             None if nnp_model is None else nnp_model.handle,
+             C.c_int64(int(hbm_budget)), int(max_jobs), C.byref(h))
+        self._h = h
+        self._model = nnp_model
+        self._n = {}
+        call("cmd_server_configure", h, precision, lanes)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cmd_server_destroy(self._h)
+            self._h = None
+
+    def clock(self):
+        t = C.c_double()
+        call("cmd_server_clock", self._h, C.byref(t))
+        return t.value
+
+    def submit(self, kind, x, type, lo, hi, nsteps, v=None, id=None,
+               arrival=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        t = np.ascontiguousarray(type, dtype=np.int32)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        ii = None if id is None else np.ascontiguousarray(id, dtype=np.int32)
+        j = C.c_int()
+        call("cmd_server_submit", self._h, int(kind), n, dp(x),
+             None if vv is None else dp(vv), ip(t),
+             None if ii is None else ip(ii), d3(lo), d3(hi), int(nsteps),
+             float(arrival), C.byref(j))
+        self._n[j.value] = n
+        return j.value
+
+    def poll(self):
+        q, r, d = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_server_poll", self._h, C.byref(q), C.byref(r), C.byref(d))
+        return q.value, r.value, d.value
+
+    def drain(self, timeout=600.0):
+        call("cmd_server_drain", self._h, float(timeout))
+
+    def job(self, j):
+        st, n = C.c_int(), C.c_int()
+        t = np.zeros(3)
+        call("cmd_server_job", self._h, int(j), C.byref(st), dp(t), C.byref(n))
+        return dict(state=st.value, arrival=t[0], admit=t[1], done=t[2],
+                    n=n.value)
+
+    def result(self, j):
+        """x, v, id (ordered by id) and thermo rows of a finished job."""
+        n = self._n[j]
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        i = np.zeros(n, dtype=np.int32)
+        r = C.c_int()
+        call("cmd_server_result", self._h, int(j), dp(x), dp(v), ip(i), 0,
+             None, C.byref(r))
+        th = np.zeros((r.value, 4))
+        call("cmd_server_result", self._h, int(j), None, None, None, r.value,
+             dp(th), C.byref(r))
+        return x, v, i, th
+
+    def stats(self):
+        o = np.zeros(6)
+        call("cmd_server_stats", self._h, dp(o))
+        return dict(atom_steps=o[0], busy_s=o[1], rounds=int(o[2]),
+                    bytes=int(o[3]), running=int(o[4]), done=int(o[5]))
 

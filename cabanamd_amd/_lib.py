@@ -164,6 +164,19 @@ _PROTOS = {
     "cmd_batch_download": ([VP, C.c_int, DP, DP, DP, IP, IP], None),
     "cmd_batch_thermo": ([VP, C.c_int, C.c_int, DP, IP], None),
     "cmd_batch_timers": ([VP, DP], None),
+    "cmd_batch_set_nnp_model": ([VP, VP], None),
+    "cmd_server_create": ([C.POINTER(Params), C.POINTER(Params), VP,
+                           C.c_int64, C.c_int, C.POINTER(VP)], None),
+    "cmd_server_destroy": ([VP], None),
+    "cmd_server_configure": ([VP, C.c_int, C.c_int], None),
+    "cmd_server_clock": ([VP, DP], None),
+    "cmd_server_submit": ([VP, C.c_int, C.c_int, DP, DP, IP, IP, DP, DP,
+                           C.c_int, C.c_double, IP], None),
+    "cmd_server_poll": ([VP, IP, IP, IP], None),
+    "cmd_server_drain": ([VP, C.c_double], None),
+    "cmd_server_job": ([VP, C.c_int, IP, DP, IP], None),
+    "cmd_server_result": ([VP, C.c_int, DP, DP, IP, C.c_int, DP, IP], None),
+    "cmd_server_stats": ([VP, DP], None),
 }
 
 _lib = None

@@ -22,7 +22,7 @@ EXE = os.path.join(HERE, "cabanamd")
 
 HIP_SOURCES = ["core.hip", "lj.hip", "neighbor.hip", "tiles.hip", "md.hip", "nnp.hip",
                "batch.hip"]
-CXX_SOURCES = ["host_input.cpp"]
+CXX_SOURCES = ["host_input.cpp", "serve.cpp"]
 ARCH = os.environ.get("CMD_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -304,6 +304,17 @@ __global__ void k_b_atom_thermo( int n_local, int scap, int cap, const double4 *
     mv2[i] = ( vx * vx + vy * vy + vz * vz ) * m;
 }
 
+__global__ void k_b_mv2( int n_local, int scap, const double4 *__restrict__ x,
+                         const double *__restrict__ v, MassTable mt, double *__restrict__ mv2 )
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n_local )
+        return;
+    const double m = mt.m[(int)x[i].w];
+    const double vx = v[i], vy = v[scap + i], vz = v[2 * scap + i];
+    mv2[i] = ( vx * vx + vy * vy + vz * vz ) * m;
+}
+
 // One block per system: sums of pe and m v^2 over its atoms, in an order
 // fixed by the system alone.
 __global__ void k_b_sys_sum( const BSys *__restrict__ T, const double *__restrict__ pe,
@@ -373,6 +384,8 @@ struct cmd_batch
     cmd_neigh *N = nullptr;
     cmd_force *F = nullptr;
     hipStream_t stream = nullptr;
+    bool nnp = false; // NNP systems (force_type 2): unfused step order
+    int prec = CMD_PREC_FP64;
     int lanes = 0; // 0: autotune at the first build
     int step = 0;
     int next_handle = 1;
@@ -592,7 +605,16 @@ cmd_status thermo( cmd_batch *B, const std::vector<char> *which )
     CMD_TRY( B->pe.reserve( std::max( n, 1 ) ) );
     CMD_TRY( B->mv2.reserve( std::max( n, 1 ) ) );
     CMD_TRY( B->sums.reserve( 2 * (size_t)ns ) );
-    if ( n > 0 )
+    if ( n > 0 && B->nnp )
+    {
+        // ForceNNP::compute_energy (force_nnp_cabana_neigh_impl.h:101-130)
+        // per system: (sum_i E_i + n * offset) * 27.211384021355236
+        CMD_TRY( nnp_atom_energy( B->F->nnp, n, B->pe.p, B->stream ) );
+        k_b_mv2<<<nblk( n ), TPB, 0, B->stream>>>( n, S->cap, S->x.p, S->v.p, S->mass_table(),
+                                                   B->mv2.p );
+        CMD_LAUNCH_CHECK();
+    }
+    else if ( n > 0 )
     {
 #define CMD_BT( GG )                                                                        \
     k_b_atom_thermo<GG><<<nblk( n ), TPB, 0, B->stream>>>( n, S->cap, N->cap, S->x.p, S->v.p, \
@@ -619,7 +641,10 @@ cmd_status thermo( cmd_batch *B, const std::vector<char> *which )
         if ( which && !( *which )[k] )
             continue;
         BMember &m = B->members[k];
-        const double pe = B->hsums[2 * k], mv2 = B->hsums[2 * k + 1];
+        double pe = B->hsums[2 * k];
+        const double mv2 = B->hsums[2 * k + 1];
+        if ( B->nnp )
+            pe = ( pe + m.n * nnp_energy_offset( B->F->nnp ) ) * 27.211384021355236;
         const double ke = 0.5 * S->mvv2e * mv2;
         // property_temperature_impl.h:55-77 (dof 3N - 3), cabanamd_impl.h:238-242
         const double T = mv2 * S->mvv2e / ( ( 3.0 * m.n - 3.0 ) * S->boltz );
@@ -632,6 +657,21 @@ cmd_status thermo( cmd_batch *B, const std::vector<char> *which )
     return CMD_OK;
 }
 
+
+// Forces of the current positions with the current list (CbnMD::init's
+// first compute; for NNP also the ghost-force fold, cabanamd_impl.h:218-223).
+cmd_status forces( cmd_batch *B )
+{
+    if ( B->S->n_local == 0 )
+        return CMD_OK;
+    if ( B->nnp )
+    {
+        CMD_TRY( cmd_force_compute( B->F, B->S, B->N ) );
+        return cmd_comm_update_force( B->S );
+    }
+    StepCtl ctl{};
+    return lj_full( B->F, B->S, B->N, ctl, nullptr, nullptr, nullptr );
+}
 
 // Times the force kernel on each list layout and keeps the fastest.
 cmd_status autotune( cmd_batch *B )
@@ -778,14 +818,14 @@ cmd_status apply_changes( cmd_batch *B )
     }
     // the new systems start from wrapped positions (CbnMD::init's exchange)
     CMD_TRY( wrap( B, S->x.p ) );
+    if ( B->nnp && B->lanes == 0 )
+        B->lanes = 1; // the NNP kernels read the list as CSR
     const bool tune = B->lanes == 0 && S->n_local > 0;
     CMD_TRY( rebuild( B, tune ? 1 : B->lanes ) );
     if ( tune )
         CMD_TRY( autotune( B ) );
     // step forces of every system (CbnMD::init's first compute for new ones)
-    StepCtl ctl{};
-    if ( S->n_local > 0 )
-        CMD_TRY( lj_full( B->F, S, B->N, ctl, nullptr, nullptr, nullptr ) );
+    CMD_TRY( forces( B ) );
     std::vector<char> fresh( B->members.size(), 0 );
     for ( size_t k = B->members.size() - nnew; k < B->members.size(); ++k )
         fresh[k] = 1;
@@ -800,8 +840,8 @@ extern "C" {
 cmd_status cmd_batch_create( const cmd_params *p, cmd_batch **out )
 {
     CMD_REQUIRE( p && out, CMD_ERR_ARG, "params and out are required" );
-    CMD_REQUIRE( p->force_type == 0, CMD_ERR_ARG,
-                 "a batch runs lj/cut systems (force_type 0)" );
+    CMD_REQUIRE( p->force_type == 0 || p->force_type == 2, CMD_ERR_ARG,
+                 "a batch runs lj/cut (force_type 0) or nnp (2) systems" );
     CMD_REQUIRE( !p->half, CMD_ERR_ARG, "a batch uses full neighbour lists" );
     CMD_REQUIRE( p->every > 0, CMD_ERR_ARG, "neigh_modify every must be positive" );
     cmd_batch *B = new cmd_batch();
@@ -813,15 +853,19 @@ cmd_status cmd_batch_create( const cmd_params *p, cmd_batch **out )
         s = cmd_system_set_mass( B->S, p->ntypes, p->mass );
     if ( s == CMD_OK )
         s = cmd_neigh_create( p->force_cutoff + p->skin, 0, p->max_neigh_guess, &B->N );
-    if ( s == CMD_OK )
+    B->nnp = p->force_type == 2;
+    if ( s == CMD_OK && !B->nnp )
         s = cmd_force_lj_create( p->ntypes, &B->F );
-    for ( int c = 0; s == CMD_OK && c < p->ncoeff; ++c )
+    for ( int c = 0; s == CMD_OK && !B->nnp && c < p->ncoeff; ++c )
         s = cmd_force_lj_init_coeff( B->F, p->coeff_i[c], p->coeff_j[c], p->coeff_eps[c],
                                      p->coeff_sigma[c], p->coeff_cut[c] );
     if ( s == CMD_OK )
     {
         B->stream = B->S->stream;
         B->N->allow_tiles = false;
+    }
+    if ( s == CMD_OK && !B->nnp )
+    {
         const LJParams P = lj_params( B->F );
         s = B->Pd.reserve( 1 );
         if ( s == CMD_OK && hipMemcpy( B->Pd.p, &P, sizeof( P ), hipMemcpyHostToDevice ) !=
@@ -861,7 +905,25 @@ cmd_status cmd_batch_destroy( cmd_batch *b )
 cmd_status cmd_batch_set_precision( cmd_batch *b, int prec )
 {
     CMD_REQUIRE( b, CMD_ERR_ARG, "batch is NULL" );
-    return cmd_force_set_precision( b->F, prec );
+    CMD_REQUIRE( prec == CMD_PREC_FP64 || prec == CMD_PREC_MIXED, CMD_ERR_ARG,
+                 "unknown precision %d", prec );
+    b->prec = prec;
+    return b->F ? cmd_force_set_precision( b->F, prec ) : CMD_OK;
+}
+
+cmd_status cmd_batch_set_nnp_model( cmd_batch *b, const cmd_nnp_model *m )
+{
+    CMD_REQUIRE( b && m, CMD_ERR_ARG, "batch and model are required" );
+    CMD_REQUIRE( b->nnp, CMD_ERR_STATE, "the batch runs lj/cut systems" );
+    CMD_REQUIRE( b->members.empty() && b->pending.empty(), CMD_ERR_STATE,
+                 "the model is set before systems are added" );
+    if ( b->F )
+    {
+        cmd_force_destroy( b->F );
+        b->F = nullptr;
+    }
+    CMD_TRY( cmd_force_nnp_create( m, &b->F ) );
+    return cmd_force_set_precision( b->F, b->prec );
 }
 
 cmd_status cmd_batch_set_lanes( cmd_batch *b, int lanes )
@@ -881,6 +943,7 @@ cmd_status cmd_batch_add( cmd_batch *b, int n, const double *x, const double *v,
 {
     CMD_REQUIRE( b && x && type && lo && hi && n > 0, CMD_ERR_ARG,
                  "batch, n > 0, x, type, lo and hi are required" );
+    CMD_REQUIRE( b->F, CMD_ERR_STATE, "an NNP batch needs its model first" );
     const double depth = b->N->neigh_cut;
     for ( int d = 0; d < 3; ++d )
         CMD_REQUIRE( hi[d] - lo[d] > 2.0 * depth, CMD_ERR_ARG,
@@ -951,6 +1014,29 @@ cmd_status cmd_batch_run( cmd_batch *b, int nsteps )
     const hipStream_t st = b->stream;
     const int G = N->lanes;
     const int s0 = b->step + 1, s1 = b->step + nsteps;
+    if ( b->nnp )
+    {
+        // CbnMD::run's order (cabanamd_impl.h:285-399) step by step
+        for ( int s = s0; s <= s1; ++s )
+        {
+            b->step = s;
+            CMD_TRY( cmd_nve_initial( S ) );
+            if ( s % p.every == 0 )
+            {
+                CMD_TRY( wrap( b, S->x.p ) );
+                CMD_TRY( rebuild( b, G ) );
+            }
+            else
+                CMD_TRY( refresh( b ) );
+            CMD_TRY( forces( b ) );
+            CMD_TRY( cmd_nve_final( S ) );
+            if ( s == s1 || ( p.thermo_rate > 0 && s % p.thermo_rate == 0 ) )
+                CMD_TRY( thermo( b, nullptr ) );
+        }
+        CMD_HIP( hipStreamSynchronize( st ) );
+        b->timers[1] += now_s() - t0;
+        return CMD_OK;
+    }
     StepCtl ctl{};
     ctl.dtf = 0.5 * S->dt / S->mvv2e;
     ctl.dtv = S->dt;

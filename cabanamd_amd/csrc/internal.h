@@ -329,6 +329,10 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
 cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N );
 cmd_status nnp_reference_energy( cmd_nnp *M, cmd_system *S, double *pe );
 cmd_status nnp_energy_sum( cmd_nnp *M, cmd_system *S, double *sum );
+// Atomic energies E_i of the last compute as FP64 into out[0..n) (device).
+cmd_status nnp_atom_energy( cmd_nnp *M, int n, double *out, hipStream_t st );
+// The per-atom offset ForceNNP::compute_energy adds (element 0's atom energy).
+double nnp_energy_offset( const cmd_nnp *M );
 cmd_status nnp_destroy( cmd_nnp *M );
 cmd_status nnp_set_precision( cmd_nnp *M, int prec );
 cmd_status nnp_model_copy( const cmd_nnp_model *m, cmd_nnp_model **out );

@@ -1760,6 +1760,25 @@ cmd_status nnp_set_precision( cmd_nnp *M, int prec )
 // ForceNNP::compute_energy (impl.h:101-130): sum of atomic energies plus
 // N_local times element 0's offset, over convEnergy (1: no normalisation),
 // plus N times meanEnergy (0), times the hard-coded 27.211384021355236.
+__global__ void k_nnp_ei_f64( int n, const float *__restrict__ Ei, double *__restrict__ out )
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i < n )
+        out[i] = (double)Ei[i];
+}
+
+cmd_status nnp_atom_energy( cmd_nnp *M, int n, double *out, hipStream_t st )
+{
+    if ( n <= 0 )
+        return CMD_OK;
+    CMD_REQUIRE( M->cap >= n, CMD_ERR_STATE, "no atomic energies for %d atoms", n );
+    k_nnp_ei_f64<<<( n + 255 ) / 256, 256, 0, st>>>( n, M->Ei.p, out );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+double nnp_energy_offset( const cmd_nnp *M ) { return M->model.el[0].atom_energy; }
+
 cmd_status nnp_reference_energy( cmd_nnp *M, cmd_system *S, double *pe )
 {
     double sum = 0.0;

@@ -349,6 +349,11 @@ typedef struct cmd_batch cmd_batch;
 cmd_status cmd_batch_create( const cmd_params *p, cmd_batch **out );
 cmd_status cmd_batch_destroy( cmd_batch *b );
 cmd_status cmd_batch_set_precision( cmd_batch *b, int prec );
+/* A batch created with force_type 2 runs NNP systems (ForceNNP, unfused
+ * step order) with this model; set before adding systems.  Thermo PE is
+ * ForceNNP::compute_energy's per system (force_nnp_cabana_neigh_impl.h:
+ * 101-130). */
+cmd_status cmd_batch_set_nnp_model( cmd_batch *b, const cmd_nnp_model *m );
 /* Lanes per atom of the list layout (1, 2, 4, 8); 0 = time each layout on
  * the first systems and keep the fastest.  Fixed once systems run. */
 cmd_status cmd_batch_set_lanes( cmd_batch *b, int lanes );
@@ -378,6 +383,49 @@ cmd_status cmd_batch_thermo( const cmd_batch *b, int handle, int max_rows,
                              double *rows, int *rows_out );
 /* Host seconds in rebuilds, step loops, thermo and joins/leaves. */
 cmd_status cmd_batch_timers( const cmd_batch *b, double out[4] );
+
+/* ---- MD request server ------------------------------------------------------
+ * No reference counterpart.  Serves independent MD jobs (LJ, and NNP when
+ * nnp parameters and a model are given) with continuous batching: two
+ * device-resident batches (one per force) step together on their own
+ * streams from their own host threads; arriving jobs are admitted in
+ * arrival order while their estimated device footprint fits hbm_budget and
+ * fewer than max_jobs run, join at the next rebuild boundary, and leave
+ * with their results copied out the step they finish.  A job's result does
+ * not depend on the other jobs.  Both parameter sets must share `every`. */
+typedef struct cmd_server cmd_server;
+
+cmd_status cmd_server_create( const cmd_params *lj, const cmd_params *nnp,
+                              const cmd_nnp_model *model, int64_t hbm_budget,
+                              int max_jobs, cmd_server **out );
+cmd_status cmd_server_destroy( cmd_server *s );
+/* Precision of both batches and lanes of the LJ list (0 = autotune). */
+cmd_status cmd_server_configure( cmd_server *s, int prec, int lanes );
+/* Seconds since the server was created (the clock of arrival times). */
+cmd_status cmd_server_clock( const cmd_server *s, double *t );
+/* A job: kind 0 (LJ) or 2 (NNP), n atoms (x, v as n x 3; v may be NULL),
+ * types, ids (NULL = 1..n), box [lo, hi), nsteps, and the arrival time on
+ * the server clock (non-decreasing over submissions).  *job indexes it. */
+cmd_status cmd_server_submit( cmd_server *s, int kind, int n, const double *x,
+                              const double *v, const int *type, const int *id,
+                              const double lo[3], const double hi[3],
+                              int nsteps, double arrival, int *job );
+/* One round: admit arrived jobs, advance one chunk, retire finished jobs. */
+cmd_status cmd_server_poll( cmd_server *s, int *queued, int *running,
+                            int *done );
+/* Poll until every submitted job is done (waiting for arrivals). */
+cmd_status cmd_server_drain( cmd_server *s, double timeout_s );
+/* state 1 queued, 2 running, 3 done; times = arrival, admission, finish. */
+cmd_status cmd_server_job( const cmd_server *s, int job, int *state,
+                           double times[3], int *n );
+/* Final positions and velocities ordered by atom id, the ids, and thermo
+ * rows (local step, T, PE/N, (PE+KE)/N). */
+cmd_status cmd_server_result( const cmd_server *s, int job, double *x,
+                              double *v, int *id, int max_rows, double *thermo,
+                              int *rows );
+/* out: finished atom-steps, seconds stepping, rounds, bytes charged, running
+ * jobs, finished jobs. */
+cmd_status cmd_server_stats( const cmd_server *s, double out[6] );
 
 #ifdef __cplusplus
 }
