@@ -267,6 +267,138 @@ def _batch_case(M, nsys, steps, prec, plain_every, seed=87287):
                 drift_max=float(np.max(drift)), timers=B.timers())
 
 
+_MODEL = {}
+
+
+def _nnp_bench_model(M):
+    """The NNP benchmark model (seeded synthetic, 70 functions per element)
+    with min-max scaling fitted to a 16x16x16 jittered lattice frame."""
+    if "m" not in _MODEL:
+        x, t, ids = M.nnp_lattice((16, 16, 16), spacing=2.5, jitter=0.1,
+                                  seed=87287)
+        L = [40.0] * 3
+        s = M.System(2)
+        s.create_domain([0, 0, 0], L, 6.3)
+        import numpy as np
+        s.upload(np.mod(x, 40.0), None, None, t, ids)
+        comm = M.Comm(s, 6.3)
+        comm.exchange_halo()
+        nl = M.NeighborVerlet(6.3, False, 100, lanes=1)
+        nl.create(s)
+        F = M.ForceNNP(M.NNPModel.synthetic(seed=87287, rc=6.0), M.PREC_FP64)
+        F.fit_scaling(s, nl)
+        _MODEL["m"] = F.model()
+        _MODEL["f"] = F
+    return _MODEL["m"]
+
+
+def _near_cube(n):
+    nx = max(1, round(n ** (1.0 / 3.0)))
+    ny = max(1, round((n / nx) ** 0.5))
+    nz = max(1, round(n / (nx * ny)))
+    return (nx, ny, nz)
+
+
+def _serve_jobs(M, njobs, seed):
+    """Served request stream: 70 % LJ / 30 % NNP, N log-uniform in
+    [1000, 200000], steps uniform in [100, 1000]; LJ jobs are fcc boxes
+    (density U[0.70, 0.90], T U[0.8, 2.0]), NNP jobs jittered lattices at
+    spacing 2.5 and T 0.05; seeds seed + 1000003 k."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(njobs):
+        nnp = rng.uniform() < 0.3
+        n = float(np.exp(rng.uniform(np.log(1000.0), np.log(200000.0))))
+        steps = int(rng.integers(100, 1001))
+        sk = seed + 1000003 * k
+        if nnp:
+            st = M.nnp_system(_near_cube(n), temp=0.05, seed=sk)
+            spec = None
+        else:
+            rho = float(rng.uniform(0.70, 0.90))
+            temp = float(rng.uniform(0.8, 2.0))
+            spec = (_near_cube(n / 4.0), rho, temp, sk)
+            st = M.fcc_system(*spec)
+        jobs.append((2 if nnp else 0, steps, st, spec))
+    return jobs
+
+
+def _oracle_lj_drift(spec, steps):
+    """Energy drift of the FP64 oracle's MD loop on an LJ job's state."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    cells, rho, temp, seed = spec
+    md = O.MD(O.lj_params(ncells=cells, rho=rho, temp=temp, seed=seed,
+                          gaussian=True, thermo=20, every=20))
+    md.init()
+    md.run(steps)
+    e = md.thermo()[:, 3]
+    return float(np.abs(e - e[0]).max() / np.abs(e).max())
+
+
+def _serve_run(M, model, jobs, arrivals, prec, oracle_jobs=0):
+    import numpy as np
+    S = M.Server(M.lattice_params(thermo=100, every=20),
+                 M.nnp_params(thermo=100), model, precision=prec, lanes=0)
+    t0 = S.clock()
+    ids = []
+    for (kind, steps, st, _), a in zip(jobs, arrivals):
+        x, v, t, i, lo, hi = st
+        ids.append(S.submit(kind, x, t, lo, hi, steps, v=v, id=i,
+                            arrival=t0 + a))
+    S.drain(900)
+    lat, drift = [], {0: [], 2: []}
+    vs_oracle = []
+    t_end = 0.0
+    for j, (kind, steps, st, spec) in zip(ids, jobs):
+        info = S.job(j)
+        lat.append(info["done"] - info["arrival"])
+        t_end = max(t_end, info["done"])
+        th = S.result(j)[3]
+        n = info["n"]
+        if kind == 2:
+            e = th[:, 2] * n / 27.211384021355236 + (th[:, 3] - th[:, 2]) * n
+        else:
+            e = th[:, 3] * n
+        drift[kind].append(float(np.abs(e - e[0]).max() / np.abs(e).max()))
+        if kind == 0 and len(vs_oracle) < oracle_jobs and n <= 20000:
+            vs_oracle.append(dict(n=n, steps=steps, gpu=drift[0][-1],
+                                  oracle_fp64=_oracle_lj_drift(spec, steps)))
+    atom_steps = sum(len(st[0]) * steps for _, steps, st, _ in jobs)
+    span = t_end - (t0 + arrivals[0])
+    return dict(jobs=len(jobs), atom_steps=atom_steps, makespan_s=span,
+                atom_steps_per_s=atom_steps / span,
+                jobs_per_s=len(jobs) / span,
+                p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)),
+                drift_lj_median=float(np.median(drift[0])) if drift[0] else None,
+                drift_lj_max=float(np.max(drift[0])) if drift[0] else None,
+                drift_nnp_median=float(np.median(drift[2])) if drift[2] else None,
+                drift_nnp_max=float(np.max(drift[2])) if drift[2] else None,
+                lj_drift_vs_fp64_oracle=vs_oracle, stats=S.stats())
+
+
+def _serve_case(M, ncap, njobs, prec, seed=87287):
+    """Capacity: ncap jobs all arriving at once (jobs/s at saturation);
+    then njobs Poisson arrivals at 0.7x that rate, reporting p50/p99 latency
+    (arrival to result), atom-steps/s and per-job energy drift."""
+    import numpy as np
+    model = _nnp_bench_model(M)
+    cap_jobs = _serve_jobs(M, ncap, seed)
+    cap = _serve_run(M, model, cap_jobs, [0.0] * ncap, prec)
+    rate = 0.7 * cap["jobs_per_s"]
+    jobs = _serve_jobs(M, njobs, seed + 7)
+    rng = np.random.default_rng(seed + 11)
+    arr = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    arr -= arr[0]
+    served = _serve_run(M, model, jobs, list(arr), prec, oracle_jobs=4)
+    served["arrival_rate_jobs_per_s"] = rate
+    served["capacity"] = cap
+    return served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +416,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-plain-every", type=int, default=4,
                     help="time the plain per-system loop on every k-th system")
+    ap.add_argument("--serve", type=int, default=40,
+                    help="jobs in the served-stream leg (0 to skip)")
+    ap.add_argument("--serve-capacity-jobs", type=int, default=24)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 2 GiB streaming copy (k_copy_probe) "
                          "to calibrate byte counters")
@@ -346,6 +481,15 @@ def main():
               f" {batch['plain_atom_steps_per_s']:.4e} plain, speedup "
               f"{batch['speedup']:.2f}, drift max {batch['drift_max']:.2e}",
               file=sys.stderr, flush=True)
+    serve = None
+    if args.serve > 0:
+        serve = _serve_case(M, args.serve_capacity_jobs, args.serve,
+                            M.PREC_MIXED)
+        print(f"# serve {serve['jobs']} jobs at {serve['arrival_rate_jobs_per_s']:.2f}"
+              f" jobs/s: p50 {serve['p50_latency_s']:.3f} s, p99 "
+              f"{serve['p99_latency_s']:.3f} s, {serve['atom_steps_per_s']:.4e} "
+              f"atom-steps/s (capacity {serve['capacity']['jobs_per_s']:.2f} "
+              f"jobs/s)", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(nc, args.cpu_steps)
@@ -383,6 +527,7 @@ def main():
         "cpu_baseline": cpu,
         "nnp": nnp,
         "batch": batch,
+        "serve": serve,
         "copy_probe_gbs": copy_gbs,
         "variants": {k: {kk: vv for kk, vv in r.items() if kk != "timers"}
                      for k, r in res.items()},

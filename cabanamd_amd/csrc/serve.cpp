@@ -18,6 +18,7 @@
 #include "../../include/cabanamd_amd.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstring>
@@ -78,7 +79,7 @@ struct cmd_server
     int max_jobs = 0, running = 0;
     double t0 = 0;
     std::vector<Job> jobs;
-    size_t next_admit = 0; // jobs are admitted in submission order
+    size_t next_admit[2] = { 0, 0 }; // per kind (lj, nnp), in submission order
     double atom_steps = 0, busy_s = 0;
     int64_t ticks = 0;
 };
@@ -87,11 +88,16 @@ namespace
 {
 cmd_batch *batch_of( cmd_server *s, const Job &j ) { return j.kind == 2 ? s->nnp : s->lj; }
 
-cmd_status admit( cmd_server *s, double now )
+// Admits arrived jobs of one kind in arrival order while memory and slots
+// allow (the head of a kind's queue blocks the rest of that kind only).
+cmd_status admit( cmd_server *s, int kind, double now )
 {
-    while ( s->next_admit < s->jobs.size() )
+    size_t &next = s->next_admit[kind == 2];
+    for ( ; next < s->jobs.size(); ++next )
     {
-        Job &j = s->jobs[s->next_admit];
+        Job &j = s->jobs[next];
+        if ( j.kind != kind )
+            continue;
         if ( j.t_arrival > now || s->running >= s->max_jobs || s->used + j.bytes > s->budget )
             break;
         cmd_batch *b = batch_of( s, j );
@@ -105,7 +111,6 @@ cmd_status admit( cmd_server *s, double now )
         j.t_admit = now;
         s->used += j.bytes;
         ++s->running;
-        ++s->next_admit;
         // the batch holds its own copy now
         std::vector<double>().swap( j.x );
         std::vector<double>().swap( j.v );
@@ -150,50 +155,105 @@ cmd_status retire( cmd_server *s, Job &j )
         return st;
     j.state = DONE;
     j.t_done = clock_s() - s->t0;
+    s->atom_steps += (double)j.n * j.nsteps;
     s->used -= j.bytes;
     --s->running;
     return CMD_OK;
 }
 
-// Steps the next chunk; *ran = steps advanced (0 when nothing can run).
-cmd_status advance( cmd_server *s, int *ran )
+// Length of the next chunk of one kind's batch: up to its next rebuild
+// boundary, no further than the earliest end of its running jobs; 0 when
+// it has nothing to step.  Jobs waiting to join only move the batch to its
+// boundary.
+cmd_status chunk( cmd_server *s, int kind, int *k_out )
 {
-    *ran = 0;
+    *k_out = 0;
+    cmd_batch *b = kind == 2 ? s->nnp : s->lj;
+    if ( !b )
+        return CMD_OK;
     int step = 0;
-    cmd_status st = cmd_batch_counts( s->lj, nullptr, nullptr, &step, nullptr );
+    cmd_status st = cmd_batch_counts( b, nullptr, nullptr, &step, nullptr );
     if ( st != CMD_OK )
         return st;
     const bool boundary = step % s->every == 0;
-    int k = s->every - step % s->every; // steps to the next boundary
-    bool stepping = false, waiting = false;
+    int k = s->every - step % s->every;
+    bool work = false;
     for ( Job &j : s->jobs )
     {
-        if ( j.state != RUNNING )
+        if ( j.state != RUNNING || j.kind != kind )
             continue;
         int n = 0, ls = 0, state = 0;
-        if ( ( st = cmd_batch_system_info( batch_of( s, j ), j.handle, &n, &ls, &state ) ) !=
-             CMD_OK )
+        if ( ( st = cmd_batch_system_info( b, j.handle, &n, &ls, &state ) ) != CMD_OK )
             return st;
+        work = true;
         if ( state == 1 && !boundary )
-        {
-            waiting = true; // joins at the next boundary
             continue;
-        }
-        stepping = true;
         k = std::min( k, j.nsteps - ls );
     }
-    // with only waiting jobs the batches move on to the boundary
-    if ( !stepping && !waiting )
-        return CMD_OK;
-    k = std::max( k, 0 );
+    *k_out = work ? std::max( k, 1 ) : 0;
+    return CMD_OK;
+}
+
+cmd_status retire_finished( cmd_server *s, int kind )
+{
+    cmd_batch *b = kind == 2 ? s->nnp : s->lj;
+    for ( Job &j : s->jobs )
+    {
+        if ( j.state != RUNNING || j.kind != kind )
+            continue;
+        int n = 0, ls = 0, state = 0;
+        cmd_status st = cmd_batch_system_info( b, j.handle, &n, &ls, &state );
+        if ( st != CMD_OK )
+            return st;
+        if ( state == 2 && ls >= j.nsteps )
+            if ( ( st = retire( s, j ) ) != CMD_OK )
+                return st;
+    }
+    return CMD_OK;
+}
+
+// One round.  The NNP batch steps one chunk on a worker thread while this
+// thread keeps admitting, stepping and retiring LJ jobs chunk by chunk, so
+// short LJ jobs do not wait behind long NNP chunks; NNP jobs are admitted
+// and retired between rounds (the worker owns the NNP batch meanwhile).
+cmd_status advance( cmd_server *s, bool *progress )
+{
+    *progress = false;
     const double t0 = clock_s();
-    cmd_status st_nnp = CMD_OK;
-    std::thread th;
-    // both batches always advance together so their rebuild steps coincide;
-    // an empty batch only moves its step counter
+    cmd_status st = CMD_OK, st_nnp = CMD_OK;
+    int kn = 0;
     if ( s->nnp )
-        th = std::thread( [&] { st_nnp = cmd_batch_run( s->nnp, k ); } );
-    st = cmd_batch_run( s->lj, k );
+    {
+        if ( ( st = admit( s, 2, t0 - s->t0 ) ) != CMD_OK )
+            return st;
+        if ( ( st = chunk( s, 2, &kn ) ) != CMD_OK )
+            return st;
+    }
+    std::atomic<bool> busy( kn > 0 );
+    std::thread th;
+    if ( kn > 0 )
+        th = std::thread( [&] {
+            st_nnp = cmd_batch_run( s->nnp, kn );
+            busy = false;
+        } );
+    do
+    {
+        if ( ( st = admit( s, 0, clock_s() - s->t0 ) ) != CMD_OK )
+            break;
+        int kl = 0;
+        if ( ( st = chunk( s, 0, &kl ) ) != CMD_OK )
+            break;
+        if ( kl > 0 )
+        {
+            if ( ( st = cmd_batch_run( s->lj, kl ) ) != CMD_OK )
+                break;
+            if ( ( st = retire_finished( s, 0 ) ) != CMD_OK )
+                break;
+            *progress = true;
+        }
+        else if ( busy )
+            std::this_thread::sleep_for( std::chrono::microseconds( 200 ) );
+    } while ( busy );
     if ( th.joinable() )
         th.join();
     s->busy_s += clock_s() - t0;
@@ -202,21 +262,11 @@ cmd_status advance( cmd_server *s, int *ran )
         return st;
     if ( st_nnp != CMD_OK )
         return st_nnp;
-    *ran = k;
-    for ( Job &j : s->jobs )
+    if ( kn > 0 )
     {
-        if ( j.state != RUNNING )
-            continue;
-        int n = 0, ls = 0, state = 0;
-        if ( ( st = cmd_batch_system_info( batch_of( s, j ), j.handle, &n, &ls, &state ) ) !=
-             CMD_OK )
+        *progress = true;
+        if ( ( st = retire_finished( s, 2 ) ) != CMD_OK )
             return st;
-        if ( state == 2 && ls >= j.nsteps )
-        {
-            s->atom_steps += (double)j.n * j.nsteps;
-            if ( ( st = retire( s, j ) ) != CMD_OK )
-                return st;
-        }
     }
     return CMD_OK;
 }
@@ -342,10 +392,8 @@ cmd_status cmd_server_poll( cmd_server *s, int *queued, int *running, int *done 
 {
     if ( !s )
         return CMD_ERR_ARG;
-    cmd_status st = admit( s, clock_s() - s->t0 );
-    int ran = 0;
-    if ( st == CMD_OK )
-        st = advance( s, &ran );
+    bool progress = false;
+    cmd_status st = advance( s, &progress );
     int q = 0, r = 0, d = 0;
     for ( const Job &j : s->jobs )
         ( j.state == QUEUED ? q : ( j.state == RUNNING ? r : d ) )++;
@@ -378,10 +426,15 @@ cmd_status cmd_server_drain( cmd_server *s, double timeout_s )
             cmd::set_error( "server drain timed out with %d queued and %d running jobs", q, r );
             return CMD_ERR_STATE;
         }
-        if ( r == 0 && s->next_admit < s->jobs.size() )
+        if ( r == 0 )
         {
-            const double wait = s->jobs[s->next_admit].t_arrival - ( clock_s() - s->t0 );
-            if ( wait > 0 )
+            // idle: sleep until the next arrival
+            double next = 1e300;
+            for ( const Job &j : s->jobs )
+                if ( j.state == QUEUED )
+                    next = std::min( next, j.t_arrival );
+            const double wait = next - ( clock_s() - s->t0 );
+            if ( wait > 0 && next < 1e299 )
                 std::this_thread::sleep_for( std::chrono::duration<double>( wait ) );
         }
     }
