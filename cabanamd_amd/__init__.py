@@ -211,13 +211,13 @@ class NeighborVerlet:
     """NeighborVerlet<System, Full|Half, CSR> (neighbor_verlet.h:19-70)."""
 
     def __init__(self, neigh_cut, half_neigh, max_neigh_guess=50, lanes=1,
-                 tile=0, lds_planes=True):
+                 tile=0):
         h = C.c_void_p()
         call("cmd_neigh_create", neigh_cut, int(half_neigh),
              int(max_neigh_guess), C.byref(h))
         self._h = h
         call("cmd_neigh_set_lanes", h, int(lanes))
-        call("cmd_neigh_set_tile", h, int(tile), int(bool(lds_planes)))
+        call("cmd_neigh_set_tile", h, int(tile))
         self.neigh_cut = neigh_cut
         self.half_neigh = bool(half_neigh)
 

@@ -93,7 +93,7 @@ _PROTOS = {
     "cmd_neigh_create": ([C.c_double, C.c_int, C.c_int, C.POINTER(VP)], None),
     "cmd_neigh_destroy": ([VP], None),
     "cmd_neigh_set_lanes": ([VP, C.c_int], None),
-    "cmd_neigh_set_tile": ([VP, C.c_int, C.c_int], None),
+    "cmd_neigh_set_tile": ([VP, C.c_int], None),
     "cmd_neigh_build": ([VP, VP], None),
     "cmd_neigh_info": ([VP, IP, LP, IP, IP], None),
     "cmd_neigh_download": ([VP, LP, IP], None),

@@ -268,7 +268,6 @@ struct cmd_neigh
     cmd::DevBuf<int64_t> tile_off; // slab offset of each tile (16-bit units)
     cmd::DevBuf<unsigned short> tslab;
     int max_tile_slots = 0;
-    bool tile_planes = true; // tiled force stages coordinate planes
     const int *gperm_view = nullptr; // the system's ghost sort at build time
 };
 

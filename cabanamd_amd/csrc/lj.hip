@@ -107,10 +107,12 @@ __device__ __forceinline__ double lj_pair( double rsq, const PairCoef &c,
     }
     else
     {
-        const float r2inv = __builtin_amdgcn_rcpf( (float)rsq );
+        // outside the cutoff the reciprocal of +inf is 0, so fp = -0 and
+        // the accumulation adds nothing: one select on the way in
+        const float r2inv = __builtin_amdgcn_rcpf( in ? (float)rsq : __builtin_inff() );
         const float r6inv = r2inv * r2inv * r2inv;
         const float fp = ( r6inv * __builtin_fmaf( c.lj1f, r6inv, -c.lj2f ) ) * r2inv;
-        const double fpair = (double)( in ? fp : 0.0f );
+        const double fpair = (double)fp;
         if ( ENERGY )
         {
             const float ep =
@@ -249,6 +251,8 @@ __device__ __forceinline__ void walk_row( const int *__restrict__ row, int nr,
 // -- integrator_nve.h:91-110 and comm_mpi.h:141-168 -- adding m v^2 to mv2.
 // With G >= 4 lanes per atom lane s handles component s; otherwise lane 0
 // does all three.  Both orders run the same operations on the same values.
+// vin, when given, holds the velocity components the lane would load
+// (component s, or all three), read early so their latency is hidden.
 template <int G, bool FUSED>
 __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                                              const double4 &xi, double fx,
@@ -257,7 +261,8 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                                              double *__restrict__ v,
                                              double *__restrict__ f,
                                              const MassTable &mt,
-                                             const StepCtl &ctl, double &mv2 )
+                                             const StepCtl &ctl, double &mv2,
+                                             const double *vin = nullptr )
 {
     if ( G >= 4 )
     {
@@ -276,7 +281,7 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                 {
                     const double m = mt.m[(int)xi.w];
                     const double dtfm = ctl.dtf / m;
-                    double vc = v[c * scap + i] + dtfm * fc;
+                    double vc = ( vin ? vin[0] : v[c * scap + i] ) + dtfm * fc;
                     mv2 += vc * vc * m;
                     if ( ctl.do_next_initial )
                     {
@@ -312,9 +317,9 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
         {
             const double m = mt.m[(int)xi.w];
             const double dtfm = ctl.dtf / m;
-            double vx = v[i] + dtfm * fx;
-            double vy = v[scap + i] + dtfm * fy;
-            double vz = v[2 * scap + i] + dtfm * fz;
+            double vx = ( vin ? vin[0] : v[i] ) + dtfm * fx;
+            double vy = ( vin ? vin[1] : v[scap + i] ) + dtfm * fy;
+            double vz = ( vin ? vin[2] : v[2 * scap + i] ) + dtfm * fz;
             mv2 += ( vx * vx + vy * vy + vz * vz ) * m;
             if ( ctl.do_next_initial )
             {
@@ -410,7 +415,7 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
 // the tile's sentinel slot (a position far outside any cutoff), so all lanes
 // run the same number of iterations with no per-pair bounds test; inactive
 // lanes sit at another far point.  A padded pair adds exact zeros.
-template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY, bool PLANES>
+template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY>
 __global__ __launch_bounds__( TPB ) void k_tile_force(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
     int lcap, int scap, int slot_cap, const double4 *__restrict__ x,
@@ -420,23 +425,18 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     const LJParams *__restrict__ Pd, PairCoef one, MassTable mt, StepCtl ctl,
     double *__restrict__ partial )
 {
-    extern __shared__ double4 s_pos[];
+    extern __shared__ double s_rec[];
     __shared__ TileHead H;
     constexpr int APW = 64 / G;
     const int t = xcd_tile( blockIdx.x, gridDim.x );
     load_head( info, t, H );
-    double *s_x = reinterpret_cast<double *>( s_pos );
-    const double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
-    const int *s_t = reinterpret_cast<const int *>( s_x + 3 * slot_cap );
-    if ( PLANES )
-        stage_planes<!ONE_TYPE>( H, x, xg, s_x, slot_cap );
-    else
-        stage_positions( H, x, xg, s_pos );
+    const char *s_byte = reinterpret_cast<const char *>( s_rec );
+    const int *s_t = reinterpret_cast<const int *>( s_rec + 3 * slot_cap );
+    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
     const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
     const int nw = woff[t + 1] - woff[t];
     const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
-    const int self0 = H.segoff[8] - H.info[TI_LB + 4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int a = lane % APW, sub = lane / APW;
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
@@ -447,8 +447,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         const int i = tl + ta;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const double4 xi =
-            active ? ( PLANES ? x[i] : s_pos[self0 + i] )
-                   : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
+            active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
         const int ti = (int)xi.w;
         const int n = active ? counts[i] : 0;
         int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
@@ -470,30 +469,47 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             const double fpair = lj_pair<PREC, ENERGY>( rsq, c, pe, vir );
             accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
         };
-        // four positions per 8-byte word; all four LDS reads are issued
-        // before the arithmetic that consumes them
+        // four records per 8-byte word of byte offsets; all four LDS reads
+        // are issued before the arithmetic that consumes them
         auto fetch = [&]( unsigned long long wd, int q ) -> double4
         {
-            const int sj = (int)( ( wd >> ( 16 * q ) ) & 0xffffull );
-            if ( PLANES )
-                return make_double4( s_x[sj], s_y[sj], s_z[sj],
-                                     ONE_TYPE ? 0.0 : (double)s_t[sj] );
-            if ( ONE_TYPE )
-            {
-                const double2 a = *reinterpret_cast<const double2 *>( s_pos + sj );
-                const double c = reinterpret_cast<const double *>( s_pos + sj )[2];
-                return make_double4( a.x, a.y, c, 0.0 );
-            }
-            return s_pos[sj];
+            const unsigned off = (unsigned)( ( wd >> ( 16 * q ) ) & 0xffffull );
+            const double *p = reinterpret_cast<const double *>( s_byte + off );
+            return make_double4( p[0], p[1], p[2],
+                                 ONE_TYPE ? 0.0 : (double)s_t[off / TILE_REC] );
         };
-        unsigned long long nxt = 0ull;
+        // the velocities the epilogue needs, loaded before the pair loop
+        double vpre[3] = { 0.0, 0.0, 0.0 };
+        if ( FUSED && active && ctl.do_final )
+        {
+            if ( G >= 4 )
+            {
+                if ( sub < 3 )
+                    vpre[0] = v[sub * scap + i];
+            }
+            else if ( sub == 0 )
+            {
+                vpre[0] = v[i];
+                vpre[1] = v[scap + i];
+                vpre[2] = v[2 * scap + i];
+            }
+        }
+        // list words three iterations ahead: under load the memory latency
+        // spans several iterations of all waves of a SIMD
+        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull;
         if ( nr > 0 )
-            nxt = __builtin_nontemporal_load( row8 );
+            q0 = __builtin_nontemporal_load( row8 );
+        if ( nr > 4 )
+            q1 = __builtin_nontemporal_load( row8 + 64 );
+        if ( nr > 8 )
+            q2 = __builtin_nontemporal_load( row8 + 128 );
         for ( int r = 0; r < nr; r += 4 )
         {
-            const unsigned long long cur = nxt;
-            if ( r + 4 < nr )
-                nxt = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 1 ) * 64 );
+            const unsigned long long cur = q0;
+            q0 = q1;
+            q1 = q2;
+            if ( r + 12 < nr )
+                q2 = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 3 ) * 64 );
             const double4 p0 = fetch( cur, 0 );
             const double4 p1 = fetch( cur, 1 );
             const double4 p2 = fetch( cur, 2 );
@@ -511,7 +527,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             fz += __shfl_xor( fz, o, 64 );
         }
         finish_atom<G, FUSED>( active, sub, i, xi, fx, fy, fz, scap, x_next, v,
-                               f, mt, ctl, mv2 );
+                               f, mt, ctl, mv2, vpre );
     }
     if ( FUSED && ctl.thermo )
         block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
@@ -731,17 +747,16 @@ static cmd_status launch_full_g( int G, int nb, hipStream_t st, int n, int scap,
     return CMD_OK;
 }
 
-template <int G, int PREC, bool ONE, bool FUSED, bool ENERGY, bool PLANES>
+template <int G, int PREC, bool ONE, bool FUSED, bool ENERGY>
 static void launch_tile_k( cmd_neigh *N, cmd_system *S, double4 *xn,
                            const LJParams *Pd, const PairCoef &c,
                            const MassTable &mt, const StepCtl &ctl,
                            double *partial )
 {
-    // staged positions plus the sentinel slot
+    // staged records plus the sentinel (and an int type per slot)
     const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
-    const size_t lds = PLANES ? (size_t)slot_cap * ( 3 * sizeof( double ) + ( ONE ? 0 : sizeof( int ) ) )
-                              : (size_t)slot_cap * sizeof( double4 );
-    k_tile_force<G, PREC, ONE, FUSED, ENERGY, PLANES><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+    const size_t lds = (size_t)slot_cap * ( TILE_REC + ( ONE ? 0 : sizeof( int ) ) );
+    k_tile_force<G, PREC, ONE, FUSED, ENERGY><<<N->tg.ntiles, TPB, lds, S->stream>>>(
         N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
         S->x.p, S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt,
         ctl, partial );
@@ -753,22 +768,15 @@ static void launch_tile_t( cmd_neigh *N, cmd_system *S, double4 *xn,
                            const MassTable &mt, const StepCtl &ctl,
                            double *partial )
 {
-    const bool planes = N->tile_planes;
     if constexpr ( FUSED )
     {
         if ( ctl.thermo )
         {
-            if ( planes )
-                launch_tile_k<G, PREC, ONE, FUSED, true, true>( N, S, xn, Pd, c, mt, ctl, partial );
-            else
-                launch_tile_k<G, PREC, ONE, FUSED, true, false>( N, S, xn, Pd, c, mt, ctl, partial );
+            launch_tile_k<G, PREC, ONE, FUSED, true>( N, S, xn, Pd, c, mt, ctl, partial );
             return;
         }
     }
-    if ( planes )
-        launch_tile_k<G, PREC, ONE, FUSED, false, true>( N, S, xn, Pd, c, mt, ctl, partial );
-    else
-        launch_tile_k<G, PREC, ONE, FUSED, false, false>( N, S, xn, Pd, c, mt, ctl, partial );
+    launch_tile_k<G, PREC, ONE, FUSED, false>( N, S, xn, Pd, c, mt, ctl, partial );
 }
 
 template <int PREC, bool ONE, bool FUSED>
@@ -1024,15 +1032,14 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     {
         int G, E;
     };
-    const Cand cands[] = { { 1, 0 }, { 2, 4 }, { 2, 6 }, { 4, 4 },
-                           { 4, 6 }, { 8, 4 }, { 8, 6 } };
+    const Cand cands[] = { { 1, 0 }, { 2, 4 }, { 2, 6 }, { 2, 8 }, { 4, 4 },
+                           { 4, 6 }, { 4, 8 }, { 8, 4 }, { 8, 6 } };
     StepCtl ctl{};
     hipEvent_t e0, e1;
     CMD_HIP( hipEventCreate( &e0 ) );
     CMD_HIP( hipEventCreate( &e1 ) );
     float best = 1e30f;
     Cand bestc = { 1, 0 };
-    bool best_planes = true;
     cmd_status st = CMD_OK;
     for ( const Cand &cd : cands )
     {
@@ -1043,34 +1050,24 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
             break;
         if ( cd.E > 0 && !N->tiled )
             continue; // the tiled layout does not apply here
-        // LDS layout of the tiled force: records, then coordinate planes
-        for ( int pl = 0; pl < ( N->tiled ? 2 : 1 ); ++pl )
-        {
-            N->tile_planes = pl == 1;
-            st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
-            if ( st != CMD_OK )
-                break;
-            (void)hipEventRecord( e0, S->stream );
-            for ( int r = 0; r < 3; ++r )
-                (void)lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
-            (void)hipEventRecord( e1, S->stream );
-            (void)hipEventSynchronize( e1 );
-            float ms = 0.0f;
-            (void)hipEventElapsedTime( &ms, e0, e1 );
-            if ( std::getenv( "CMD_VERBOSE" ) )
-                std::fprintf( stderr, "# autotune lanes=%d tile=%d%s: %.4f ms/launch\n",
-                              cd.G, cd.E,
-                              N->tiled ? ( N->tile_planes ? " (lds planes)" : " (lds records)" ) : "",
-                              ms / 3.0f );
-            if ( ms < best )
-            {
-                best = ms;
-                bestc = cd;
-                best_planes = N->tile_planes;
-            }
-        }
+        st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
         if ( st != CMD_OK )
             break;
+        (void)hipEventRecord( e0, S->stream );
+        for ( int r = 0; r < 3; ++r )
+            (void)lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+        (void)hipEventRecord( e1, S->stream );
+        (void)hipEventSynchronize( e1 );
+        float ms = 0.0f;
+        (void)hipEventElapsedTime( &ms, e0, e1 );
+        if ( std::getenv( "CMD_VERBOSE" ) )
+            std::fprintf( stderr, "# autotune lanes=%d tile=%d: %.4f ms/launch\n", cd.G, cd.E,
+                          ms / 3.0f );
+        if ( ms < best )
+        {
+            best = ms;
+            bestc = cd;
+        }
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
@@ -1079,7 +1076,6 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     F->tuned = true;
     N->lanes = bestc.G;
     N->tile_e = bestc.E;
-    N->tile_planes = best_planes;
     return neigh_build( N, S );
 }
 

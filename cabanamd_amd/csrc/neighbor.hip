@@ -462,7 +462,7 @@ cmd_status cmd_neigh_destroy( cmd_neigh *n )
     return CMD_OK;
 }
 
-cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height, int lds_planes )
+cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
     CMD_REQUIRE( height >= 0 && height <= 14, CMD_ERR_ARG,
@@ -473,7 +473,6 @@ cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height, int lds_planes )
         n->n_rows = 0; // layout changed: must rebuild
         n->csr_valid = false;
     }
-    n->tile_planes = lds_planes != 0;
     return CMD_OK;
 }
 

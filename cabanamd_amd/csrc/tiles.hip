@@ -11,7 +11,8 @@
 // column that is one contiguous range of local atoms (locals are cell sorted
 // by create_binning) and one contiguous range of ghosts (ghosts are cell
 // sorted here).  A block stages those 18 ranges of positions in LDS and the
-// list stores, for each neighbour, its 16-bit LDS slot.  Rows are walked in
+// list stores, for each neighbour, the 16-bit byte offset of its LDS record
+// (slot * TILE_REC).  Rows are walked in
 // the order the oracle uses (stencil columns, then cells, then atoms by
 // index), so lists and FP64 sums are unchanged.
 //
@@ -33,7 +34,9 @@ namespace cmd
 static constexpr int TPB = 256;
 static constexpr int BTPB = 512;       // list build block
 static constexpr int EMAX = 16;
-static constexpr int MAX_SLOTS = 1792; // 56 KB of double4 per block
+static constexpr int MAX_SLOTS = 1792; // 42 KB of 24-byte records per block
+static_assert( ( MAX_SLOTS + 1 ) * TILE_REC + 16 < 65536,
+               "record offsets must fit the 16-bit list entries" );
 // Per-tile runs, atom range and staged count; waves per tile for the slab.
 // slots[0] gets the largest staged count, slots[1] the number of local atoms
 // the tiles cover (all of them only when every local atom sits in an
@@ -156,9 +159,9 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
     const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
-    // rows start as sentinel slots (the tile's slot TOTAL)
+    // rows start as the sentinel record (the tile's slot TOTAL)
     {
-        const unsigned sent = (unsigned)H.segoff[18];
+        const unsigned sent = (unsigned)( H.segoff[18] * TILE_REC );
         const unsigned sent2 = sent | ( sent << 16 );
         const uint4 fill = make_uint4( sent2, sent2, sent2, sent2 );
         const int n16 = nw * wsz / 8;
@@ -292,7 +295,7 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
                         const unsigned r = k / G;
                         const unsigned L = ( k % G ) * APW + ta % APW;
                         chunk[( ( ( ta / APW ) * R4 + ( r >> 2 ) ) * 64 + L ) * 4 + ( r & 3 )] =
-                            (unsigned short)sj;
+                            (unsigned short)( sj * TILE_REC );
                     };
                     if ( acc1 && k1 < (unsigned)cap )
                         put( l, k1 );
@@ -353,7 +356,8 @@ __global__ __launch_bounds__( TPB ) void k_tile_to_csr(
         for ( int k = 0; k < n; ++k )
         {
             const int r = k / G, lane = ( k % G ) * APW + aw;
-            const int sj = st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )];
+            const int sj =
+                st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )] / TILE_REC;
             int seg = 0;
             while ( sj >= H.segoff[seg + 1] )
                 ++seg;

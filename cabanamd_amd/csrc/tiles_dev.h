@@ -78,12 +78,22 @@ __device__ inline void load_head( const int *__restrict__ info, int t, TileHead 
 // beyond every cutoff, so a padded pair contributes exact zeros.
 static constexpr double TILE_FAR = 1.0e30;
 
-// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS,
-// followed by the sentinel slot.
-__device__ inline void stage_positions( const TileHead &H, const double4 *__restrict__ x,
-                                 const double4 *__restrict__ xg,
-                                 double4 *__restrict__ s_pos )
+// Bytes of one staged position in the force kernel's LDS records (x, y, z
+// doubles).  Tiled list entries are byte offsets of records (slot * 24), so
+// a gather is one address add and two LDS reads with immediate offsets; the
+// largest tile (MAX_SLOTS staged atoms plus the sentinel) must keep every
+// offset inside 16 bits.
+static constexpr int TILE_REC = 24;
+
+// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS as
+// 24-byte records, followed by the sentinel record; with WITH_TYPE the types
+// follow as ints, `cap` records on.
+template <bool WITH_TYPE>
+__device__ inline void stage_records( const TileHead &H, const double4 *__restrict__ x,
+                                      const double4 *__restrict__ xg,
+                                      double *__restrict__ s_rec, int cap )
 {
+    int *s_t = reinterpret_cast<int *>( s_rec + 3 * cap );
     const int total = H.segoff[18];
     constexpr int B = 4; // slots per thread whose loads are in flight together
     int seg = 0;
@@ -102,14 +112,27 @@ __device__ inline void stage_positions( const TileHead &H, const double4 *__rest
             const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
             p[u] = base[idx];
         }
-        // stores past the end go to the sentinel slot, rewritten below
+        // stores past the end go to the sentinel record, rewritten below
 #pragma unroll
         for ( int u = 0; u < B; ++u )
-            s_pos[min( k0 + u * (int)blockDim.x, total )] = p[u];
+        {
+            const int k = min( k0 + u * (int)blockDim.x, total );
+            s_rec[3 * k] = p[u].x;
+            s_rec[3 * k + 1] = p[u].y;
+            s_rec[3 * k + 2] = p[u].z;
+            if ( WITH_TYPE )
+                s_t[k] = (int)p[u].w;
+        }
     }
     __syncthreads();
     if ( threadIdx.x == 0 )
-        s_pos[total] = make_double4( TILE_FAR, TILE_FAR, TILE_FAR, 0.0 );
+    {
+        s_rec[3 * total] = TILE_FAR;
+        s_rec[3 * total + 1] = TILE_FAR;
+        s_rec[3 * total + 2] = TILE_FAR;
+        if ( WITH_TYPE )
+            s_t[total] = 0;
+    }
     __syncthreads();
 }
 

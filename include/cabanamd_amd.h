@@ -154,10 +154,9 @@ cmd_status cmd_neigh_destroy( cmd_neigh *n );
 /* Lanes per atom of the list's device layout (1, 2, 4 or 8; default 1).  The
  * force kernels follow the layout of the list they are given. */
 cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes );
-/* Tiled layout controls (no reference counterpart; for tuning and tests):
- * tile height in binning cells (0 = choose) and whether the tiled force
- * kernel stages coordinate planes (1) or 32-byte records (0) in LDS. */
-cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height, int lds_planes );
+/* Tiled layout control (no reference counterpart; for tuning and tests):
+ * tile height in binning cells (0 = choose). */
+cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height );
 cmd_status cmd_neigh_build( cmd_neigh *n, cmd_system *s );
 cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
                            int *max_row, int *max_neigh_guess );

@@ -230,11 +230,11 @@ def test_neighbor_on_binning_grid_exact(gpu, half, lanes):
     assert np.array_equal(idx, idx_o)
 
 
-@pytest.mark.parametrize("planes", [True, False])
+@pytest.mark.parametrize("tile", [0, 8])
 @pytest.mark.parametrize("wrapped", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("lanes", [2, 4, 8])
-def test_lj_tiled_force(gpu, prec, lanes, wrapped, planes):
+def test_lj_tiled_force(gpu, prec, lanes, wrapped, tile):
     """Full list built on the binning grid with 2+ lanes per atom uses the
     LDS-tiled layout when every local atom lies in an interior cell; atoms
     left unwrapped in the padding cells make the build use the global layout.
@@ -251,7 +251,7 @@ def test_lj_tiled_force(gpu, prec, lanes, wrapped, planes):
     M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
     comm = M.Comm(s, 2.8)
     comm.exchange_halo()
-    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes, lds_planes=planes)
+    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes, tile=tile)
     nl.create(s)
     F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
     F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
@@ -404,8 +404,7 @@ MULTI = [["pair_coeff", "1", "1", "1.0", "1.0", "2.5"],
          ["pair_coeff", "2", "2", "1.3", "0.9", "2.2"]]
 
 
-@pytest.mark.parametrize("path", ["global1", "tiled-planes", "tiled-records",
-                                  "half"])
+@pytest.mark.parametrize("path", ["global1", "tiled", "tiled-e8", "half"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_lj_two_types(gpu, prec, path):
     """Two atom types with their own eps, sigma and cutoff per pair
@@ -427,7 +426,7 @@ def test_lj_two_types(gpu, prec, path):
     half = path == "half"
     lanes = 1 if path == "global1" else 2
     nl = M.NeighborVerlet(2.8, half, 50, lanes=lanes,
-                          lds_planes=(path != "tiled-records"))
+                          tile=(8 if path == "tiled-e8" else 0))
     nl.create(s)
     F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
     F.init_coeff(MULTI)
