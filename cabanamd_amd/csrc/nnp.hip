@@ -53,7 +53,8 @@ static constexpr int NA_MAX = AG_MAX * AP_MAX; // 64
 static constexpr int NSF_LD = 96;              // row stride of G / dE/dG
 static constexpr int KPAD = 96;                // MFMA K of layer 1 (6 x 16)
 static constexpr int HPAD = 32;                // hidden width (2 x 16)
-static constexpr int NN_MAX = 192;             // neighbours inside rc
+static constexpr int NN_MAX = 128;             // neighbours inside rc
+static constexpr int TCAP = 512;               // compacted neighbour pairs per wave
 static constexpr int NNP_TPB = 256;
 
 struct NnpElemPlan
@@ -80,7 +81,7 @@ struct NnpElemPlan
 struct NnpPlan
 {
     int nelem;
-    float rc, rc2;
+    float rc, rc2, pi_rc; // pi_rc = pi / rc
     NnpElemPlan el[NNP_MAXEL];
 };
 } // namespace cmd
@@ -294,18 +295,111 @@ static cmd_status build_plan( const cmd_nnp_model &m, NnpPlan &P )
     }
     P.rc = (float)rc;
     P.rc2 = (float)( rc * rc );
+    P.pi_rc = (float)( 3.14159265358979323846 / rc );
     return CMD_OK;
 }
 
 // ---------------------------------------------------------------- device
-__device__ __forceinline__ float fcut( float r, float rc )
+// Cosine cutoff 0.5 (cos(pi r / rc) + 1) and its derivative; k = pi / rc.
+__device__ __forceinline__ float fcut( float r, float rc, float k )
 {
-    return r < rc ? 0.5f * ( __cosf( 3.14159265358979f * r / rc ) + 1.0f ) : 0.0f;
+    return r < rc ? 0.5f * ( __cosf( k * r ) + 1.0f ) : 0.0f;
 }
-__device__ __forceinline__ float dfcut( float r, float rc )
+__device__ __forceinline__ float dfcut( float r, float rc, float k )
 {
-    return r < rc ? -0.5f * 3.14159265358979f / rc * __sinf( 3.14159265358979f * r / rc )
-                  : 0.0f;
+    return r < rc ? -0.5f * k * __sinf( k * r ) : 0.0f;
+}
+
+// The angular constants of one element, held in scalar registers through
+// the triplet loops (read once per wave from LDS, so the compiler cannot
+// reload them from memory inside the loop).
+struct AngK
+{
+    int nag, nap;
+    unsigned lampos; // bit pp: lambda_pp > 0
+    float eta[AG_MAX], rs[AG_MAX];
+    float zeta[AP_MAX], lpref[AP_MAX], zm1[AP_MAX], zl[AP_MAX];
+};
+
+__device__ __forceinline__ AngK load_angk( const NnpElemPlan &Q, float *stage )
+{
+    const int lane = threadIdx.x & 63;
+    // stage: eta[8] rs[8] zeta[8] lpref[8] lam[8]
+    if ( lane < 40 )
+    {
+        const int k = lane & 7, w = lane >> 3;
+        float v = 0.f;
+        if ( w == 0 )
+            v = Q.a_eta[k];
+        else if ( w == 1 )
+            v = Q.a_rs[k];
+        else if ( w == 2 )
+            v = Q.a_zeta[k];
+        else if ( w == 3 )
+            v = Q.a_lpref[k];
+        else
+            v = Q.a_lam[k];
+        stage[lane] = v;
+    }
+    __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+    AngK K;
+    K.nag = __builtin_amdgcn_readfirstlane( Q.nag );
+    K.nap = __builtin_amdgcn_readfirstlane( Q.nap );
+    K.lampos = 0u;
+#pragma unroll
+    for ( int k = 0; k < AG_MAX; ++k )
+    {
+        K.eta[k] = __builtin_bit_cast( float, __builtin_amdgcn_readfirstlane(
+                                                   __builtin_bit_cast( int, stage[k] ) ) );
+        K.rs[k] = __builtin_bit_cast( float, __builtin_amdgcn_readfirstlane(
+                                                  __builtin_bit_cast( int, stage[8 + k] ) ) );
+    }
+#pragma unroll
+    for ( int k = 0; k < AP_MAX; ++k )
+    {
+        const float z = __builtin_bit_cast( float, __builtin_amdgcn_readfirstlane(
+                                                       __builtin_bit_cast( int, stage[16 + k] ) ) );
+        const float lp = __builtin_bit_cast( float, __builtin_amdgcn_readfirstlane(
+                                                        __builtin_bit_cast( int, stage[24 + k] ) ) );
+        const float lam = __builtin_bit_cast( float, __builtin_amdgcn_readfirstlane(
+                                                         __builtin_bit_cast( int, stage[32 + k] ) ) );
+        K.zeta[k] = z;
+        K.lpref[k] = lp;
+        K.zm1[k] = z - 1.f;
+        K.zl[k] = z * lam;
+        if ( lam > 0.f )
+            K.lampos |= 1u << k;
+    }
+    __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+    return K;
+}
+
+// Angular factors of the element's (lambda, zeta) grid for cos theta = cs:
+// Pw = 2^(1-zeta) (1 + lambda cs)^zeta and Dw = zeta lambda 2^(1-zeta)
+// (1 + lambda cs)^(zeta-1), the derivative of Pw with respect to cs, through
+// the hardware log2/exp2 (exp2(-inf) = 0); base^(zeta-1) = 1 for zeta = 1
+// also at base = 0 (collinear neighbours).
+template <bool WANT_D>
+__device__ __forceinline__ void ang_powers( const AngK &K, float cs, float *Pw, float *Dw )
+{
+    const float lbp = __builtin_amdgcn_logf( fmaxf( 1.0f + cs, 0.0f ) );
+    const float lbm = __builtin_amdgcn_logf( fmaxf( 1.0f - cs, 0.0f ) );
+#pragma unroll
+    for ( int pp = 0; pp < AP_MAX; ++pp )
+    {
+        Pw[pp] = 0.0f;
+        if ( WANT_D )
+            Dw[pp] = 0.0f;
+        if ( pp < K.nap )
+        {
+            const float lb = ( K.lampos >> pp ) & 1u ? lbp : lbm;
+            Pw[pp] = __builtin_amdgcn_exp2f( K.zeta[pp] * lb + K.lpref[pp] );
+            if ( WANT_D )
+                Dw[pp] = K.zl[pp] * __builtin_amdgcn_exp2f(
+                                        ( K.zm1[pp] != 0.f ? K.zm1[pp] * lb : 0.f ) +
+                                        K.lpref[pp] );
+        }
+    }
 }
 
 // Reduce-scatter across the 64 lanes: in: v[S] per lane; out: v[0] of
@@ -339,11 +433,12 @@ __device__ __forceinline__ void wave_reduce_scatter( float *v )
         v[0] += __shfl_xor( v[0], 32, 64 );
 }
 
+// Neighbours of one atom inside rc: d = (x_j - x_i, r) and
+// e = (1 / r, fc(r), dfc(r), element as float bits), two 16-byte reads each.
 struct NbLds
 {
-    float4 d[NN_MAX]; // x_j - x_i, r
-    float fc[NN_MAX], dfc[NN_MAX];
-    int el[NN_MAX];
+    float4 d[NN_MAX];
+    float4 e[NN_MAX];
     int j[NN_MAX];
 };
 
@@ -352,7 +447,8 @@ struct NbLds
 __device__ int gather_neighbours( int i, const double4 *__restrict__ x,
                                   const int64_t *__restrict__ off,
                                   const int *__restrict__ idx, float rc,
-                                  float rc2, NbLds &L, int *__restrict__ flag )
+                                  float rc2, float kpi, NbLds &L,
+                                  int *__restrict__ flag )
 {
     const int lane = threadIdx.x & 63;
     const double4 xi = x[i];
@@ -381,11 +477,11 @@ __device__ int gather_neighbours( int i, const double4 *__restrict__ x,
                                   __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) );
         if ( in && pos < NN_MAX )
         {
-            const float r = sqrtf( r2 );
+            const float r = __builtin_amdgcn_sqrtf( r2 );
             L.d[pos] = make_float4( dx, dy, dz, r );
-            L.fc[pos] = fcut( r, rc );
-            L.dfc[pos] = dfcut( r, rc );
-            L.el[pos] = (int)x[j].w;
+            L.e[pos] = make_float4( __builtin_amdgcn_rcpf( r ), fcut( r, rc, kpi ),
+                                    dfcut( r, rc, kpi ),
+                                    __builtin_bit_cast( float, (int)x[j].w ) );
             L.j[pos] = j;
         }
         cnt += (int)__popcll( m );
@@ -415,21 +511,75 @@ __device__ __forceinline__ void pair_of( int p, int n, int &a, int &q )
     q = p - aa * ( 2 * n - aa - 1 ) / 2 + aa + 1;
 }
 
+// The neighbour pairs (a < q) of the gathered list whose third side is
+// inside the cutoff, |x_q - x_a| < rc: tested 64 at a time and compacted
+// into buf (TCAP entries per wave, a | q << 16), then handed to body(a, q)
+// with every lane busy (lane l takes entries l*per .. l*per + per - 1).
+// Used where the body keeps many accumulators in registers.
+template <class Body>
+__device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
+                                             unsigned *buf, Body body )
+{
+    const int lane = threadIdx.x & 63;
+    const int np = n * ( n - 1 ) / 2;
+    for ( int p0 = 0; p0 < np; )
+    {
+        int cnt = 0;
+        while ( cnt <= TCAP - 64 && p0 < np )
+        {
+            const int p = p0 + lane;
+            bool ok = false;
+            unsigned code = 0u;
+            if ( p < np )
+            {
+                int a, q;
+                pair_of( p, n, a, q );
+                const float4 A = L.d[a], B = L.d[q];
+                const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
+                ok = cx * cx + cy * cy + cz * cz < rc2;
+                code = (unsigned)a | ( (unsigned)q << 16 );
+            }
+            const unsigned long long m = __ballot( ok );
+            if ( ok )
+                buf[cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                              (unsigned)( m >> 32 ),
+                              __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) )] = code;
+            cnt += (int)__popcll( m );
+            p0 += 64;
+        }
+        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+        const int per = ( cnt + 63 ) >> 6;
+        for ( int j = 0; j < per; ++j )
+        {
+            const int e = lane * per + j;
+            if ( e < cnt )
+            {
+                const unsigned c = buf[e];
+                body( (int)( c & 0xffffu ), (int)( c >> 16 ) );
+            }
+        }
+        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+    }
+}
+
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     int n_local, const double4 *__restrict__ x, const int64_t *__restrict__ off,
     const int *__restrict__ idx, const NnpPlan *__restrict__ P,
     float *__restrict__ G, int *__restrict__ flag )
 {
     __shared__ NbLds lds[NNP_TPB / 64];
+    __shared__ unsigned tbuf[NNP_TPB / 64][TCAP];
+    __shared__ float kst[NNP_TPB / 64][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * ( NNP_TPB / 64 ) + wv;
     if ( i >= n_local )
         return;
     NbLds &L = lds[wv];
-    const float rc = P->rc, rc2 = P->rc2;
-    const int n = gather_neighbours( i, x, off, idx, rc, rc2, L, flag );
+    const float rc = P->rc, rc2 = P->rc2, kpi = P->pi_rc;
+    const int n = gather_neighbours( i, x, off, idx, rc, rc2, kpi, L, flag );
     const int e = __builtin_amdgcn_readfirstlane( (int)x[i].w );
     const NnpElemPlan &Q = P->el[e];
+    const AngK K = load_angk( Q, kst[wv] );
     float *Gi = G + (size_t)i * NSF_LD;
     // radial: lanes over neighbours
     {
@@ -439,8 +589,9 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
             acc[s] = 0.0f;
         for ( int a = lane; a < n; a += 64 )
         {
-            const float r = L.d[a].w, fc = L.fc[a];
-            const int ej = L.el[a];
+            const float4 Ea = L.e[a];
+            const float r = L.d[a].w, fc = Ea.y;
+            const int ej = __builtin_bit_cast( int, Ea.w );
 #pragma unroll
             for ( int g = 0; g < RG_MAX; ++g )
             {
@@ -463,49 +614,37 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
                 Gi[Q.r_out[g][k]] = acc[0];
         }
     }
-    // angular: lanes over neighbour pairs
+    // angular: lanes over the compacted neighbour pairs
     {
         float acc[NA_MAX];
 #pragma unroll
         for ( int s = 0; s < NA_MAX; ++s )
             acc[s] = 0.0f;
-        const int np = n * ( n - 1 ) / 2;
-        for ( int p = lane; p < np; p += 64 )
-        {
-            int a, q;
-            pair_of( p, n, a, q );
-            const float4 A = L.d[a], B = L.d[q];
+        for_triplets( L, n, rc2, tbuf[wv], [&]( int a, int q ) {
+            const float4 A = L.d[a], B = L.d[q], Ea = L.e[a], Eb = L.e[q];
             const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
-            const float r3s = cx * cx + cy * cy + cz * cz;
-            if ( r3s >= rc2 )
-                continue;
-            const float r3 = sqrtf( r3s );
-            const float F = L.fc[a] * L.fc[q] * fcut( r3, rc );
-            const float cs = ( A.x * B.x + A.y * B.y + A.z * B.z ) / ( A.w * B.w );
-            const float lbp = __log2f( fmaxf( 1.0f + cs, 0.0f ) );
-            const float lbm = __log2f( fmaxf( 1.0f - cs, 0.0f ) );
+            const float r3 = __builtin_amdgcn_sqrtf( cx * cx + cy * cy + cz * cz );
+            const float F = Ea.y * Eb.y * fcut( r3, rc, kpi );
+            const float cs = ( A.x * B.x + A.y * B.y + A.z * B.z ) * ( Ea.x * Eb.x );
             float Pw[AP_MAX];
+            ang_powers<false>( K, cs, Pw, nullptr );
 #pragma unroll
             for ( int pp = 0; pp < AP_MAX; ++pp )
-                Pw[pp] = pp < Q.nap
-                             ? exp2f( Q.a_zeta[pp] * ( Q.a_lam[pp] > 0.f ? lbp : lbm ) +
-                                      Q.a_lpref[pp] ) *
-                                   F
-                             : 0.0f;
+                Pw[pp] *= F;
 #pragma unroll
             for ( int g = 0; g < AG_MAX; ++g )
             {
-                if ( g < Q.nag )
+                if ( g < K.nag )
                 {
-                    const float rs = Q.a_rs[g];
+                    const float rs = K.rs[g];
                     const float t1 = A.w - rs, t2 = B.w - rs, t3 = r3 - rs;
-                    const float Eg = __expf( -Q.a_eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
+                    const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
 #pragma unroll
                     for ( int pp = 0; pp < AP_MAX; ++pp )
                         acc[g * AP_MAX + pp] += Eg * Pw[pp];
                 }
             }
-        }
+        } );
         wave_reduce_scatter<NA_MAX>( acc );
         const int g = lane / AP_MAX, pp = lane % AP_MAX;
         if ( g < Q.nag && pp < Q.nap )
@@ -768,15 +907,18 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     __shared__ float sF[NNP_TPB / 64][NN_MAX][3];
     __shared__ float sC[NNP_TPB / 64][NSF_LD];
     __shared__ float4 sCg[NNP_TPB / 64][NA_MAX / 4]; // c of angular (g, p) in grid order
+    __shared__ unsigned tbuf[NNP_TPB / 64][TCAP];
+    __shared__ float kst[NNP_TPB / 64][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * ( NNP_TPB / 64 ) + wv;
     if ( i >= n_local )
         return;
     NbLds &L = lds[wv];
-    const float rc = P->rc, rc2 = P->rc2;
-    const int n = gather_neighbours( i, x, off, idx, rc, rc2, L, flag );
+    const float rc = P->rc, rc2 = P->rc2, kpi = P->pi_rc;
+    const int n = gather_neighbours( i, x, off, idx, rc, rc2, kpi, L, flag );
     const int e = __builtin_amdgcn_readfirstlane( (int)x[i].w );
     const NnpElemPlan &Q = P->el[e];
+    const AngK K = load_angk( Q, kst[wv] );
     for ( int k = lane; k < NSF_LD; k += 64 )
         sC[wv][k] = k < Q.nsf ? dEdG[(size_t)i * NSF_LD + k] : 0.f;
     for ( int a = lane; a < n; a += 64 )
@@ -791,9 +933,9 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     // radial
     for ( int a = lane; a < n; a += 64 )
     {
-        const float4 A = L.d[a];
-        const float r = A.w, fc = L.fc[a], dfc = L.dfc[a];
-        const int ej = L.el[a];
+        const float4 A = L.d[a], Ea = L.e[a];
+        const float r = A.w, fc = Ea.y, dfc = Ea.z;
+        const int ej = __builtin_bit_cast( int, Ea.w );
         float gsum = 0.f; // sum_s c_s dG_s/dr
 #pragma unroll
         for ( int g = 0; g < RG_MAX; ++g )
@@ -811,7 +953,7 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
                 gsum = fmaf( cw, d, gsum );
             }
         }
-        const float s = gsum / r;
+        const float s = gsum * Ea.x;
         const float vx = s * A.x, vy = s * A.y, vz = s * A.z;
         // dG/dx_j = g' u; F_j -= c g' u, F_i += c g' u
         sF[wv][a][0] -= vx;
@@ -834,55 +976,47 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
     const float4 *cg4 = sCg[wv];
-    const int np = n * ( n - 1 ) / 2;
-    for ( int p = lane; p < np; p += 64 )
-    {
-        int a, q;
-        pair_of( p, n, a, q );
-        const float4 A = L.d[a], B = L.d[q];
-        const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
-        const float r3s = cx * cx + cy * cy + cz * cz;
-        if ( r3s >= rc2 )
-            continue;
-        const float r1 = A.w, r2 = B.w, r3 = sqrtf( r3s );
-        const float f1 = L.fc[a], f2 = L.fc[q], f3 = fcut( r3, rc );
-        const float d1 = L.dfc[a], d2 = L.dfc[q], d3 = dfcut( r3, rc );
-        const float F = f1 * f2 * f3;
-        const float inv12 = 1.f / ( r1 * r2 );
-        const float cs = ( A.x * B.x + A.y * B.y + A.z * B.z ) * inv12;
-        const float lbp = __log2f( fmaxf( 1.0f + cs, 0.0f ) );
-        const float lbm = __log2f( fmaxf( 1.0f - cs, 0.0f ) );
-        // per (lambda, zeta): P = 2^(1-zeta) base^zeta and
-        // zeta lambda 2^(1-zeta) base^(zeta-1)
-        float Pw[AP_MAX], Dw[AP_MAX];
-#pragma unroll
-        for ( int pp = 0; pp < AP_MAX; ++pp )
+    // angular: each lane takes a contiguous block of the compacted (a-major)
+    // pair list, so the reaction on a stays in registers until a changes;
+    // the one on q goes to LDS.
+    int cur = -1;
+    float ax = 0.f, ay = 0.f, az = 0.f; // reaction on row atom cur
+    float4 A = make_float4( 0.f, 0.f, 0.f, 1.f ), Ea = make_float4( 1.f, 0.f, 0.f, 0.f );
+    for_triplets( L, n, rc2, tbuf[wv], [&]( int a, int q ) {
+        if ( a != cur )
         {
-            if ( pp < Q.nap )
+            if ( cur >= 0 )
             {
-                const float lb = Q.a_lam[pp] > 0.f ? lbp : lbm;
-                Pw[pp] = exp2f( Q.a_zeta[pp] * lb + Q.a_lpref[pp] );
-                // base^(zeta-1) = 1 for zeta = 1, also where base = 0
-                // (collinear neighbours; log2 base = -inf)
-                const float zm1 = Q.a_zeta[pp] - 1.f;
-                Dw[pp] = Q.a_zeta[pp] * Q.a_lam[pp] *
-                         exp2f( ( zm1 != 0.f ? zm1 * lb : 0.f ) + Q.a_lpref[pp] );
+                atomicAdd( &sF[wv][cur][0], ax );
+                atomicAdd( &sF[wv][cur][1], ay );
+                atomicAdd( &sF[wv][cur][2], az );
             }
-            else
-            {
-                Pw[pp] = 0.f;
-                Dw[pp] = 0.f;
-            }
+            cur = a;
+            ax = ay = az = 0.f;
+            A = L.d[a];
+            Ea = L.e[a];
         }
+        const float4 B = L.d[q], Eb = L.e[q];
+        const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
+        const float r1 = A.w, r2 = B.w;
+        const float r3 = __builtin_amdgcn_sqrtf( cx * cx + cy * cy + cz * cz );
+        const float ia = Ea.x, ib = Eb.x, ic = __builtin_amdgcn_rcpf( r3 );
+        const float f1 = Ea.y, f2 = Eb.y, f3 = fcut( r3, rc, kpi );
+        const float d1 = Ea.z, d2 = Eb.z, d3 = dfcut( r3, rc, kpi );
+        const float F = f1 * f2 * f3;
+        const float inv12 = ia * ib;
+        const float cs = ( A.x * B.x + A.y * B.y + A.z * B.z ) * inv12;
+        float Pw[AP_MAX], Dw[AP_MAX];
+        ang_powers<true>( K, cs, Pw, Dw );
         float Gs = 0.f, As = 0.f, B1 = 0.f, B2 = 0.f, B3 = 0.f;
 #pragma unroll
         for ( int g = 0; g < AG_MAX; ++g )
         {
-            if ( g < Q.nag )
+            if ( g < K.nag )
             {
-                const float rs = Q.a_rs[g];
+                const float rs = K.rs[g];
                 const float t1 = r1 - rs, t2 = r2 - rs, t3 = r3 - rs;
-                const float Eg = __expf( -Q.a_eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
+                const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
                 float sp = 0.f, sd = 0.f;
 #pragma unroll
                 for ( int q4 = 0; q4 < AP_MAX / 4; ++q4 )
@@ -899,7 +1033,7 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
                 const float w = sp * Eg; // sum_p c P E
                 Gs += w;
                 As = fmaf( sd, Eg, As );
-                const float m2 = -2.f * Q.a_eta[g] * w;
+                const float m2 = -2.f * K.eta[g] * w;
                 B1 = fmaf( m2, t1, B1 );
                 B2 = fmaf( m2, t2, B2 );
                 B3 = fmaf( m2, t3, B3 );
@@ -907,7 +1041,6 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
         }
         // gradients of sum_s c_s T_s with respect to A = x_a - x_i and
         // B = x_q - x_i (T = pref P E F, r3 = |B - A|)
-        const float ia = 1.f / r1, ib = 1.f / r2, ic = 1.f / r3;
         const float dFa1 = d1 * f2 * f3, dF3 = f1 * f2 * d3, dFb2 = f1 * d2 * f3;
         float ga[3], gb[3];
         const float Av[3] = { A.x, A.y, A.z }, Bv[3] = { B.x, B.y, B.z },
@@ -921,15 +1054,21 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
             ga[k] = F * ( As * dca + B1 * u1 - B3 * u3 ) + Gs * ( dFa1 * u1 - dF3 * u3 );
             gb[k] = F * ( As * dcb + B2 * u2 + B3 * u3 ) + Gs * ( dFb2 * u2 + dF3 * u3 );
         }
-        atomicAdd( &sF[wv][a][0], -ga[0] );
-        atomicAdd( &sF[wv][a][1], -ga[1] );
-        atomicAdd( &sF[wv][a][2], -ga[2] );
+        ax -= ga[0];
+        ay -= ga[1];
+        az -= ga[2];
         atomicAdd( &sF[wv][q][0], -gb[0] );
         atomicAdd( &sF[wv][q][1], -gb[1] );
         atomicAdd( &sF[wv][q][2], -gb[2] );
         fix += ga[0] + gb[0];
         fiy += ga[1] + gb[1];
         fiz += ga[2] + gb[2];
+    } );
+    if ( cur >= 0 )
+    {
+        atomicAdd( &sF[wv][cur][0], ax );
+        atomicAdd( &sF[wv][cur][1], ay );
+        atomicAdd( &sF[wv][cur][2], az );
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
 #pragma unroll
