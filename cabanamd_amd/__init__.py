@@ -926,3 +926,32 @@ class Server:
         return dict(atom_steps=o[0], busy_s=o[1], rounds=int(o[2]),
                     bytes=int(o[3]), running=int(o[4]), done=int(o[5]))
 
+
+def read_data(path, charge=False, ntypes_mass=8):
+    """A LAMMPS data file (read_data.h:100-385): dict with x, v, q, type
+    (0-based), id, lo, hi, ntypes and mass (NaN where the file has none)."""
+    n, nt = C.c_int(), C.c_int()
+    lo, hi = np.zeros(3), np.zeros(3)
+    call("cmd_data_read", str(path).encode(), int(bool(charge)), C.byref(n),
+         C.byref(nt), dp(lo), dp(hi), None, None, None, None, None, None)
+    k = n.value
+    x, v, q = np.zeros((k, 3)), np.zeros((k, 3)), np.zeros(k)
+    t, i = np.zeros(k, np.int32), np.zeros(k, np.int32)
+    mass = np.full(ntypes_mass, np.nan)
+    call("cmd_data_read", str(path).encode(), int(bool(charge)), C.byref(n),
+         C.byref(nt), dp(lo), dp(hi), dp(x), dp(v), dp(q), ip(t), ip(i),
+         dp(mass))
+    return dict(x=x, v=v, q=q, type=t, id=i, lo=lo, hi=hi, ntypes=nt.value,
+                mass=mass[:nt.value])
+
+
+def write_data(path, x, v, type, id, lo, hi, ntypes):
+    """write_data (read_data.h:387-422) at full double precision."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(type, dtype=np.int32)
+    i = np.ascontiguousarray(id, dtype=np.int32)
+    call("cmd_data_write", str(path).encode(), len(x), int(ntypes),
+         dp(np.asarray(lo, dtype=np.float64)),
+         dp(np.asarray(hi, dtype=np.float64)), dp(x), dp(v), ip(t), ip(i))
+

@@ -58,7 +58,9 @@ class Params(C.Structure):
                 ("coeff_eps", C.c_double * 16),
                 ("coeff_sigma", C.c_double * 16),
                 ("coeff_cut", C.c_double * 16),
-                ("force_type", C.c_int), ("nnp_dir", C.c_char * 256)]
+                ("force_type", C.c_int), ("nnp_dir", C.c_char * 256),
+                ("atom_style_charge", C.c_int), ("data_in", C.c_char * 256),
+                ("data_out", C.c_char * 256)]
 
 
 # name -> argtypes (restype is always c_int = cmd_status unless listed)
@@ -137,6 +139,10 @@ _PROTOS = {
     "cmd_params_parse_deck": ([C.c_char_p, C.c_int, C.POINTER(Params)], None),
     "cmd_lattice_create": ([C.POINTER(Params), C.POINTER(Domain), IP, DP, DP,
                             IP, IP], None),
+    "cmd_data_read": ([C.c_char_p, C.c_int, IP, IP, DP, DP, DP, DP, DP, IP,
+                       IP, DP], None),
+    "cmd_data_write": ([C.c_char_p, C.c_int, C.c_int, DP, DP, DP, DP, IP, IP],
+                       None),
     "cmd_md_create": ([C.POINTER(Params), C.POINTER(VP)], None),
     "cmd_md_destroy": ([VP], None),
     "cmd_md_set_precision": ([VP, C.c_int], None),

@@ -14,6 +14,7 @@
 #include <sstream>
 #include <string>
 #include <vector>
+#include <map>
 
 #include "../../include/cabanamd_amd.h"
 
@@ -297,6 +298,7 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
                 if ( !need( 2 ) || ( w[1] != "atomic" && w[1] != "charge" ) )
                     return bad( "'atom_style' command only supports 'atomic' and "
                                 "'charge' in CabanaMD" );
+                p->atom_style_charge = w[1] == "charge";
             }
             else if ( key == "lattice" )
             {
@@ -373,9 +375,15 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
                 p->mass[t] = std::stod( w[2] );
             }
             else if ( key == "read_data" || key == "write_data" )
-                return bad( "'" + key +
-                            "' (LAMMPS data files) is not supported by this "
-                            "engine yet" );
+            {
+                // inputFile_impl.h:315-326
+                if ( !need( 2 ) )
+                    return bad( "'" + key + "' needs a file name" );
+                char *dst = key == "read_data" ? p->data_in : p->data_out;
+                if ( w[1].size() >= sizeof( p->data_in ) )
+                    return bad( "data file name too long" );
+                std::snprintf( dst, sizeof( p->data_in ), "%s", w[1].c_str() );
+            }
             else if ( key == "dump" )
             {
                 // VTK output is outside this engine; the command is accepted
@@ -525,7 +533,8 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
     }
     if ( force_iteration >= 0 )
         p->half = force_iteration ? 1 : 0;
-    if ( !have_region )
+    // atoms come from a region lattice or from read_data
+    if ( !have_region && !p->data_in[0] )
         return deck_error( "the deck defines no region block" );
     return CMD_OK;
 }
@@ -629,6 +638,242 @@ cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
         v[3 * i + 0] *= s;
         v[3 * i + 1] *= s;
         v[3 * i + 2] *= s;
+    }
+    return CMD_OK;
+}
+
+} // extern "C"
+
+// ---- LAMMPS data files (read_data.h:100-422, restated)
+namespace
+{
+// Next line that is not blank after removing a # comment; false at EOF.
+bool data_line( std::ifstream &in, std::string &line )
+{
+    while ( std::getline( in, line ) )
+    {
+        line = line.substr( 0, line.find( '#' ) );
+        if ( line.find_first_not_of( " \r\t\n" ) != std::string::npos )
+            return true;
+    }
+    return false;
+}
+
+std::string trimmed( const std::string &s )
+{
+    const size_t a = s.find_first_not_of( " \r\t\n" );
+    if ( a == std::string::npos )
+        return "";
+    const size_t b = s.find_last_not_of( " \r\t\n" );
+    return s.substr( a, b - a + 1 );
+}
+
+struct DataFile
+{
+    int natoms = 0, ntypes = 0;
+    double lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+    std::vector<double> x, v, q, mass;
+    std::vector<int> type, id;
+    bool has_v = false;
+};
+
+cmd_status parse_data( const char *path, int charge, DataFile &D )
+{
+    std::ifstream in( path );
+    if ( !in.good() )
+        return deck_error( std::string( "could not read data file " ) + path );
+    std::string line;
+    if ( !std::getline( in, line ) ) // the first line is a title
+        return deck_error( "empty data file" );
+    // header: until "zlo zhi" (read_lammps_header)
+    bool box[3] = { false, false, false };
+    while ( !box[2] )
+    {
+        if ( !data_line( in, line ) )
+            return deck_error( "data file header ends before 'zlo zhi'" );
+        const char *c = line.c_str();
+        if ( line.find( "atom types" ) != std::string::npos )
+            std::sscanf( c, "%d", &D.ntypes );
+        else if ( line.find( "atoms" ) != std::string::npos )
+            std::sscanf( c, "%d", &D.natoms );
+        else if ( line.find( "xlo xhi" ) != std::string::npos )
+            box[0] = std::sscanf( c, "%lg %lg", &D.lo[0], &D.hi[0] ) == 2;
+        else if ( line.find( "ylo yhi" ) != std::string::npos )
+            box[1] = std::sscanf( c, "%lg %lg", &D.lo[1], &D.hi[1] ) == 2;
+        else if ( line.find( "zlo zhi" ) != std::string::npos )
+            box[2] = std::sscanf( c, "%lg %lg", &D.lo[2], &D.hi[2] ) == 2;
+    }
+    if ( !box[0] || !box[1] || D.natoms < 0 || D.ntypes < 1 || D.ntypes > CMD_MAX_TYPES )
+        return deck_error( "data file header needs atoms, atom types (1..8) and the box" );
+    D.mass.assign( D.ntypes, std::numeric_limits<double>::quiet_NaN() );
+    bool have_atoms = false;
+    while ( data_line( in, line ) )
+    {
+        const std::string key = trimmed( line );
+        if ( key == "Atoms" )
+        {
+            for ( int k = 0; k < D.natoms; ++k )
+            {
+                if ( !data_line( in, line ) )
+                    return deck_error( "data file ends inside Atoms" );
+                int idv = 0, t = 0;
+                double a = 0, b = 0, c = 0, qq = 0;
+                const int got = charge ? std::sscanf( line.c_str(), "%d %d %lg %lg %lg %lg",
+                                                      &idv, &t, &qq, &a, &b, &c )
+                                       : std::sscanf( line.c_str(), "%d %d %lg %lg %lg", &idv,
+                                                      &t, &a, &b, &c );
+                if ( got != ( charge ? 6 : 5 ) )
+                    return deck_error( "malformed Atoms line: " + line );
+                if ( t < 1 || t > D.ntypes )
+                    return deck_error( "atom type out of range: " + line );
+                // kept when inside [lo, hi) (one rank owns the whole box)
+                if ( a >= D.lo[0] && b >= D.lo[1] && c >= D.lo[2] && a < D.hi[0] &&
+                     b < D.hi[1] && c < D.hi[2] )
+                {
+                    D.id.push_back( idv );
+                    D.type.push_back( t - 1 );
+                    D.x.push_back( a );
+                    D.x.push_back( b );
+                    D.x.push_back( c );
+                    D.q.push_back( qq );
+                }
+            }
+            D.v.assign( D.x.size(), 0.0 );
+            have_atoms = true;
+        }
+        else if ( key == "Velocities" )
+        {
+            if ( !have_atoms )
+                return deck_error( "Must read Atoms before Velocities" );
+            std::map<int, size_t> where;
+            for ( size_t k = 0; k < D.id.size(); ++k )
+                where[D.id[k]] = k;
+            for ( int k = 0; k < D.natoms; ++k )
+            {
+                if ( !data_line( in, line ) )
+                    return deck_error( "data file ends inside Velocities" );
+                int idv = 0;
+                double a = 0, b = 0, c = 0;
+                if ( std::sscanf( line.c_str(), "%d %lg %lg %lg", &idv, &a, &b, &c ) != 4 )
+                    return deck_error( "malformed Velocities line: " + line );
+                const auto it = where.find( idv );
+                if ( it != where.end() )
+                {
+                    D.v[3 * it->second] = a;
+                    D.v[3 * it->second + 1] = b;
+                    D.v[3 * it->second + 2] = c;
+                }
+            }
+            D.has_v = true;
+        }
+        else if ( key == "Masses" )
+        {
+            for ( int k = 0; k < D.ntypes; ++k )
+            {
+                if ( !data_line( in, line ) )
+                    return deck_error( "data file ends inside Masses" );
+                int t = 0;
+                double m = 0;
+                if ( std::sscanf( line.c_str(), "%d %lg", &t, &m ) != 2 || t < 1 ||
+                     t > D.ntypes )
+                    return deck_error( "malformed Masses line: " + line );
+                D.mass[t - 1] = m;
+            }
+        }
+        else if ( key == "Pair Coeffs" )
+        {
+            // ignored: pair_coeff comes from the input deck
+            for ( int k = 0; k < D.ntypes; ++k )
+                if ( !data_line( in, line ) )
+                    return deck_error( "data file ends inside Pair Coeffs" );
+        }
+        else
+            return deck_error( "Unknown identifier in data file: " + key );
+    }
+    if ( !have_atoms )
+        return deck_error( "data file has no Atoms section" );
+    if ( (int)D.id.size() != D.natoms )
+        return deck_error( "Created incorrect # of atoms." );
+    return CMD_OK;
+}
+} // namespace
+
+extern "C" {
+
+cmd_status cmd_data_read( const char *path, int charge_style, int *n, int *ntypes,
+                          double lo[3], double hi[3], double *x, double *v, double *q,
+                          int *type, int *id, double *mass )
+{
+    if ( !path || !n )
+    {
+        cmd::set_error( "path and n are required" );
+        return CMD_ERR_ARG;
+    }
+    DataFile D;
+    const cmd_status st = parse_data( path, charge_style, D );
+    if ( st != CMD_OK )
+        return st;
+    *n = (int)D.id.size();
+    if ( ntypes )
+        *ntypes = D.ntypes;
+    for ( int k = 0; k < 3; ++k )
+    {
+        if ( lo )
+            lo[k] = D.lo[k];
+        if ( hi )
+            hi[k] = D.hi[k];
+    }
+    if ( !x )
+        return CMD_OK;
+    std::copy( D.x.begin(), D.x.end(), x );
+    if ( v )
+        std::copy( D.v.begin(), D.v.end(), v );
+    if ( q )
+        std::copy( D.q.begin(), D.q.end(), q );
+    if ( type )
+        std::copy( D.type.begin(), D.type.end(), type );
+    if ( id )
+        std::copy( D.id.begin(), D.id.end(), id );
+    if ( mass )
+        for ( int t = 0; t < D.ntypes; ++t )
+            if ( D.mass[t] == D.mass[t] )
+                mass[t] = D.mass[t];
+    return CMD_OK;
+}
+
+cmd_status cmd_data_write( const char *path, int n, int ntypes, const double lo[3],
+                           const double hi[3], const double *x, const double *v,
+                           const int *type, const int *id )
+{
+    if ( !path || n < 0 || !lo || !hi || ( n > 0 && ( !x || !type || !id ) ) )
+    {
+        cmd::set_error( "path, box, and x, type, id for n > 0 are required" );
+        return CMD_ERR_ARG;
+    }
+    std::FILE *f = std::fopen( path, "w" );
+    if ( !f )
+    {
+        cmd::set_error( "cannot write %s", path );
+        return CMD_ERR_IO;
+    }
+    std::fprintf( f, "LAMMPS data file from CabanaMD\n\n%d atoms\n%d atom types\n\n", n,
+                  ntypes );
+    const char *ax[3] = { "x", "y", "z" };
+    for ( int k = 0; k < 3; ++k )
+        std::fprintf( f, "%.17g %.17g %slo %shi\n", lo[k], hi[k], ax[k], ax[k] );
+    std::fprintf( f, "\nAtoms # atomic\n\n" );
+    for ( int i = 0; i < n; ++i )
+        std::fprintf( f, "%d %d %.17g %.17g %.17g\n", id[i], type[i] + 1, x[3 * i],
+                      x[3 * i + 1], x[3 * i + 2] );
+    std::fprintf( f, "\nVelocities\n\n" );
+    for ( int i = 0; i < n; ++i )
+        std::fprintf( f, "%d %.17g %.17g %.17g\n", id[i], v ? v[3 * i] : 0.0,
+                      v ? v[3 * i + 1] : 0.0, v ? v[3 * i + 2] : 0.0 );
+    const bool ok = std::fclose( f ) == 0;
+    if ( !ok )
+    {
+        cmd::set_error( "write to %s failed", path );
+        return CMD_ERR_IO;
     }
     return CMD_OK;
 }

@@ -431,6 +431,31 @@ cmd_status cmd_md_init( cmd_md *md )
         low[k] = p.lattice_constant * p.region_lo[k];
         high[k] = p.lattice_constant * p.region_hi[k];
     }
+    if ( !md->custom_state && p.data_in[0] )
+    {
+        // read_lammps_data_file (read_data.h:310-385) when the deck has
+        // read_data: atoms, velocities, masses and box from the file
+        int n = 0, nt = 0;
+        double lo[3], hi[3];
+        CMD_TRY( cmd_data_read( p.data_in, p.atom_style_charge, &n, &nt, lo, hi, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, nullptr ) );
+        CMD_REQUIRE( nt <= p.ntypes || p.ntypes == 1, CMD_ERR_ARG,
+                     "data file has %d atom types", nt );
+        md->p.ntypes = std::max( p.ntypes, nt );
+        md->init_x.resize( 3 * (size_t)n );
+        md->init_v.resize( 3 * (size_t)n );
+        md->init_type.resize( n );
+        md->init_id.resize( n );
+        CMD_TRY( cmd_data_read( p.data_in, p.atom_style_charge, &n, &nt, lo, hi,
+                                md->init_x.data(), md->init_v.data(), nullptr,
+                                md->init_type.data(), md->init_id.data(), md->p.mass ) );
+        for ( int k = 0; k < 3; ++k )
+        {
+            md->init_lo[k] = lo[k];
+            md->init_hi[k] = hi[k];
+        }
+        md->custom_state = true;
+    }
     if ( md->custom_state )
         for ( int k = 0; k < 3; ++k )
         {
@@ -514,13 +539,46 @@ cmd_status cmd_md_init( cmd_md *md )
     return CMD_OK;
 }
 
+// write_data at the end of CbnMD::run (cabanamd_impl.h:430-431): the owned
+// atoms in atom-id order.
+static cmd_status write_data_file( cmd_md *md )
+{
+    cmd_system *S = md->S;
+    const int n = S->n_local + S->n_ghost;
+    std::vector<double> x( 3 * (size_t)n ), v( 3 * (size_t)n );
+    std::vector<int> type( n ), id( n );
+    CMD_TRY( cmd_system_download( S, x.data(), v.data(), nullptr, type.data(), id.data() ) );
+    const int nl = S->n_local;
+    std::vector<int> o( nl );
+    for ( int k = 0; k < nl; ++k )
+        o[k] = k;
+    std::sort( o.begin(), o.end(), [&]( int a, int b ) { return id[a] < id[b]; } );
+    std::vector<double> xs( 3 * (size_t)nl ), vs( 3 * (size_t)nl );
+    std::vector<int> ts( nl ), is( nl );
+    for ( int k = 0; k < nl; ++k )
+    {
+        for ( int c = 0; c < 3; ++c )
+        {
+            xs[3 * k + c] = x[3 * (size_t)o[k] + c];
+            vs[3 * k + c] = v[3 * (size_t)o[k] + c];
+        }
+        ts[k] = type[o[k]];
+        is[k] = id[o[k]];
+    }
+    return cmd_data_write( md->p.data_out, nl, md->p.ntypes, md->d.low, md->d.high,
+                           xs.data(), vs.data(), ts.data(), is.data() );
+}
+
 cmd_status cmd_md_run( cmd_md *md, int nsteps )
 {
     CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
     CMD_REQUIRE( md->inited, CMD_ERR_STATE, "run before init" );
     if ( nsteps <= 0 )
         return CMD_OK;
-    return run_steps( md, nsteps );
+    CMD_TRY( run_steps( md, nsteps ) );
+    if ( md->p.data_out[0] )
+        CMD_TRY( write_data_file( md ) );
+    return CMD_OK;
 }
 
 cmd_status cmd_md_system( cmd_md *md, cmd_system **s )

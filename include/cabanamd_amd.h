@@ -282,6 +282,9 @@ typedef struct cmd_params
     double coeff_eps[16], coeff_sigma[16], coeff_cut[16];
     int force_type; /* 0 lj/cut, 2 nnp (types.h:80-85) */
     char nnp_dir[256];
+    int atom_style_charge; /* atom_style charge (else atomic) */
+    char data_in[256];     /* read_data file ("" = lattice) */
+    char data_out[256];    /* write_data file at the end of each run */
 } cmd_params;
 
 /* Parse a LAMMPS-subset deck (inputFile_impl.h:141-533).  Unknown or
@@ -295,6 +298,25 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
 cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d,
                                int *n, double *x, double *v, int *type,
                                int *id );
+
+/* LAMMPS data files (read_data.h:100-385, restated).  Header keywords
+ * "atoms", "atom types", "xlo xhi", "ylo yhi", "zlo zhi"; sections Atoms
+ * (atomic: id type x y z; charge: id type q x y z), Velocities (id vx vy vz),
+ * Masses (type mass) and Pair Coeffs (skipped, as the reference does).
+ * Types in the file are 1-based and returned 0-based.  Atoms outside
+ * [lo, hi) are dropped like the reference's single-rank read, which then
+ * reports a wrong atom count as an error.  Two-call protocol: with x == NULL
+ * only n, ntypes, lo and hi are returned.  mass (ntypes entries, may be
+ * NULL) gets the Masses section or is left unchanged; q may be NULL. */
+cmd_status cmd_data_read( const char *path, int charge_style, int *n, int *ntypes,
+                          double lo[3], double hi[3], double *x, double *v,
+                          double *q, int *type, int *id, double *mass );
+/* write_data (read_data.h:387-422): header, "Atoms # atomic" and
+ * "Velocities", with every value at full double precision (the reference
+ * prints 6 significant digits). */
+cmd_status cmd_data_write( const char *path, int n, int ntypes, const double lo[3],
+                           const double hi[3], const double *x, const double *v,
+                           const int *type, const int *id );
 
 /* ---- timestep loop ---------------------------------------------------------
  * CbnMD<System,Neighbor>::init / run (cabanamd_impl.h:70-432).  run() fuses

@@ -369,6 +369,119 @@ class CbnMD
     Handle<cmd_md, cmd_md_destroy> h_;
 };
 
+// The model pair_style nnp loads through n2p2 (force_nnp_cabana_neigh_impl.h
+// :41-66): symmetry functions, scaling and per-element networks.
+class NNPModel
+{
+  public:
+    // n2p2 directory (input.nn, scaling.data, weights.NNN.data) with the
+    // element names and atomic numbers of the system's types
+    static NNPModel load_n2p2( const char *dir, const std::vector<const char *> &names,
+                               const std::vector<int> &znum )
+    {
+        cmd_nnp_model *m = nullptr;
+        check( cmd_nnp_model_load_n2p2( dir, (int)names.size(), names.data(), znum.data(),
+                                        &m ) );
+        return NNPModel( m );
+    }
+    cmd_nnp_model *handle() const { return h_.get(); }
+
+  private:
+    explicit NNPModel( cmd_nnp_model *m )
+        : h_( m )
+    {
+    }
+    Handle<cmd_nnp_model, cmd_nnp_model_destroy> h_;
+};
+
+// ForceNNP (force_nnp_cabana_neigh_impl.h:41-130).  compute() writes local
+// and ghost forces; fold them with Comm::update_force as CbnMD does.
+class ForceNNP
+{
+  public:
+    explicit ForceNNP( const NNPModel &m )
+    {
+        cmd_force *f = nullptr;
+        check( cmd_force_nnp_create( m.handle(), &f ) );
+        h_ = Handle<cmd_force, cmd_force_destroy>( f );
+    }
+    void set_precision( int prec ) { check( cmd_force_set_precision( h_.get(), prec ) ); }
+    void compute( System *s, NeighborVerlet *n )
+    {
+        check( cmd_force_compute( h_.get(), s->handle(), n->handle() ) );
+    }
+    double compute_energy( System *s, NeighborVerlet *n )
+    {
+        double pe = 0.0;
+        check( cmd_force_energy( h_.get(), s->handle(), n->handle(), &pe ) );
+        return pe;
+    }
+    const char *name() const { return "Force:NNPCabanaVerletHIP"; }
+    cmd_force *handle() const { return h_.get(); }
+
+  private:
+    Handle<cmd_force, cmd_force_destroy> h_;
+};
+
+// Many independent systems stepped together (no reference counterpart).
+class Batch
+{
+  public:
+    explicit Batch( const cmd_params &p )
+    {
+        cmd_batch *b = nullptr;
+        check( cmd_batch_create( &p, &b ) );
+        h_ = Handle<cmd_batch, cmd_batch_destroy>( b );
+    }
+    void set_precision( int prec ) { check( cmd_batch_set_precision( h_.get(), prec ) ); }
+    void set_lanes( int lanes ) { check( cmd_batch_set_lanes( h_.get(), lanes ) ); }
+    void set_nnp_model( const NNPModel &m )
+    {
+        check( cmd_batch_set_nnp_model( h_.get(), m.handle() ) );
+    }
+    int add( int n, const double *x, const double *v, const int *type, const int *id,
+             const double lo[3], const double hi[3] )
+    {
+        int h = 0;
+        check( cmd_batch_add( h_.get(), n, x, v, type, id, lo, hi, &h ) );
+        return h;
+    }
+    void remove( int handle ) { check( cmd_batch_remove( h_.get(), handle ) ); }
+    void run( int nsteps ) { check( cmd_batch_run( h_.get(), nsteps ) ); }
+    cmd_batch *handle() const { return h_.get(); }
+
+  private:
+    Handle<cmd_batch, cmd_batch_destroy> h_;
+};
+
+// MD request server with continuous batching (no reference counterpart).
+class Server
+{
+  public:
+    Server( const cmd_params &lj, const cmd_params *nnp, const NNPModel *model,
+            int64_t hbm_budget, int max_jobs )
+    {
+        cmd_server *s = nullptr;
+        check( cmd_server_create( &lj, nnp, model ? model->handle() : nullptr, hbm_budget,
+                                  max_jobs, &s ) );
+        h_ = Handle<cmd_server, cmd_server_destroy>( s );
+    }
+    int submit( int kind, int n, const double *x, const double *v, const int *type,
+                const int *id, const double lo[3], const double hi[3], int nsteps,
+                double arrival )
+    {
+        int j = 0;
+        check( cmd_server_submit( h_.get(), kind, n, x, v, type, id, lo, hi, nsteps, arrival,
+                                  &j ) );
+        return j;
+    }
+    void drain( double timeout_s ) { check( cmd_server_drain( h_.get(), timeout_s ) ); }
+    cmd_server *handle() const { return h_.get(); }
+
+  private:
+    Handle<cmd_server, cmd_server_destroy> h_;
+};
+
 } // namespace cabanamd_amd
 
 #endif
