@@ -862,9 +862,19 @@ cmd_status cmd_data_write( const char *path, int n, int ntypes, const double lo[
     for ( int k = 0; k < 3; ++k )
         std::fprintf( f, "%.17g %.17g %slo %shi\n", lo[k], hi[k], ax[k], ax[k] );
     std::fprintf( f, "\nAtoms # atomic\n\n" );
+    // positions wrapped into [lo, hi) so that read_data keeps every atom
+    // (between rebuilds atoms may sit just outside the box)
+    auto wrap = [&]( double c, int k )
+    {
+        const double L = hi[k] - lo[k];
+        double w = c - L * std::floor( ( c - lo[k] ) / L );
+        if ( !( w < hi[k] ) || w < lo[k] )
+            w = lo[k];
+        return w;
+    };
     for ( int i = 0; i < n; ++i )
-        std::fprintf( f, "%d %d %.17g %.17g %.17g\n", id[i], type[i] + 1, x[3 * i],
-                      x[3 * i + 1], x[3 * i + 2] );
+        std::fprintf( f, "%d %d %.17g %.17g %.17g\n", id[i], type[i] + 1,
+                      wrap( x[3 * i], 0 ), wrap( x[3 * i + 1], 1 ), wrap( x[3 * i + 2], 2 ) );
     std::fprintf( f, "\nVelocities\n\n" );
     for ( int i = 0; i < n; ++i )
         std::fprintf( f, "%d %.17g %.17g %.17g\n", id[i], v ? v[3 * i] : 0.0,

@@ -313,7 +313,8 @@ cmd_status cmd_data_read( const char *path, int charge_style, int *n, int *ntype
                           double *q, int *type, int *id, double *mass );
 /* write_data (read_data.h:387-422): header, "Atoms # atomic" and
  * "Velocities", with every value at full double precision (the reference
- * prints 6 significant digits). */
+ * prints 6 significant digits) and positions wrapped into [lo, hi), so the
+ * file reads back whole at any step (the reference writes them unwrapped). */
 cmd_status cmd_data_write( const char *path, int n, int ntypes, const double lo[3],
                            const double hi[3], const double *x, const double *v,
                            const int *type, const int *id );

@@ -91,6 +91,14 @@ def test_write_then_read_is_exact(tmp_path):
     assert "Atoms # atomic" in text and "\nVelocities\n" in text
 
 
+def test_write_data_wraps_positions(tmp_path):
+    x = np.array([[-0.25, 5.0, 10.0], [10.5, -1e-18, 3.0]])
+    p = os.path.join(tmp_path, "w.data")
+    M.write_data(p, x, np.zeros_like(x), [0, 0], [1, 2], [0, 0, 0], [10, 10, 10], 1)
+    d = M.read_data(p)
+    assert np.allclose(d["x"], [[9.75, 5.0, 0.0], [0.5, 0.0, 3.0]], atol=1e-15)
+
+
 def test_deck_read_and_write_data_keywords(tmp_path):
     deck = _write(tmp_path, "units lj\natom_style atomic\nread_data in.data\n"
                   "write_data out.data\npair_style lj/cut 2.5\n"
