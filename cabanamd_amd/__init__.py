@@ -674,6 +674,18 @@ class CbnMD:
         call("cmd_md_run", self._h,
              int(self.params.nsteps if nsteps is None else nsteps))
 
+    def dump(self, path, step):
+        """dump_binary (cabanamd_impl.h:434-481): PATH/output.<step>.000."""
+        call("cmd_md_dump", self._h, str(path).encode(), int(step))
+
+    def check_correctness(self, ref_path, step, file=None):
+        """check_correctness (cabanamd_impl.h:485-642) against the snapshot
+        of `step`: |dx|, max dx, |dv|, max dv, |df|, max df."""
+        out = np.zeros(6)
+        call("cmd_md_check_correctness", self._h, str(ref_path).encode(),
+             int(step), None if file is None else str(file).encode(), dp(out))
+        return out
+
     def compute_forces(self):
         """Force::compute on the current positions with the current list
         (and the ghost-force fold for a half list), as at a rebuild-free

@@ -158,6 +158,9 @@ _PROTOS = {
     "cmd_md_timers": ([VP, DP], None),
     "cmd_md_profile": ([VP, C.c_int], None),
     "cmd_md_kernel_stats": ([VP, IP, DP], None),
+    "cmd_md_dump": ([VP, C.c_char_p, C.c_int], None),
+    "cmd_md_check_correctness": ([VP, C.c_char_p, C.c_int, C.c_char_p, DP],
+                                 None),
     "cmd_batch_create": ([C.POINTER(Params), C.POINTER(VP)], None),
     "cmd_batch_destroy": ([VP], None),
     "cmd_batch_set_precision": ([VP, C.c_int], None),

@@ -25,6 +25,8 @@ struct Options
     int force_iteration = -1; // deck decides (newton on/off)
     int precision = CMD_PREC_FP64;
     int lanes = 0; // 0: autotune
+    int dump_rate = 0, check_rate = 0; // --dumpbinary / --correctness
+    std::string dump_path, ref_path, check_file;
 };
 
 void usage( std::ostream &o )
@@ -38,7 +40,10 @@ void usage( std::ostream &o )
          "  --neigh-parallel SERIAL|TEAM|TEAM_VECTOR (accepted; wavefront mapping)\n"
          "  --device-type HIP\n"
          "  --precision FP64|MIXED     pair arithmetic (default FP64)\n"
-         "  --lanes 0|1|2|4|8          lanes per atom, 0 = autotune\n";
+         "  --lanes 0|1|2|4|8          lanes per atom, 0 = autotune\n"
+         "  --dumpbinary N PATH        snapshot every N steps to PATH/output.*\n"
+         "  --correctness N PATH FILE  compare with PATH/output.* every N steps,\n"
+         "                             differences appended to FILE\n";
 }
 
 [[noreturn]] void fail( const std::string &msg )
@@ -107,6 +112,21 @@ Options parse( int argc, char **argv )
                 o.precision = CMD_PREC_MIXED;
             else
                 fail( "Unknown commandline option: " + a + " " + v );
+        }
+        else if ( a == "--dumpbinary" )
+        {
+            o.dump_rate = std::stoi( value( i ) );
+            o.dump_path = value( i );
+            if ( o.dump_rate <= 0 )
+                fail( "--dumpbinary needs a positive rate" );
+        }
+        else if ( a == "--correctness" )
+        {
+            o.check_rate = std::stoi( value( i ) );
+            o.ref_path = value( i );
+            o.check_file = value( i );
+            if ( o.check_rate <= 0 )
+                fail( "--correctness needs a positive rate" );
         }
         else if ( a == "--lanes" )
             o.lanes = std::stoi( value( i ) );
@@ -181,15 +201,34 @@ int main( int argc, char **argv )
             out << 0 << "\t" << fmt( "%.6f", rows[0][1] ) << "\t"
                 << fmt( "%.6f", rows[0][2] ) << "\t" << fmt( "%.6f", rows[0][3] )
                 << "\t" << fmt( "%.2f", 0.0 ) << "\t" << fmt( "%e", 0.0 ) << "\n";
+        // dump_binary / check_correctness at step 0 and every rate steps
+        // (cabanamd_impl.h:245-251, 392-396)
+        auto at_step = [&]( int step )
+        {
+            if ( o.dump_rate > 0 && step % o.dump_rate == 0 )
+                check( cmd_md_dump( md.handle(), o.dump_path.c_str(), step ) );
+            if ( o.check_rate > 0 && step % o.check_rate == 0 )
+            {
+                double d[6];
+                check( cmd_md_check_correctness( md.handle(), o.ref_path.c_str(), step,
+                                                 o.check_file.c_str(), d ) );
+            }
+        };
+        at_step( 0 );
         using clock = std::chrono::steady_clock;
         const auto t0 = clock::now();
         double last = 0.0;
         int done = 0;
         while ( done < p.nsteps )
         {
-            const int chunk = std::min( thermo, p.nsteps - done );
+            int chunk = std::min( thermo - done % thermo, p.nsteps - done );
+            if ( o.dump_rate > 0 )
+                chunk = std::min( chunk, o.dump_rate - done % o.dump_rate );
+            if ( o.check_rate > 0 )
+                chunk = std::min( chunk, o.check_rate - done % o.check_rate );
             md.run( chunk );
             done += chunk;
+            at_step( done );
             const double now =
                 std::chrono::duration<double>( clock::now() - t0 ).count();
             rows = md.thermo();

@@ -13,6 +13,8 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <string>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -567,6 +569,166 @@ static cmd_status write_data_file( cmd_md *md )
     }
     return cmd_data_write( md->p.data_out, nl, md->p.ntypes, md->d.low, md->d.high,
                            xs.data(), vs.data(), ts.data(), is.data() );
+}
+
+// Owned atoms with x, v, f in atom-id order (host copies).
+struct Snapshot
+{
+    std::vector<int> id, type;
+    std::vector<double> x, v, f;
+};
+
+static cmd_status snapshot( cmd_md *md, Snapshot &out )
+{
+    cmd_system *S = md->S;
+    const int n = S->n_local + S->n_ghost;
+    std::vector<double> x( 3 * (size_t)n ), v( 3 * (size_t)n ), f( 3 * (size_t)n );
+    std::vector<int> type( n ), id( n );
+    CMD_TRY( cmd_system_download( S, x.data(), v.data(), f.data(), type.data(), id.data() ) );
+    const int nl = S->n_local;
+    std::vector<int> o( nl );
+    for ( int k = 0; k < nl; ++k )
+        o[k] = k;
+    std::sort( o.begin(), o.end(), [&]( int a, int b ) { return id[a] < id[b]; } );
+    out.id.resize( nl );
+    out.type.resize( nl );
+    out.x.resize( 3 * (size_t)nl );
+    out.v.resize( 3 * (size_t)nl );
+    out.f.resize( 3 * (size_t)nl );
+    for ( int k = 0; k < nl; ++k )
+    {
+        out.id[k] = id[o[k]];
+        out.type[k] = type[o[k]];
+        for ( int c = 0; c < 3; ++c )
+        {
+            out.x[3 * k + c] = x[3 * (size_t)o[k] + c];
+            out.v[3 * k + c] = v[3 * (size_t)o[k] + c];
+            out.f[3 * k + c] = f[3 * (size_t)o[k] + c];
+        }
+    }
+    return CMD_OK;
+}
+
+static std::string dump_name( const char *dir, int step )
+{
+    char b[64];
+    std::snprintf( b, sizeof( b ), "/output.%010d.%03d", step, 0 );
+    return std::string( dir ) + b;
+}
+
+extern "C" cmd_status cmd_md_dump( cmd_md *md, const char *dir, int step )
+{
+    CMD_REQUIRE( md && dir, CMD_ERR_ARG, "md and dir are required" );
+    CMD_REQUIRE( md->inited, CMD_ERR_STATE, "dump before init" );
+    Snapshot s;
+    CMD_TRY( snapshot( md, s ) );
+    const std::string name = dump_name( dir, step );
+    std::FILE *fp = std::fopen( name.c_str(), "wb" );
+    CMD_REQUIRE( fp, CMD_ERR_IO, "Cannot open dump file: %s", name.c_str() );
+    const int n = (int)s.id.size();
+    bool ok = std::fwrite( "CMDDUMP1", 1, 8, fp ) == 8;
+    ok = ok && std::fwrite( &n, sizeof( int ), 1, fp ) == 1;
+    ok = ok && std::fwrite( &step, sizeof( int ), 1, fp ) == 1;
+    for ( int i = 0; ok && i < n; ++i )
+    {
+        const double q = 0.0;
+        ok = std::fwrite( &s.id[i], sizeof( int ), 1, fp ) == 1 &&
+             std::fwrite( &s.type[i], sizeof( int ), 1, fp ) == 1 &&
+             std::fwrite( &q, sizeof( double ), 1, fp ) == 1 &&
+             std::fwrite( &s.x[3 * i], sizeof( double ), 3, fp ) == 3 &&
+             std::fwrite( &s.v[3 * i], sizeof( double ), 3, fp ) == 3 &&
+             std::fwrite( &s.f[3 * i], sizeof( double ), 3, fp ) == 3;
+    }
+    ok = ( std::fclose( fp ) == 0 ) && ok;
+    CMD_REQUIRE( ok, CMD_ERR_IO, "write to %s failed", name.c_str() );
+    return CMD_OK;
+}
+
+extern "C" cmd_status cmd_md_check_correctness( cmd_md *md, const char *refdir, int step,
+                                                const char *file, double out[6] )
+{
+    CMD_REQUIRE( md && refdir, CMD_ERR_ARG, "md and refdir are required" );
+    CMD_REQUIRE( md->inited, CMD_ERR_STATE, "check before init" );
+    Snapshot s;
+    CMD_TRY( snapshot( md, s ) );
+    const std::string name = dump_name( refdir, step );
+    std::FILE *fp = std::fopen( name.c_str(), "rb" );
+    CMD_REQUIRE( fp, CMD_ERR_IO, "Cannot open input file: %s", name.c_str() );
+    char magic[8] = {};
+    int n = 0, st = 0;
+    bool ok = std::fread( magic, 1, 8, fp ) == 8 && std::memcmp( magic, "CMDDUMP1", 8 ) == 0 &&
+              std::fread( &n, sizeof( int ), 1, fp ) == 1 &&
+              std::fread( &st, sizeof( int ), 1, fp ) == 1;
+    if ( !ok )
+    {
+        std::fclose( fp );
+        set_error( "%s is not a snapshot file", name.c_str() );
+        return CMD_ERR_IO;
+    }
+    if ( n != (int)s.id.size() )
+    {
+        std::fclose( fp );
+        set_error( "Mismatch in current and reference atom counts" );
+        return CMD_ERR_STATE;
+    }
+    std::map<int, int> where;
+    for ( int k = 0; k < n; ++k )
+        where[s.id[k]] = k;
+    double sr = 0, sv = 0, sf = 0, mr = 0, mv = 0, mf = 0;
+    for ( int i = 0; i < n; ++i )
+    {
+        int rid = 0, rt = 0;
+        double q = 0, rx[3], rv[3], rf[3];
+        ok = std::fread( &rid, sizeof( int ), 1, fp ) == 1 &&
+             std::fread( &rt, sizeof( int ), 1, fp ) == 1 &&
+             std::fread( &q, sizeof( double ), 1, fp ) == 1 &&
+             std::fread( rx, sizeof( double ), 3, fp ) == 3 &&
+             std::fread( rv, sizeof( double ), 3, fp ) == 3 &&
+             std::fread( rf, sizeof( double ), 3, fp ) == 3;
+        if ( !ok )
+            break;
+        const auto it = where.find( rid );
+        if ( it == where.end() )
+        {
+            std::fclose( fp );
+            set_error( "Unable to find current id matching reference id: %d", rid );
+            return CMD_ERR_STATE;
+        }
+        const int k = it->second;
+        double dr = 0, dv = 0, df = 0;
+        for ( int c = 0; c < 3; ++c )
+        {
+            const double a = s.x[3 * k + c] - rx[c], b = s.v[3 * k + c] - rv[c],
+                         d = s.f[3 * k + c] - rf[c];
+            dr += a * a;
+            dv += b * b;
+            df += d * d;
+            mr = std::max( mr, std::fabs( a ) );
+            mv = std::max( mv, std::fabs( b ) );
+            mf = std::max( mf, std::fabs( d ) );
+        }
+        sr += dr;
+        sv += dv;
+        sf += df;
+    }
+    std::fclose( fp );
+    CMD_REQUIRE( ok, CMD_ERR_IO, "Error reading reference data." );
+    const double res[6] = { std::sqrt( sr ), mr, std::sqrt( sv ), mv, std::sqrt( sf ), mf };
+    if ( out )
+        for ( int k = 0; k < 6; ++k )
+            out[k] = res[k];
+    if ( file )
+    {
+        std::FILE *fo = std::fopen( file, step == 0 ? "w" : "a" );
+        CMD_REQUIRE( fo, CMD_ERR_IO, "cannot write %s", file );
+        if ( step == 0 )
+            std::fprintf( fo, "# timestep deltarnorm maxdelr deltavnorm maxdelv "
+                              "deltafnorm maxdelf\n" );
+        std::fprintf( fo, "%d %g %g %g %g %g %g\n", step, res[0], res[1], res[2], res[3],
+                      res[4], res[5] );
+        std::fclose( fo );
+    }
+    return CMD_OK;
 }
 
 cmd_status cmd_md_run( cmd_md *md, int nsteps )

@@ -347,6 +347,19 @@ cmd_status cmd_md_force( cmd_md *md, cmd_force **f );
  * *rows_out and copies min(max_rows, count) rows (5 doubles each). */
 cmd_status cmd_md_thermo( const cmd_md *md, int max_rows, double *rows,
                           int *rows_out );
+/* Binary snapshot of the owned atoms (CbnMD::dump_binary,
+ * cabanamd_impl.h:434-481) in a fixed format: file
+ * <dir>/output.<step as %010d>.000 holding the 8 bytes "CMDDUMP1", int32 n,
+ * int32 step, then per atom in atom-id order: int32 id, int32 type, and
+ * doubles q (0), x[3], v[3], f[3].  f is the last computed force. */
+cmd_status cmd_md_dump( cmd_md *md, const char *dir, int step );
+/* CbnMD::check_correctness (cabanamd_impl.h:485-642): compare the current
+ * owned atoms with the snapshot of `step` in refdir, matched by atom id.
+ * out[6] = sqrt(sum |dx|^2), max |dx_k|, the same for v and f.  When file is
+ * not NULL the reference's line "step dr maxdr dv maxdv df maxdf" is
+ * appended (the file is started with its header at step 0). */
+cmd_status cmd_md_check_correctness( cmd_md *md, const char *refdir, int step,
+                                     const char *file, double out[6] );
 /* Phase times in seconds (CbnMD::run timers, cabanamd_impl.h:273-282):
  * force, neigh, comm, integrate, other, total. */
 cmd_status cmd_md_timers( const cmd_md *md, double out[6] );

@@ -44,3 +44,62 @@ def test_read_data_run_matches_lattice_run(gpu, tmp_path):
     assert np.all((d["x"] >= 0) & (d["x"] < L))
     dx = d["x"] - xb
     assert np.abs(dx - L * np.round(dx / L)).max() < 1e-12
+
+
+def test_dump_and_correctness(gpu, tmp_path):
+    """Snapshots of one run checked against a second identical run (all
+    differences zero) and against a mixed-precision run (small)."""
+    M = gpu
+    ref = os.path.join(tmp_path, "ref")
+    os.makedirs(ref)
+    p = M.lattice_params(ncells=(5, 5, 5), thermo=10, nsteps=20)
+    a = M.CbnMD(p, precision=M.PREC_FP64, lanes=1)
+    a.init()
+    a.dump(ref, 0)
+    a.run(10)
+    a.dump(ref, 10)
+    assert os.path.exists(os.path.join(ref, "output.0000000010.000"))
+    log = os.path.join(tmp_path, "correctness.dat")
+    b = M.CbnMD(p, precision=M.PREC_FP64, lanes=1)
+    b.init()
+    assert np.array_equal(b.check_correctness(ref, 0, log), np.zeros(6))
+    b.run(10)
+    assert np.array_equal(b.check_correctness(ref, 10, log), np.zeros(6))
+    lines = open(log).read().splitlines()
+    assert lines[0].startswith("# timestep deltarnorm")
+    assert [ln.split()[0] for ln in lines[1:]] == ["0", "10"]
+    c = M.CbnMD(p, precision=M.PREC_MIXED, lanes=2)
+    c.init()
+    c.run(10)
+    d = c.check_correctness(ref, 10)
+    assert 0 < d[1] < 1e-6 and d[5] < 1e-3
+    with pytest.raises(M.CmdError):
+        c.check_correctness(ref, 20)  # no such snapshot
+
+
+def test_cli_dumpbinary_and_correctness(gpu, tmp_path):
+    import subprocess
+    import sys
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "cabanamd_amd", "cabanamd")
+    deck = os.path.join(tmp_path, "in.lj")
+    with open(deck, "w") as fh:
+        fh.write("units lj\natom_style atomic\nlattice fcc 0.8442\n"
+                 "region box block 0 5 0 5 0 5\ncreate_box 1 box\n"
+                 "create_atoms 1 box\nmass 1 1.0\nvelocity all create 1.44 87287 "
+                 "loop geom\npair_style lj/cut 2.5\npair_coeff 1 1 1.0 1.0 2.5\n"
+                 "neighbor 0.3 bin\nneigh_modify every 20 one 50\n"
+                 "fix 1 all nve\nthermo 10\nrun 20\n")
+    ref = os.path.join(tmp_path, "snap")
+    os.makedirs(ref)
+    r = subprocess.run([exe, "-il", deck, "--lanes", "1", "--dumpbinary", "5", ref],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert len([f for f in os.listdir(ref) if f.startswith("output.")]) == 5
+    out = os.path.join(tmp_path, "corr.dat")
+    r = subprocess.run([exe, "-il", deck, "--lanes", "1", "--correctness", "10", ref, out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [ln.split() for ln in open(out).read().splitlines()[1:]]
+    assert [row[0] for row in rows] == ["0", "10", "20"]
+    assert all(float(v) == 0.0 for row in rows for v in row[1:])
