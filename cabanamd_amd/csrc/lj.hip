@@ -147,7 +147,7 @@ __device__ __forceinline__ void accumulate( double &fx, double &fy, double &fz,
 __device__ void block_sum4( double a, double b, double c, double d,
                             double *__restrict__ out )
 {
-    __shared__ double red[4][TPB / 64];
+    __shared__ double red[4][8]; // up to 8 waves per block
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for ( int o = 32; o > 0; o >>= 1 )
@@ -168,7 +168,7 @@ __device__ void block_sum4( double a, double b, double c, double d,
     if ( threadIdx.x < 4 )
     {
         double s = 0.0;
-        for ( int w = 0; w < TPB / 64; ++w )
+        for ( int w = 0; w < (int)( blockDim.x >> 6 ); ++w )
             s += red[threadIdx.x][w];
         out[threadIdx.x] = s;
     }
@@ -1032,9 +1032,22 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     {
         int G, E;
     };
+    // lanes per atom x tile height
     const Cand cands[] = { { 1, 0 }, { 2, 4 }, { 2, 6 }, { 2, 8 }, { 4, 4 },
                            { 4, 6 }, { 4, 8 }, { 8, 4 }, { 8, 6 } };
+    // time the fused step the run loop launches (force, both half-kicks and
+    // the drift into x_alt); the velocities it kicks are restored after
     StepCtl ctl{};
+    ctl.dtf = 0.5 * S->dt / S->mvv2e;
+    ctl.dtv = S->dt;
+    for ( int k = 0; k < 3; ++k )
+        ctl.L[k] = S->dom.L[k];
+    ctl.do_final = 1;
+    ctl.do_next_initial = 1;
+    CMD_TRY( S->x_alt.reserve( S->cap ) );
+    CMD_TRY( S->v_alt.reserve( 3 * (size_t)S->cap ) );
+    const size_t vbytes = sizeof( double ) * 3 * (size_t)S->cap;
+    CMD_HIP( hipMemcpyAsync( S->v_alt.p, S->v.p, vbytes, hipMemcpyDeviceToDevice, S->stream ) );
     hipEvent_t e0, e1;
     CMD_HIP( hipEventCreate( &e0 ) );
     CMD_HIP( hipEventCreate( &e1 ) );
@@ -1050,19 +1063,22 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
             break;
         if ( cd.E > 0 && !N->tiled )
             continue; // the tiled layout does not apply here
-        st = lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+        constexpr int reps = 6;
+        for ( int r = 0; r < 2; ++r )
+            st = lj_full( F, S, N, ctl, S->x_alt.p, nullptr, nullptr );
         if ( st != CMD_OK )
             break;
         (void)hipEventRecord( e0, S->stream );
-        for ( int r = 0; r < 3; ++r )
-            (void)lj_full( F, S, N, ctl, nullptr, nullptr, nullptr );
+        for ( int r = 0; r < reps; ++r )
+            (void)lj_full( F, S, N, ctl, S->x_alt.p, nullptr, nullptr );
         (void)hipEventRecord( e1, S->stream );
         (void)hipEventSynchronize( e1 );
+        (void)hipMemcpyAsync( S->v.p, S->v_alt.p, vbytes, hipMemcpyDeviceToDevice, S->stream );
         float ms = 0.0f;
         (void)hipEventElapsedTime( &ms, e0, e1 );
         if ( std::getenv( "CMD_VERBOSE" ) )
-            std::fprintf( stderr, "# autotune lanes=%d tile=%d: %.4f ms/launch\n", cd.G, cd.E,
-                          ms / 3.0f );
+            std::fprintf( stderr, "# autotune lanes=%d tile=%d: %.4f ms/step\n", cd.G, cd.E,
+                          ms / reps );
         if ( ms < best )
         {
             best = ms;
@@ -1071,6 +1087,7 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
+    CMD_HIP( hipMemcpyAsync( S->v.p, S->v_alt.p, vbytes, hipMemcpyDeviceToDevice, S->stream ) );
     CMD_TRY( st );
     F->lanes = bestc.G;
     F->tuned = true;

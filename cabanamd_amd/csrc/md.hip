@@ -43,6 +43,10 @@ struct cmd_md
     bool inited = false;
     int prec = CMD_PREC_FP64;
     int lanes = 0;
+    int tile = 0; // tile height of the tiled list (0 = default or tuned)
+    // the layout tuned at init (on the starting lattice) is tuned again at
+    // the first rebuild, when the atoms have left their sites
+    bool retune = false;
     std::vector<double> thermo; // rows of 5: step, T, PE/N, (PE+KE)/N, W
     cmd_nnp_model *nnp_model = nullptr; // injected NNP model (owned)
     bool custom_state = false;          // initial state given by the caller
@@ -137,7 +141,14 @@ static cmd_status rebuild( cmd_md *md )
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
     md->end( k );
     k = md->begin( PH_NEIGH );
-    CMD_TRY( cmd_neigh_build( md->N, S ) );
+    if ( md->retune )
+    {
+        md->retune = false;
+        CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
+        md->F->lanes = md->N->lanes;
+    }
+    else
+        CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->end( k );
     // ghost force rows are never written by full-list kernels: keep them 0
     if ( !md->N->half )
@@ -387,6 +398,16 @@ cmd_status cmd_md_set_lanes( cmd_md *md, int lanes )
     return CMD_OK;
 }
 
+cmd_status cmd_md_set_tile( cmd_md *md, int height )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "the tile height is set before init" );
+    CMD_REQUIRE( height >= 0 && height <= 14, CMD_ERR_ARG,
+                 "tile height must be 0 (choose) or 1..14 cells" );
+    md->tile = height;
+    return CMD_OK;
+}
+
 cmd_status cmd_md_set_nnp_model( cmd_md *md, const cmd_nnp_model *m )
 {
     CMD_REQUIRE( md && m, CMD_ERR_ARG, "NULL argument" );
@@ -521,13 +542,18 @@ cmd_status cmd_md_init( cmd_md *md )
         md->N->allow_tiles = false; // the NNP kernels read the CSR view
     else if ( md->lanes > 0 )
         CMD_TRY( cmd_neigh_set_lanes( md->N, md->lanes ) );
+    if ( !nnp && md->tile > 0 )
+        CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
     CMD_TRY( cmd_comm_exchange( S ) );
     CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
                                  md->neigh_cut, 1, 1, 0, 1, nullptr ) );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
     if ( md->lanes == 0 && !p.half && !nnp )
+    {
         CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
+        md->retune = true;
+    }
     else
         CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->F->lanes = md->N->lanes;

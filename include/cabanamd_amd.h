@@ -330,6 +330,9 @@ cmd_status cmd_md_create( const cmd_params *p, cmd_md **out );
 cmd_status cmd_md_destroy( cmd_md *md );
 cmd_status cmd_md_set_precision( cmd_md *md, int prec );
 cmd_status cmd_md_set_lanes( cmd_md *md, int lanes );
+/* Tile height (cells) of the tiled list when lanes are fixed (no reference
+ * counterpart; for tuning): 0 keeps the default. */
+cmd_status cmd_md_set_tile( cmd_md *md, int height );
 /* Before init (no reference counterpart): an NNP model to use instead of
  * loading pair_style nnp's directory, and an initial state (atoms and box)
  * to use instead of the deck's lattice and velocities. */
