@@ -224,50 +224,127 @@ __device__ __forceinline__ int cell_of_pos( const CellGrid &g, double4 p )
     return ( a * g.n[1] + b ) * g.n[2] + c;
 }
 
+// Atoms arrive mostly in cell order (they were sorted at the last rebuild
+// and have moved less than a cell since), so a wave's lanes form runs of
+// equal cell ids.  One lane per run does the atomic for the whole run.
+// Valid lanes are a prefix of the wave.
+struct CellRun
+{
+    bool head;    // first lane of its run
+    int head_lane;
+    int len;      // run length (meaningful on the head lane)
+};
+
+__device__ __forceinline__ CellRun cell_run( int c, bool valid )
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up( c, 1, 64 );
+    CellRun R;
+    R.head = valid && ( lane == 0 || prev != c );
+    const unsigned long long hm = __builtin_amdgcn_ballot_w64( R.head );
+    const int nvalid = (int)__popcll( __builtin_amdgcn_ballot_w64( valid ) );
+    const unsigned long long upto = ( 2ull << lane ) - 1ull; // lanes 0..lane
+    const unsigned long long below = hm & upto, above = hm & ~upto;
+    R.head_lane = below ? 63 - __clzll( (long long)below ) : lane;
+    R.len = ( above ? __ffsll( (long long)above ) - 1 : nvalid ) - lane;
+    return R;
+}
+
 __global__ void k_cell_count( const double4 *__restrict__ x, int n,
                               CellGrid g, int *__restrict__ cell_of,
                               int *__restrict__ count )
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n )
-        return;
-    int c = cell_of_pos( g, x[i] );
-    cell_of[i] = c;
-    atomicAdd( &count[c], 1 );
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    int c = -1;
+    if ( valid )
+    {
+        c = cell_of_pos( g, x[i] );
+        cell_of[i] = c;
+    }
+    const CellRun R = cell_run( c, valid );
+    if ( R.head )
+        atomicAdd( &count[c], R.len );
 }
 
+__global__ void k_cell_count_ids( int n, const int *__restrict__ cell_of,
+                                  int *__restrict__ count )
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const int c = valid ? cell_of[i] : -1;
+    const CellRun R = cell_run( c, valid );
+    if ( R.head )
+        atomicAdd( &count[c], R.len );
+}
+
+// Slots inside each cell in arrival order: a run keeps its order, runs of
+// one cell land in the order their atomics happen.
 __global__ void k_cell_fill( int n, const int *__restrict__ cell_of,
                              const int *__restrict__ start,
-                             int *__restrict__ cursor, int *__restrict__ perm )
+                             int *__restrict__ cursor, int *__restrict__ out )
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n )
-        return;
-    int c = cell_of[i];
-    int slot = start[c] + atomicAdd( &cursor[c], 1 );
-    perm[slot] = i;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const int c = valid ? cell_of[i] : -1;
+    const CellRun R = cell_run( c, valid );
+    int b = 0;
+    if ( R.head )
+        b = atomicAdd( &cursor[c], R.len );
+    b = __shfl( b, R.head_lane, 64 );
+    if ( valid )
+        out[start[c] + b + ( ( (int)threadIdx.x & 63 ) - R.head_lane )] = i;
 }
 
-// Restore index order inside each cell (insertion sort; cells hold tens of
-// atoms), which makes the sort stable and deterministic.
-__global__ void k_cell_order( int ncell, const int *__restrict__ start,
-                              int *__restrict__ perm )
+// Index order inside each cell, which makes the sort stable and
+// deterministic: an atom's place is the number of its cell's atoms with a
+// smaller index (cells hold tens of atoms at the densities binned here).
+__global__ void k_cell_rank( int n, const int *__restrict__ cell_of,
+                             const int *__restrict__ start,
+                             const int *__restrict__ in, int *__restrict__ perm )
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( c >= ncell )
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( k >= n )
         return;
+    const int v = in[k];
+    const int c = cell_of[v];
     const int b = start[c], e = start[c + 1];
-    for ( int k = b + 1; k < e; ++k )
+    int r = 0;
+    for ( int m = b; m < e; ++m )
+        r += in[m] < v;
+    perm[b + r] = v;
+}
+
+// Shared tail of both sorts: scan of the counts, fill, rank.
+static cmd_status cell_sort_tail( int n, int ncell, CellSort &cs, Scratch &sc,
+                                  hipStream_t st )
+{
+    CMD_TRY( scan_exclusive( cs.count.p, cs.start.p, ncell, sc, st, nullptr ) );
+    if ( n > 0 )
     {
-        int v = perm[k];
-        int m = k - 1;
-        while ( m >= b && perm[m] > v )
-        {
-            perm[m + 1] = perm[m];
-            --m;
-        }
-        perm[m + 1] = v;
+        k_cell_fill<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
+                                                      cs.start.p, cs.cursor.p,
+                                                      cs.tmp.p );
+        CMD_LAUNCH_CHECK();
+        k_cell_rank<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
+                                                      cs.start.p, cs.tmp.p,
+                                                      cs.perm.p );
+        CMD_LAUNCH_CHECK();
     }
+    return CMD_OK;
+}
+
+static cmd_status cell_sort_reserve( int n, int ncell, CellSort &cs, hipStream_t st )
+{
+    cs.ncell = ncell;
+    CMD_TRY( cs.count.reserve( ncell ) );
+    CMD_TRY( cs.cursor.reserve( ncell ) );
+    CMD_TRY( cs.start.reserve( ncell + 1 ) );
+    CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
+    CMD_TRY( cs.tmp.reserve( std::max( n, 1 ) ) );
+    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
+    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    return CMD_OK;
 }
 
 cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
@@ -275,41 +352,15 @@ cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,
 {
     const int ncell = g.n[0] * g.n[1] * g.n[2];
     CMD_REQUIRE( ncell > 0, CMD_ERR_ARG, "cell grid is empty" );
-    cs.ncell = ncell;
-    CMD_TRY( cs.count.reserve( ncell ) );
-    CMD_TRY( cs.cursor.reserve( ncell ) );
-    CMD_TRY( cs.start.reserve( ncell + 1 ) );
     CMD_TRY( cs.cell_of.reserve( std::max( n, 1 ) ) );
-    CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
-    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
-    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    CMD_TRY( cell_sort_reserve( n, ncell, cs, st ) );
     if ( n > 0 )
     {
         k_cell_count<<<nblocks_for( n ), TPB, 0, st>>>( x, n, g, cs.cell_of.p,
                                                        cs.count.p );
         CMD_LAUNCH_CHECK();
     }
-    CMD_TRY( scan_exclusive( cs.count.p, cs.start.p, ncell, sc, st, nullptr ) );
-    if ( n > 0 )
-    {
-        k_cell_fill<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
-                                                      cs.start.p, cs.cursor.p,
-                                                      cs.perm.p );
-        CMD_LAUNCH_CHECK();
-        k_cell_order<<<nblocks_for( ncell ), TPB, 0, st>>>( ncell, cs.start.p,
-                                                           cs.perm.p );
-        CMD_LAUNCH_CHECK();
-    }
-    return CMD_OK;
-}
-
-__global__ void k_cell_count_ids( int n, const int *__restrict__ cell_of,
-                                  int *__restrict__ count )
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n )
-        return;
-    atomicAdd( &count[cell_of[i]], 1 );
+    return cell_sort_tail( n, ncell, cs, sc, st );
 }
 
 cmd_status cell_sort_ids( int n, int ncell, CellSort &cs, Scratch &sc,
@@ -318,31 +369,14 @@ cmd_status cell_sort_ids( int n, int ncell, CellSort &cs, Scratch &sc,
     CMD_REQUIRE( ncell > 0, CMD_ERR_ARG, "cell grid is empty" );
     CMD_REQUIRE( (int64_t)cs.cell_of.n >= n, CMD_ERR_STATE,
                  "cell ids were not computed" );
-    cs.ncell = ncell;
-    CMD_TRY( cs.count.reserve( ncell ) );
-    CMD_TRY( cs.cursor.reserve( ncell ) );
-    CMD_TRY( cs.start.reserve( ncell + 1 ) );
-    CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
-    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
-    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    CMD_TRY( cell_sort_reserve( n, ncell, cs, st ) );
     if ( n > 0 )
     {
         k_cell_count_ids<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
                                                            cs.count.p );
         CMD_LAUNCH_CHECK();
     }
-    CMD_TRY( scan_exclusive( cs.count.p, cs.start.p, ncell, sc, st, nullptr ) );
-    if ( n > 0 )
-    {
-        k_cell_fill<<<nblocks_for( n ), TPB, 0, st>>>( n, cs.cell_of.p,
-                                                      cs.start.p, cs.cursor.p,
-                                                      cs.perm.p );
-        CMD_LAUNCH_CHECK();
-        k_cell_order<<<nblocks_for( ncell ), TPB, 0, st>>>( ncell, cs.start.p,
-                                                           cs.perm.p );
-        CMD_LAUNCH_CHECK();
-    }
-    return CMD_OK;
+    return cell_sort_tail( n, ncell, cs, sc, st );
 }
 
 // ---------------------------------------------------------------- NVE

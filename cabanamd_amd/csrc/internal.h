@@ -180,6 +180,7 @@ struct TileGrid
 struct CellSort
 {
     DevBuf<int> cell_of, count, cursor, start, perm;
+    DevBuf<int> tmp; // per-cell fill in arrival order, before the rank pass
     int ncell = 0;
 };
 cmd_status cell_sort( const double4 *x, int n, const CellGrid &g, CellSort &cs,

@@ -112,16 +112,108 @@ __device__ __forceinline__ bool tile_accept( const double4 &pi, const double4 &p
 }
 
 // List build, one wave per cell of the tile.  All atoms of a cell share the
-// same 27-cell stencil, so the wave walks the stencil's candidates once, 64
-// at a time (lane = candidate, positions from LDS planes), and tests each
-// chunk against every atom of the cell in turn; ballot + mbcnt place each
-// accepted candidate in the atom's row, so rows come out in walk order.  The
+// same 27-cell stencil, so the wave loads the stencil's candidates into
+// registers, BCH chunks of 64 at a time (lane = candidate, positions from
+// LDS planes), and tests every atom of the cell against them in turn, the
+// atom's position and row length being wave-uniform (scalar loads and
+// registers); ballot + mbcnt place each accepted candidate in the atom's
+// row, so rows come out in walk order.  The
 // stencil is a list of contiguous LDS ranges (per column: the locals of its
 // three cells, or locals and ghosts cell by cell where ghosts are present),
 // concatenated so that chunks run across range ends.  Rows are assembled in
-// LDS in the slab layout, pre-filled with the sentinel slot, and written out
-// with 16-byte stores.
+// LDS one after another (pre-filled with the sentinel slot), so an entry's
+// address is the row start plus its index, and are transposed into the slab
+// layout on the way out.
 static constexpr int MAXR = 9 * 3 * 2; // stencil ranges of one cell
+static constexpr int BCH = 10;         // candidate chunks of 64 held in registers
+
+// Bytes of the coordinate planes of `slots` staged atoms, rounded up so that
+// what follows is 16-byte aligned.
+__host__ __device__ constexpr size_t tile_planes_bytes( int slots )
+{
+    return ( (size_t)slots * 3 * sizeof( double ) + 15 ) & ~(size_t)15;
+}
+
+// Entries of one LDS row of the list build: the slab's entries per atom
+// (R4 groups of 4 G) plus 2, an odd number of 4-byte words.
+__host__ __device__ constexpr int tile_row_stride( int R4, int G )
+{
+    return R4 * 4 * G + 2;
+}
+
+// One atom against NC register chunks of candidates: every chunk's test
+// first (independent chains, one mask each), then the row entries in chunk
+// order.  A lane that does not accept writes its own slot of a trash area
+// instead, so the stores need no branch.  Returns the atom's new row length.
+template <int NC, int G, bool HALF>
+__device__ __forceinline__ unsigned build_row( const double ( &px )[BCH], const double ( &py )[BCH],
+                                               const double ( &pz )[BCH], const int ( &po )[BCH],
+                                               const double4 pa, int vs, unsigned short *row,
+                                               unsigned k0, double rc2, int cap,
+                                               unsigned short *trash )
+{
+    const int lane = threadIdx.x & 63;
+    // the atom itself: candidate vs of this pass (chunk vs >> 6, lane
+    // vs & 63; outside the pass when the chunk is not in 0 .. NC - 1)
+    const int cself = vs >> 6;
+    const unsigned long long notself = ~( 1ull << ( vs & 63 ) );
+    unsigned long long m[NC];
+#pragma unroll
+    for ( int c = 0; c < NC; ++c )
+    {
+        const double dx = pa.x - px[c];
+        const double dy = pa.y - py[c];
+        const double dz = pa.z - pz[c];
+        const double r = dx * dx + dy * dy + dz * dz;
+        // ballots of the plain compares are their lane masks as they stand
+        // (a ballot of a combined predicate is rematerialised first)
+        m[c] = __builtin_amdgcn_ballot_w64( r <= rc2 ) & ( c == cself ? notself : ~0ull );
+        if ( HALF )
+        {
+            const bool up = ( px[c] > pa.x ) |
+                            ( ( px[c] == pa.x ) &
+                              ( ( py[c] > pa.y ) | ( ( py[c] == pa.y ) & ( pz[c] > pa.z ) ) ) );
+            m[c] &= __builtin_amdgcn_ballot_w64( up );
+        }
+    }
+#pragma unroll
+    for ( int c = 0; c < NC; ++c )
+    {
+        const unsigned k = __builtin_amdgcn_mbcnt_hi(
+            (unsigned)( m[c] >> 32 ), __builtin_amdgcn_mbcnt_lo( (unsigned)m[c], k0 ) );
+        const bool put =
+            __builtin_amdgcn_inverse_ballot_w64( m[c] & __builtin_amdgcn_ballot_w64( k < (unsigned)cap ) );
+        *( put ? row + k : trash + lane ) = (unsigned short)po[c];
+        k0 += (unsigned)__popcll( m[c] );
+    }
+    return k0;
+}
+
+// The atoms a0 .. a0 + na - 1 of a cell (candidates vs0 + l of this pass,
+// LDS rows row0 + l * RS) against NC register chunks of candidates, one
+// atom at a time: its
+// position and row length are wave-uniform (scalar loads and registers).
+// Lane l of cnt holds atom a0 + l's row length.
+template <int NC, int G, bool HALF>
+__device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const double ( &py )[BCH],
+                                             const double ( &pz )[BCH], const int ( &po )[BCH],
+                                             const double4 *__restrict__ x, int a0, int na,
+                                             int vs0, unsigned short *row0, int RS, double rc2,
+                                             int cap, unsigned short *trash, int &cnt )
+{
+    const int lane = threadIdx.x & 63;
+    double4 pn = x[a0];
+    for ( int l = 0; l < na; ++l )
+    {
+        const double4 pa = pn;
+        pn = x[a0 + min( l + 1, na - 1 )];
+        unsigned k0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
+        k0 = build_row<NC, G, HALF>( px, py, pz, po, pa, vs0 + l, row0 + l * RS, k0,
+                                     rc2, cap, trash );
+        if ( lane == l )
+            cnt = (int)k0;
+    }
+}
 
 template <int G, bool HALF>
 __global__ __launch_bounds__( BTPB ) void k_tile_build(
@@ -151,20 +243,29 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
     }
     stage_planes<false>( H, x, xg, s_x, slot_cap );
     const double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
-    unsigned short *chunk = reinterpret_cast<unsigned short *>( s_x + 3 * slot_cap );
+    // after the planes (rounded to 16 bytes): a 64-entry trash area, then
+    // the rows
+    unsigned short *trash = reinterpret_cast<unsigned short *>(
+        reinterpret_cast<char *>( s_x ) + tile_planes_bytes( slot_cap ) );
+    unsigned short *chunk = trash + 64;
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     const int wsz = R4 * 256; // slab entries of one slab wave
+    // LDS rows: one per atom of the tile's slab waves, entries in order;
+    // the stride is an odd number of words, which spreads the copy-out's
+    // strided reads over the banks
+    const int RS = tile_row_stride( R4, G );
     const int nw = woff[t + 1] - woff[t];
     const int tl = H.info[TI_TL];
     const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
+    const int sent_slot = H.segoff[18];
     // rows start as the sentinel record (the tile's slot TOTAL)
     {
-        const unsigned sent = (unsigned)( H.segoff[18] * TILE_REC );
+        const unsigned sent = (unsigned)( sent_slot * TILE_REC );
         const unsigned sent2 = sent | ( sent << 16 );
         const uint4 fill = make_uint4( sent2, sent2, sent2, sent2 );
-        const int n16 = nw * wsz / 8;
+        const int n16 = nw * APW * RS / 8;
         for ( int k = threadIdx.x; k < n16; k += blockDim.x )
             reinterpret_cast<uint4 *>( chunk )[k] = fill;
     }
@@ -194,6 +295,7 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
         }
         const int nr = __builtin_amdgcn_readfirstlane( __shfl( off, 8, 64 ) );
         off -= nr_q;
+        const int off0 = off;
         if ( lane < 9 )
         {
             if ( !gh )
@@ -230,6 +332,13 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
             rp[0] = 0;
         const int total = __builtin_amdgcn_readfirstlane( __shfl( inc, nr - 1, 64 ) );
         __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+        // candidate index of the cell's first atom: its slot lies in the
+        // centre column's (lane 4's) local range of cell ce
+        int vb = 0;
+        if ( lane == 4 )
+            vb = gh ? rp[off0 + 2 * ( ce - elo )]
+                    : rp[off0] + ( s_lcs[4][ce] - s_lcs[4][elo] );
+        const int vbase = __builtin_amdgcn_readlane( vb, 4 );
         // the cell's atoms, up to 64 at a time (lane l holds atom a0 + l)
         const int cc = col_base( T, a, b ) + e;
         const int c0 = __builtin_amdgcn_readfirstlane( ls[cc] );
@@ -238,74 +347,56 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
         {
             const int na = min( 64, c1 - a0 );
             int cnt = 0; // row length of atom a0 + lane
-            for ( int v0 = 0; v0 < total; v0 += 64 )
+            for ( int v0 = 0; v0 < total; v0 += BCH * 64 )
             {
-                const int v = v0 + lane;
-                const bool valid = v < total;
-                // range holding candidate v: binary search over the prefix
-                int lo = 0, hi = nr;
-                while ( hi - lo > 1 )
+                // BCH chunks of candidates in registers (lane = candidate);
+                // past the end: the sentinel position, which no test accepts
+                double px[BCH], py[BCH], pz[BCH];
+                int po[BCH]; // record byte offset, the value a row stores
+#pragma unroll
+                for ( int c = 0; c < BCH; ++c )
                 {
-                    const int mid = ( lo + hi ) >> 1;
-                    if ( rp[mid] <= v )
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                const int sj = valid ? rs[lo] + ( v - rp[lo] ) : si0 + a0;
-                const double pjx = s_x[sj], pjy = s_y[sj], pjz = s_z[sj];
-                // two atoms per pass: independent distance chains in flight
-                for ( int l = 0; l < na; l += 2 )
-                {
-                    const bool two = l + 1 < na;
-                    const int l2 = two ? l + 1 : l;
-                    const int s1 = si0 + a0 + l, s2 = si0 + a0 + l2;
-                    const double ax = s_x[s1], ay = s_y[s1], az = s_z[s1];
-                    const double bx = s_x[s2], by = s_y[s2], bz = s_z[s2];
-                    const double dx1 = ax - pjx, dy1 = ay - pjy, dz1 = az - pjz;
-                    const double dx2 = bx - pjx, dy2 = by - pjy, dz2 = bz - pjz;
-                    const double r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-                    const double r2 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
-                    bool acc1 = valid & ( sj != s1 ) & ( r1 <= rc2 );
-                    bool acc2 = two & valid & ( sj != s2 ) & ( r2 <= rc2 );
-                    if ( HALF )
+                    const int v = v0 + c * 64 + lane;
+                    int sj = sent_slot;
+                    if ( v0 + c * 64 < total )
                     {
-                        acc1 = acc1 & ( ( pjx > ax ) |
-                                        ( ( pjx == ax ) &
-                                          ( ( pjy > ay ) | ( ( pjy == ay ) & ( pjz > az ) ) ) ) );
-                        acc2 = acc2 & ( ( pjx > bx ) |
-                                        ( ( pjx == bx ) &
-                                          ( ( pjy > by ) | ( ( pjy == by ) & ( pjz > bz ) ) ) ) );
+                        // range holding candidate v: binary search over the
+                        // prefix of range lengths
+                        int lo = 0, hi = nr;
+                        while ( hi - lo > 1 )
+                        {
+                            const int mid = ( lo + hi ) >> 1;
+                            if ( rp[mid] <= v )
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        if ( v < total )
+                            sj = rs[lo] + ( v - rp[lo] );
                     }
-                    const unsigned long long m1 = __ballot( acc1 );
-                    const unsigned long long m2 = __ballot( acc2 );
-                    const int c1 = __builtin_amdgcn_readlane( cnt, l );
-                    const int c2 = __builtin_amdgcn_readlane( cnt, l2 );
-                    const unsigned k1 =
-                        (unsigned)c1 + __builtin_amdgcn_mbcnt_hi(
-                                           (unsigned)( m1 >> 32 ),
-                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m1, 0u ) );
-                    const unsigned k2 =
-                        (unsigned)c2 + __builtin_amdgcn_mbcnt_hi(
-                                           (unsigned)( m2 >> 32 ),
-                                           __builtin_amdgcn_mbcnt_lo( (unsigned)m2, 0u ) );
-                    auto put = [&]( int ll, unsigned k )
-                    {
-                        const unsigned ta = (unsigned)( a0 + ll - tl );
-                        const unsigned r = k / G;
-                        const unsigned L = ( k % G ) * APW + ta % APW;
-                        chunk[( ( ( ta / APW ) * R4 + ( r >> 2 ) ) * 64 + L ) * 4 + ( r & 3 )] =
-                            (unsigned short)( sj * TILE_REC );
-                    };
-                    if ( acc1 && k1 < (unsigned)cap )
-                        put( l, k1 );
-                    if ( acc2 && k2 < (unsigned)cap )
-                        put( l2, k2 );
-                    if ( lane == l )
-                        cnt = c1 + (int)__popcll( m1 );
-                    if ( two && lane == l2 )
-                        cnt = c2 + (int)__popcll( m2 );
+                    px[c] = s_x[sj];
+                    py[c] = s_y[sj];
+                    pz[c] = s_z[sj];
+                    po[c] = sj * TILE_REC;
                 }
+                const int nch = min( BCH, ( total - v0 + 63 ) >> 6 );
+                const int vs0 = vbase + ( a0 - c0 ) - v0;
+                unsigned short *row0 = chunk + ( a0 - tl ) * RS;
+                if ( nch <= 4 )
+                    build_atoms<4, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                                             cap, trash, cnt );
+                else if ( nch <= 6 )
+                    build_atoms<6, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                                             cap, trash, cnt );
+                else if ( nch <= 8 )
+                    build_atoms<8, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                                             cap, trash, cnt );
+                else if ( nch == 9 )
+                    build_atoms<9, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                                             cap, trash, cnt );
+                else
+                    build_atoms<BCH, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS,
+                                               rc2, cap, trash, cnt );
             }
             if ( lane < na )
             {
@@ -318,11 +409,20 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
     }
     __syncthreads();
     {
-        const int n16 = nw * wsz / 8;
-        const uint4 *src = reinterpret_cast<const uint4 *>( chunk );
-        uint4 *dst = reinterpret_cast<uint4 *>( slab + (size_t)woff[t] * wsz );
-        for ( int k = threadIdx.x; k < n16; k += blockDim.x )
-            dst[k] = src[k];
+        // slab layout: per slab wave, R4 groups of 64 eight-byte words; word
+        // (group g4, lane L) holds entries (4 g4 + j) G + L / APW, j = 0..3,
+        // of the wave's atom L % APW
+        const int nwords = nw * R4 * 64;
+        uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)woff[t] * wsz );
+        for ( int w = threadIdx.x; w < nwords; w += blockDim.x )
+        {
+            const int sw = w / ( R4 * 64 ), rem = w - sw * ( R4 * 64 );
+            const int g4 = rem >> 6, L = rem & 63;
+            const unsigned short *src =
+                chunk + ( sw * APW + L % APW ) * RS + g4 * 4 * G + L / APW;
+            dst[w] = make_uint2( (unsigned)src[0] | ( (unsigned)src[G] << 16 ),
+                                 (unsigned)src[2 * G] | ( (unsigned)src[3 * G] << 16 ) );
+        }
     }
 #pragma unroll
     for ( int o = 32; o > 0; o >>= 1 )
@@ -542,11 +642,13 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             // box) belong to no tile: use the global layout
             if ( h[1] != S->n_local )
                 return CMD_OK;
+            const int R4 = ( ( N->cap + G - 1 ) / G + 3 ) / 4;
             const size_t wbytes =
-                (size_t)( ( ( N->cap + G - 1 ) / G + 3 ) / 4 ) * 256 * sizeof( unsigned short );
+                (size_t)( 64 / G ) * tile_row_stride( R4, G ) * sizeof( unsigned short );
             // coordinate planes of the staged atoms and the sentinel slot,
-            // then the rows of every slab wave of the tile
-            lds = (size_t)( h[0] + 1 ) * 3 * sizeof( double ) +
+            // the trash area of rejected entries, then the rows of every
+            // slab wave of the tile
+            lds = tile_planes_bytes( h[0] + 1 ) + 64 * sizeof( unsigned short ) +
                   (size_t)std::max( h[2], 1 ) * wbytes;
             if ( h[0] <= MAX_SLOTS && lds <= hard )
                 break;
