@@ -596,56 +596,164 @@ cmd_status wrap_local( cmd_system *S )
 }
 
 // ---------------------------------------------------------------- ghosts
-// TagHaloPack (comm_mpi.h:240-319): flag atoms of the current set near the
-// face of phase p.
-__global__ void k_ghost_flag( int n, const double4 *__restrict__ x, int dim,
-                              int even, double thr, int *__restrict__ flag )
+// The six-phase halo build (comm_mpi_impl.h:280-367, TagHaloPack
+// comm_mpi.h:240-319, TagHaloPBC :323-353) in one pass.  Phase p works on
+// dimension p / 2: an even phase sends the atoms of the current set with
+// x >= hi - depth, shifted by -L; an odd phase those with x <= lo + depth,
+// shifted by +L, skipping the ghosts the even phase of its dimension has
+// just made.  Ghosts of earlier phases are sent on, and their coordinate in
+// the phase's dimension is their root's.  So a local atom has one image per
+// set of phases (at most one per dimension) whose conditions its own
+// coordinates meet, and since each phase appends in the order of its
+// sources, the ghosts come out grouped by that set, the sets in the order
+// below, each group in ascending root index.  Bit p of a set is phase p.
+static constexpr int HALO_SETS = 26;
+__host__ __device__ constexpr int halo_set( int j )
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n )
-        return;
-    const double4 p = x[i];
-    const double c = dim == 0 ? p.x : ( dim == 1 ? p.y : p.z );
-    flag[i] = even ? ( c >= thr ) : ( c <= thr );
+    constexpr int s[HALO_SETS] = { 1,  2,  4,  5,  6,  8,  9,  10, 16, 17, 18, 20, 21,
+                                   22, 24, 25, 26, 32, 33, 34, 36, 37, 38, 40, 41, 42 };
+    return s[j];
+}
+// the set an image was sent on from (its set without its last phase), as
+// an index into the table above; -1 for the local atom itself
+__host__ __device__ constexpr int halo_source( int j )
+{
+    constexpr int s[HALO_SETS] = { -1, -1, -1, 0,  1, -1, 0, 1, -1, 0, 1, 2, 3,
+                                   4,  5,  6,  7, -1, 0, 1, 2, 3, 4, 5, 6, 7 };
+    return s[j];
+}
+__host__ __device__ constexpr int halo_phase( int j )
+{
+    return halo_set( j ) >= 32 ? 5 : halo_set( j ) >= 16 ? 4 : halo_set( j ) >= 8 ? 3
+         : halo_set( j ) >= 4 ? 2 : halo_set( j ) >= 2 ? 1 : 0;
 }
 
-// Cabana::gather + TagHaloPBC (comm_mpi.h:323-353): append the flagged
-// atoms as ghosts in pack order, shifted by -L (even phase) or +L (odd).
-__global__ void k_ghost_scatter( int n, double4 *__restrict__ x,
-                                 int *__restrict__ id,
-                                 const int *__restrict__ flag,
-                                 const int *__restrict__ pos, int base,
-                                 int n_local, int send_off, int phase,
-                                 double L, int *__restrict__ send_idx,
-                                 int *__restrict__ root, int *__restrict__ code )
+struct HaloFaces
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( i >= n || !flag[i] )
-        return;
-    const int k = pos[i];
-    const int g = base + k;
-    double4 p = x[i];
-    const int dim = phase / 2;
-    const bool even = ( phase % 2 ) == 0;
-    if ( dim == 0 )
-        p.x = even ? p.x - L : p.x + L;
-    else if ( dim == 1 )
-        p.y = even ? p.y - L : p.y + L;
-    else
-        p.z = even ? p.z - L : p.z + L;
-    x[g] = p;
-    id[g] = id[i];
-    send_idx[send_off + k] = i;
-    const int gi = g - n_local;
-    if ( i < n_local )
+    double hi[3]; // even phases send x >= hi[d]
+    double lo[3]; // odd phases send x <= lo[d]
+    double L[3];
+};
+
+__device__ __forceinline__ int halo_mask( const HaloFaces &H, double4 p )
+{
+    const double c[3] = { p.x, p.y, p.z };
+    int m = 0;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+        m |= ( c[d] >= H.hi[d] ? 1 : 0 ) << ( 2 * d ) | ( c[d] <= H.lo[d] ? 1 : 0 ) << ( 2 * d + 1 );
+    return m;
+}
+
+// Images per set and block (cnt[j * nblk + block]), the totals per set, and
+// each atom's phase mask.
+__global__ __launch_bounds__( TPB ) void k_halo_count( int n, const double4 *__restrict__ x,
+                                                       HaloFaces H, int *__restrict__ mask,
+                                                       int nblk, int *__restrict__ cnt,
+                                                       int *__restrict__ tot )
+{
+    __shared__ int s_c[HALO_SETS];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    if ( threadIdx.x < HALO_SETS )
+        s_c[threadIdx.x] = 0;
+    __syncthreads();
+    int m = 0;
+    if ( i < n )
     {
-        root[gi] = i;
-        code[gi] = 1 << phase;
+        m = halo_mask( H, x[i] );
+        mask[i] = m;
     }
-    else
+    if ( __builtin_amdgcn_ballot_w64( m != 0 ) )
     {
-        root[gi] = root[i - n_local];
-        code[gi] = code[i - n_local] | ( 1 << phase );
+#pragma unroll
+        for ( int j = 0; j < HALO_SETS; ++j )
+        {
+            const unsigned long long b =
+                __builtin_amdgcn_ballot_w64( ( m & halo_set( j ) ) == halo_set( j ) );
+            if ( lane == 0 && b )
+                atomicAdd( &s_c[j], (int)__popcll( b ) );
+        }
+    }
+    __syncthreads();
+    if ( threadIdx.x < HALO_SETS )
+    {
+        const int c = s_c[threadIdx.x];
+        cnt[threadIdx.x * nblk + blockIdx.x] = c;
+        if ( c )
+            atomicAdd( &tot[threadIdx.x], c );
+    }
+}
+
+// Writes every image: ghost slot n_local + k of set j's k-th image, where
+// off[j * nblk + block] (the exclusive scan of the counts) is where the
+// block's images of set j start.  send_idx records each ghost's source as
+// the phase saw it (the local atom, or the ghost it was sent on from).
+__global__ __launch_bounds__( TPB ) void k_halo_fill( int n, double4 *__restrict__ x,
+                                                      int *__restrict__ id,
+                                                      const int *__restrict__ mask,
+                                                      int nblk, const int *__restrict__ off,
+                                                      HaloFaces H, int *__restrict__ send_idx,
+                                                      int *__restrict__ root,
+                                                      int *__restrict__ code )
+{
+    __shared__ int s_w[HALO_SETS][TPB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = i < n ? mask[i] : 0;
+    const bool any = __builtin_amdgcn_ballot_w64( m != 0 ) != 0;
+#pragma unroll
+    for ( int j = 0; j < HALO_SETS; ++j )
+    {
+        const unsigned long long b =
+            any ? __builtin_amdgcn_ballot_w64( ( m & halo_set( j ) ) == halo_set( j ) ) : 0ull;
+        if ( lane == 0 )
+            s_w[j][wv] = (int)__popcll( b );
+    }
+    __syncthreads();
+    if ( !any )
+        return;
+    double4 p0 = make_double4( 0.0, 0.0, 0.0, 0.0 );
+    int idv = 0;
+    if ( m )
+    {
+        p0 = x[i];
+        idv = id[i];
+    }
+    int pos[HALO_SETS];
+#pragma unroll
+    for ( int j = 0; j < HALO_SETS; ++j )
+    {
+        const int c = halo_set( j );
+        const bool has = ( m & c ) == c;
+        const unsigned long long b = __builtin_amdgcn_ballot_w64( has );
+        int k = off[j * nblk + blockIdx.x];
+        for ( int w = 0; w < wv; ++w )
+            k += s_w[j][w];
+        k += (int)__builtin_amdgcn_mbcnt_hi( (unsigned)( b >> 32 ),
+                                             __builtin_amdgcn_mbcnt_lo( (unsigned)b, 0u ) );
+        pos[j] = k;
+        if ( has )
+        {
+            double4 p = p0;
+            if ( c & 1 )
+                p.x -= H.L[0];
+            if ( c & 2 )
+                p.x += H.L[0];
+            if ( c & 4 )
+                p.y -= H.L[1];
+            if ( c & 8 )
+                p.y += H.L[1];
+            if ( c & 16 )
+                p.z -= H.L[2];
+            if ( c & 32 )
+                p.z += H.L[2];
+            x[n + k] = p;
+            id[n + k] = idv;
+            root[k] = i;
+            code[k] = c;
+            send_idx[k] = halo_source( j ) < 0 ? i : n + pos[halo_source( j ) < 0 ? 0 : halo_source( j )];
+        }
     }
 }
 
@@ -992,8 +1100,6 @@ cmd_status cmd_system::reserve( int n )
     CMD_TRY( ghost_root.reserve( newcap, stream, true, n_ghost ) );
     CMD_TRY( ghost_code.reserve( newcap, stream, true, n_ghost ) );
     CMD_TRY( send_idx.reserve( newcap, stream, true, n_ghost ) );
-    CMD_TRY( flag.reserve( newcap ) );
-    CMD_TRY( pos.reserve( newcap + 1 ) );
     // planar v and f: copy plane by plane
     auto regrow_planes = [&]( DevBuf<double> &b ) -> cmd_status
     {
@@ -1191,50 +1297,49 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
     CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
     CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
     s->xg_valid = false;
-    const int n_local = s->n_local;
-    int n_ghost = 0, send_total = 0;
-    for ( int phase = 0; phase < 6; ++phase )
+    const int n = s->n_local;
+    s->n_ghost = 0;
+    for ( int k = 0; k < 6; ++k )
+        s->counts[k] = 0;
+    if ( n == 0 )
+        return CMD_OK;
+    HaloFaces H;
+    for ( int d = 0; d < 3; ++d )
     {
-        const int dim = phase / 2;
-        const int even = ( phase % 2 ) == 0;
-        const int nparticles =
-            n_local + n_ghost - ( even ? 0 : s->counts[phase - 1] );
-        const double thr = even ? s->dom.local_hi[dim] - comm_depth
-                                : s->dom.local_lo[dim] + comm_depth;
-        int64_t cnt = 0;
-        if ( nparticles > 0 )
-        {
-            k_ghost_flag<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
-                nparticles, s->x.p, dim, even, thr, s->flag.p );
-            CMD_LAUNCH_CHECK();
-        }
-        CMD_TRY( scan_exclusive( s->flag.p, s->pos.p, nparticles, s->sc,
-                                 s->stream, &cnt ) );
-        const int base = n_local + n_ghost;
-        if ( base + cnt > s->cap )
-        {
-            s->n_ghost = n_ghost; // keep the ghosts built so far on regrow
-            CMD_TRY( s->reserve( (int)( ( base + cnt ) * 1.1 ) + 64 ) );
-            // flags and positions are scratch: rebuild them at the new size
-            k_ghost_flag<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
-                nparticles, s->x.p, dim, even, thr, s->flag.p );
-            CMD_LAUNCH_CHECK();
-            CMD_TRY( scan_exclusive( s->flag.p, s->pos.p, nparticles, s->sc,
-                                     s->stream, &cnt ) );
-        }
-        if ( cnt > 0 )
-        {
-            k_ghost_scatter<<<nblocks_for( nparticles ), TPB, 0, s->stream>>>(
-                nparticles, s->x.p, s->id.p, s->flag.p, s->pos.p, base,
-                n_local, send_total, phase, s->dom.L[dim], s->send_idx.p,
-                s->ghost_root.p, s->ghost_code.p );
-            CMD_LAUNCH_CHECK();
-        }
-        s->counts[phase] = (int)cnt;
-        send_total += (int)cnt;
-        n_ghost += (int)cnt;
+        H.hi[d] = s->dom.local_hi[d] - comm_depth;
+        H.lo[d] = s->dom.local_lo[d] + comm_depth;
+        H.L[d] = s->dom.L[d];
     }
-    s->n_ghost = n_ghost;
+    const int nblk = nblocks_for( n );
+    const size_t nc = (size_t)HALO_SETS * nblk;
+    // counts, their scan (nc + 1 entries: the scan appends the total), the
+    // totals and the atoms' phase masks (kept apart from the system's
+    // arrays, which may be regrown below)
+    CMD_TRY( s->halo_blk.reserve( 2 * nc + 1 + HALO_SETS + n ) );
+    int *cnt = s->halo_blk.p, *off = cnt + nc, *tot = off + nc + 1, *mask = tot + HALO_SETS;
+    CMD_TRY( fill_i32( tot, 0, HALO_SETS, s->stream ) );
+    k_halo_count<<<nblk, TPB, 0, s->stream>>>( n, s->x.p, H, mask, nblk, cnt, tot );
+    CMD_LAUNCH_CHECK();
+    CMD_TRY( scan_exclusive( cnt, off, (int)nc, s->sc, s->stream, nullptr ) );
+    int th[HALO_SETS];
+    CMD_HIP( hipMemcpyAsync( th, tot, sizeof( th ), hipMemcpyDeviceToHost, s->stream ) );
+    CMD_HIP( hipStreamSynchronize( s->stream ) );
+    int ng = 0;
+    for ( int j = 0; j < HALO_SETS; ++j )
+    {
+        s->counts[halo_phase( j )] += th[j];
+        ng += th[j];
+    }
+    if ( n + ng > s->cap )
+        CMD_TRY( s->reserve( (int)( ( n + ng ) * 1.1 ) + 64 ) );
+    if ( ng > 0 )
+    {
+        k_halo_fill<<<nblk, TPB, 0, s->stream>>>( n, s->x.p, s->id.p, mask, nblk, off,
+                                                  H, s->send_idx.p, s->ghost_root.p,
+                                                  s->ghost_code.p );
+        CMD_LAUNCH_CHECK();
+    }
+    s->n_ghost = ng;
     return CMD_OK;
 }
 

@@ -215,7 +215,9 @@ struct cmd_system
     // reference's per-phase source lists back to back.
     cmd::DevBuf<int> ghost_root, ghost_code, send_idx;
     int counts[6] = { 0, 0, 0, 0, 0, 0 };
-    cmd::DevBuf<int> flag, pos;
+    // halo build scratch: per-block image counts of the 26 phase sets, their
+    // scan, the totals and the atoms' phase masks
+    cmd::DevBuf<int> halo_blk;
 
     cmd::CellSort bins;
     // grid of the last sorting create_binning (valid until atoms change)
