@@ -533,6 +533,158 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
 }
 
+// LDS bytes before the reaction records of the tiled half-list kernel: the
+// position records and, with several types, an int type per slot.
+__host__ __device__ constexpr size_t half_acc_offset( int slot_cap, bool one_type )
+{
+    return ( (size_t)slot_cap * TILE_REC + ( one_type ? 0 : (size_t)slot_cap * sizeof( int ) ) +
+             7 ) & ~(size_t)7;
+}
+
+// Tiled half list (compute_force_half, force_lj_cabana_neigh_impl.h:205-259):
+// each listed pair once, both sides accumulated.  The block stages its
+// tile's neighbourhood as records, with an FP64 reaction record per staged
+// slot in LDS at the same byte offset in a second area.  A pair adds d fpair
+// to atom i (registers) and subtracts it from slot j's reaction record (LDS
+// atomics); the atom's own total joins its slot's record.  When every row
+// of the tile is done, each slot with a nonzero record adds it to f of its
+// atom with global atomics (an atom is staged by several tiles); ghost
+// slots go to their ghost rows, which update_force folds.  Energy weights
+// follow compute_energy_half: 1 for a local j, 1/2 for a ghost j.
+template <int G, int PREC, bool ONE_TYPE, bool ENERGY>
+__global__ __launch_bounds__( TPB ) void k_tile_half(
+    TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
+    int lcap, int scap, int slot_cap, int n_local, const double4 *__restrict__ x,
+    const double4 *__restrict__ xg, const int *__restrict__ gperm,
+    double *__restrict__ f, const int *__restrict__ counts,
+    const unsigned short *__restrict__ slab, const LJParams *__restrict__ Pd,
+    PairCoef one, double *__restrict__ partial )
+{
+    extern __shared__ double s_rec[];
+    __shared__ TileHead H;
+    constexpr int APW = 64 / G;
+    const int t = xcd_tile( blockIdx.x, gridDim.x );
+    load_head( info, t, H );
+    char *s_byte = reinterpret_cast<char *>( s_rec );
+    const int *s_t = reinterpret_cast<const int *>( s_rec + 3 * slot_cap );
+    // reaction records, then (energy only) a ghost flag per slot
+    char *s_acc = s_byte + half_acc_offset( slot_cap, ONE_TYPE );
+    int *s_gh = reinterpret_cast<int *>( s_acc + (size_t)slot_cap * TILE_REC );
+    const int total = H.segoff[18];
+    for ( int k = threadIdx.x; k < 3 * slot_cap; k += blockDim.x )
+        reinterpret_cast<double *>( s_acc )[k] = 0.0;
+    if ( ENERGY )
+    {
+        int seg = 0;
+        for ( int k = threadIdx.x; k < slot_cap; k += blockDim.x )
+        {
+            while ( seg < 18 && k >= H.segoff[seg + 1] )
+                ++seg;
+            s_gh[k] = ( seg & 1 ) && k < total;
+        }
+    }
+    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap ); // ends with a barrier
+    const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
+    const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
+    const int nw = woff[t + 1] - woff[t];
+    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
+    const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int a = lane % APW, sub = lane / APW;
+    double pe = 0.0, vir = 0.0;
+    for ( int w = wid; w < nw; w += TPB / 64 )
+    {
+        const int ta = w * APW + a;
+        const bool active = ta < nat;
+        const int i = tl + ta;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const double4 xi =
+            active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
+        const int ti = (int)xi.w;
+        const int n = active ? counts[i] : 0;
+        int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
+#pragma unroll
+        for ( int o = 32; o > 0; o >>= 1 )
+            nr = max( nr, __shfl_xor( nr, o, 64 ) );
+        nr = __builtin_amdgcn_readfirstlane( nr );
+        const unsigned long long *row8 = reinterpret_cast<const unsigned long long *>(
+            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
+        auto body = [&]( unsigned off )
+        {
+            const double *p = reinterpret_cast<const double *>( s_byte + off );
+            const double dx = xi.x - p[0];
+            const double dy = xi.y - p[1];
+            const double dz = xi.z - p[2];
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const PairCoef c =
+                coef<ONE_TYPE>( Pd, one, ti, ONE_TYPE ? 0 : s_t[off / TILE_REC] );
+            double ep = 0.0, wp = 0.0;
+            const double fpair = lj_pair<PREC, ENERGY>( rsq, c, ep, wp );
+            if ( ENERGY )
+            {
+                pe += ( s_gh[off / TILE_REC] ? 0.5 : 1.0 ) * ep;
+                vir += wp;
+            }
+            if ( rsq < c.cs )
+            {
+                const double ax = dx * fpair, ay = dy * fpair, az = dz * fpair;
+                fx += ax;
+                fy += ay;
+                fz += az;
+                double *r = reinterpret_cast<double *>( s_acc + off );
+                unsafeAtomicAdd( r, -ax );
+                unsafeAtomicAdd( r + 1, -ay );
+                unsafeAtomicAdd( r + 2, -az );
+            }
+        };
+        for ( int r = 0; r < nr; r += 4 )
+        {
+            const unsigned long long wd = __builtin_nontemporal_load( row8 + (size_t)( r / 4 ) * 64 );
+            body( (unsigned)( wd & 0xffffull ) );
+            body( (unsigned)( ( wd >> 16 ) & 0xffffull ) );
+            body( (unsigned)( ( wd >> 32 ) & 0xffffull ) );
+            body( (unsigned)( wd >> 48 ) );
+        }
+#pragma unroll
+        for ( int o = APW; o < 64; o <<= 1 )
+        {
+            fx += __shfl_xor( fx, o, 64 );
+            fy += __shfl_xor( fy, o, 64 );
+            fz += __shfl_xor( fz, o, 64 );
+        }
+        if ( active && sub == 0 )
+        {
+            double *r = reinterpret_cast<double *>( s_acc + ( si0 + i ) * TILE_REC );
+            unsafeAtomicAdd( r, fx );
+            unsafeAtomicAdd( r + 1, fy );
+            unsafeAtomicAdd( r + 2, fz );
+        }
+    }
+    __syncthreads();
+    // every staged slot with a reaction: locals by run start, ghosts through
+    // the ghost sort permutation to their ghost rows
+    {
+        int seg = 0;
+        for ( int k = threadIdx.x; k < total; k += blockDim.x )
+        {
+            const double *r = reinterpret_cast<const double *>( s_acc + k * TILE_REC );
+            const double ax = r[0], ay = r[1], az = r[2];
+            while ( k >= H.segoff[seg + 1] )
+                ++seg;
+            if ( ax == 0.0 && ay == 0.0 && az == 0.0 )
+                continue;
+            const int q = seg >> 1, o = k - H.segoff[seg];
+            const int idx = ( seg & 1 ) ? n_local + gperm[H.info[TI_GB + q] + o]
+                                        : H.info[TI_LB + q] + o;
+            unsafeAtomicAdd( &f[idx], ax );
+            unsafeAtomicAdd( &f[scap + idx], ay );
+            unsafeAtomicAdd( &f[2 * scap + idx], az );
+        }
+    }
+    if ( ENERGY )
+        block_sum4( 0.0, pe, vir, 0.0, partial + 4 * blockIdx.x );
+}
+
 // Half list: each listed pair once; the reaction goes to j by atomics
 // (j may be a ghost, folded later by update_force).  Energy weights follow
 // compute_energy_half: 1 for a local j, 1/2 for a ghost j.
@@ -917,12 +1069,69 @@ static void launch_half_t( const cmd_force *F, cmd_system *S, const cmd_neigh *N
     }
 }
 
+template <int G, int PREC, bool ONE, bool ENERGY>
+static void launch_tile_half_k( const cmd_neigh *N, cmd_system *S, const LJParams *Pd,
+                                const PairCoef &c, double *partial )
+{
+    const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
+    const size_t lds = half_acc_offset( slot_cap, ONE ) + (size_t)slot_cap * TILE_REC +
+                       ( ENERGY ? (size_t)slot_cap * sizeof( int ) : 0 );
+    k_tile_half<G, PREC, ONE, ENERGY><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
+        S->n_local, S->x.p, S->xg.p, S->gsort.perm.p, S->f.p, N->counts.p,
+        N->tslab.p, Pd, c, partial );
+}
+
+template <int G, int PREC>
+static void launch_tile_half_t( const cmd_neigh *N, cmd_system *S, bool one, bool thermo,
+                                const LJParams *Pd, const PairCoef &c, double *partial )
+{
+    if ( one )
+    {
+        if ( thermo )
+            launch_tile_half_k<G, PREC, true, true>( N, S, Pd, c, partial );
+        else
+            launch_tile_half_k<G, PREC, true, false>( N, S, Pd, c, partial );
+    }
+    else
+    {
+        if ( thermo )
+            launch_tile_half_k<G, PREC, false, true>( N, S, Pd, c, partial );
+        else
+            launch_tile_half_k<G, PREC, false, false>( N, S, Pd, c, partial );
+    }
+}
+
+template <int PREC>
+static cmd_status launch_tile_half( const cmd_neigh *N, cmd_system *S, bool one,
+                                    bool thermo, const LJParams *Pd, const PairCoef &c,
+                                    double *partial )
+{
+    switch ( N->lanes )
+    {
+    case 2:
+        launch_tile_half_t<2, PREC>( N, S, one, thermo, Pd, c, partial );
+        break;
+    case 4:
+        launch_tile_half_t<4, PREC>( N, S, one, thermo, Pd, c, partial );
+        break;
+    case 8:
+        launch_tile_half_t<8, PREC>( N, S, one, thermo, Pd, c, partial );
+        break;
+    default:
+        set_error( "tiled lists need 2, 4 or 8 lanes per atom" );
+        return CMD_ERR_ARG;
+    }
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
 cmd_status lj_half( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
                     int thermo, double *partial, int *nblocks_out )
 {
     const int n = S->n_local;
     const int G = N->lanes;
-    const int nb = blocks_for( n, G );
+    const int nb = N->tiled ? N->tg.ntiles : blocks_for( n, G );
     if ( nblocks_out )
         *nblocks_out = nb;
     if ( n == 0 )
@@ -933,6 +1142,15 @@ cmd_status lj_half( const cmd_force *F, cmd_system *S, const cmd_neigh *N,
     LJParams *Pd = nullptr;
     if ( F->ntypes != 1 )
         CMD_TRY( upload_params( F, S->stream, &Pd ) );
+    if ( N->tiled )
+    {
+        CMD_REQUIRE( S->xg_valid, CMD_ERR_STATE,
+                     "tiled list needs the sorted ghosts of its build" );
+        const bool one = F->ntypes == 1;
+        if ( F->prec == CMD_PREC_FP64 )
+            return launch_tile_half<CMD_PREC_FP64>( N, S, one, thermo != 0, Pd, c, partial );
+        return launch_tile_half<CMD_PREC_MIXED>( N, S, one, thermo != 0, Pd, c, partial );
+    }
     switch ( G )
     {
     case 1:

@@ -542,6 +542,8 @@ cmd_status cmd_md_init( cmd_md *md )
         md->N->allow_tiles = false; // the NNP kernels read the CSR view
     else if ( md->lanes > 0 )
         CMD_TRY( cmd_neigh_set_lanes( md->N, md->lanes ) );
+    else if ( p.half )
+        CMD_TRY( cmd_neigh_set_lanes( md->N, 2 ) ); // tiled half list
     if ( !nnp && md->tile > 0 )
         CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force

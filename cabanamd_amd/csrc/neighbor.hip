@@ -319,7 +319,7 @@ cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
                 use_bin = false;
         }
     }
-    if ( use_bin && N->allow_tiles && !N->half && N->lanes >= 2 )
+    if ( use_bin && N->allow_tiles && N->lanes >= 2 )
     {
         if ( !S->xg_valid )
             CMD_TRY( tiles_prepare_ghosts( S ) );

@@ -277,6 +277,76 @@ def test_lj_tiled_force(gpu, prec, lanes, wrapped, tile):
     assert err2 < (1e-13 if prec == "fp64" else 1e-5), err2
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_lj_tiled_half_force(gpu, prec, lanes):
+    """Half list on the binning grid with 2+ lanes per atom: the tiled
+    kernel sums reactions per staged atom in LDS and adds them to f with
+    atomics; after the ghost fold, forces, energy and virial against the
+    oracle's half-list force and fold (force_lj_cabana_neigh_impl.h:205-259,
+    comm_mpi_impl.h:410-441)."""
+    M = gpu
+    ncells = (10, 9, 8)
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    O.wrap(x, O.create_domain([0, 0, 0], hi, gc).L)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
+    comm = M.Comm(s, 2.8)
+    comm.exchange_halo()
+    nl = M.NeighborVerlet(2.8, True, 50, lanes=lanes)
+    nl.create(s)
+    F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
+    F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
+    F.compute(s, nl)
+    comm.update_force()
+    xg, _, fg, tg, _ = s.download()
+    off, idx = nl.get()
+    n = len(x)
+    lj1, lj2, cs = O.lj_coeff(1, [(1, 1, 1.0, 1.0, 2.5)])
+    fo = O.lj_force(xg, tg, n, 1, lj1, lj2, cs, off, idx, True)
+    send, counts = comm.ghost_lists()
+    O.ghost_fold(fo, n, send, counts)
+    err = np.linalg.norm(fg[:n] - fo[:n]) / np.linalg.norm(fo[:n])
+    assert err < (1e-12 if prec == "fp64" else 1e-5), err
+    assert np.abs(fg[:n].sum(0)).max() < 1e-8
+    eo = O.lj_energy(xg, tg, n, 1, lj1, lj2, cs, off, idx, True)
+    wo = O.lj_virial(xg, tg, n, 1, lj1, lj2, cs, off, idx, True)
+    assert abs(F.compute_energy(s, nl) - eo) <= 1e-12 * abs(eo)
+    assert abs(F.compute_virial(s, nl) - wo) <= 1e-12 * abs(wo)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_md_loop_half_tiled(gpu, prec):
+    """CbnMD with a half list and two lanes per atom (the tiled half kernel,
+    its thermo energy with the reference's 1/2 weight for ghost pairs, and
+    the ghost fold) against the FP64 oracle's loop."""
+    M = gpu
+    kw = dict(ncells=(10, 10, 10), nsteps=100, thermo=10, half=True)
+    md = M.CbnMD(M.lattice_params(**kw),
+                 precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED,
+                 lanes=2)
+    md.init()
+    md.run(100)
+    ref = O.MD(O.lj_params(**kw))
+    ref.init()
+    ref.run(100)
+    th, tho = md.thermo(), ref.thermo()
+    assert th.shape[0] == tho.shape[0] == 11
+    x, v, f, _, _ = md.arrays()
+    xo, vo, fo, _, _ = ref.arrays()
+    n = md.n_local
+    if prec == "fp64":
+        assert np.allclose(th[:, :4], tho, rtol=1e-10, atol=1e-13)
+        assert np.abs(x[:n] - xo[:n]).max() < 1e-9
+        assert np.abs(v[:n] - vo[:n]).max() < 1e-9
+    else:
+        assert np.allclose(th[:, :4], tho, rtol=1e-5, atol=1e-7)
+        assert np.abs(x[:n] - xo[:n]).max() < 1e-5
+
+
 def _force_case(M, ncells, half, prec, lanes):
     _, x, v, t, i = _liquid_state(ncells)
     hi, gc = _domain_for(ncells)
