@@ -546,6 +546,8 @@ cmd_status cmd_md_init( cmd_md *md )
         CMD_TRY( cmd_neigh_set_lanes( md->N, 2 ) ); // tiled half list
     if ( !nnp && md->tile > 0 )
         CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
+    else if ( !nnp && p.half )
+        CMD_TRY( cmd_neigh_set_tile( md->N, 6 ) ); // measured best for half lists
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
     CMD_TRY( cmd_comm_exchange( S ) );
     CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
