@@ -249,6 +249,12 @@ class NeighborVerlet:
         return dict(n_rows=r.value, nnz=nnz.value, max_row=mr.value,
                     max_neigh_guess=g.value)
 
+    def layout(self):
+        """Layout of the last build: tiled (bool), tile height, lanes."""
+        t, e, g = C.c_int(), C.c_int(), C.c_int()
+        call("cmd_neigh_layout", self._h, C.byref(t), C.byref(e), C.byref(g))
+        return dict(tiled=bool(t.value), tile_height=e.value, lanes=g.value)
+
     @property
     def max_neigh_guess(self):
         return self.info()["max_neigh_guess"]

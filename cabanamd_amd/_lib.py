@@ -98,6 +98,7 @@ _PROTOS = {
     "cmd_neigh_set_tile": ([VP, C.c_int], None),
     "cmd_neigh_build": ([VP, VP], None),
     "cmd_neigh_info": ([VP, IP, LP, IP, IP], None),
+    "cmd_neigh_layout": ([VP, IP, IP, IP], None),
     "cmd_neigh_download": ([VP, LP, IP], None),
     "cmd_force_lj_create": ([C.c_int, C.POINTER(VP)], None),
     "cmd_force_destroy": ([VP], None),

@@ -514,6 +514,19 @@ cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
     return CMD_OK;
 }
 
+cmd_status cmd_neigh_layout( const cmd_neigh *n, int *tiled, int *tile_height,
+                             int *lanes )
+{
+    CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    if ( tiled )
+        *tiled = n->tiled ? 1 : 0;
+    if ( tile_height )
+        *tile_height = n->tiled ? n->tg.E : 0;
+    if ( lanes )
+        *lanes = n->lanes;
+    return CMD_OK;
+}
+
 cmd_status cmd_neigh_download( cmd_neigh *n, int64_t *offsets, int *idx )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );

@@ -160,6 +160,11 @@ cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height );
 cmd_status cmd_neigh_build( cmd_neigh *n, cmd_system *s );
 cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
                            int *max_row, int *max_neigh_guess );
+/* Layout of the last build (no reference counterpart; for tuning and
+ * tests): tiled = 1 for the LDS-tiled layout, with its tile height in
+ * cells; lanes per atom.  Any pointer may be NULL. */
+cmd_status cmd_neigh_layout( const cmd_neigh *n, int *tiled, int *tile_height,
+                             int *lanes );
 /* offsets: n_rows+1 int64; idx: nnz ints (either may be NULL). */
 cmd_status cmd_neigh_download( cmd_neigh *n, int64_t *offsets, int *idx );
 
