@@ -375,15 +375,23 @@ __device__ __forceinline__ AngK load_angk( const NnpElemPlan &Q, float *stage )
 }
 
 // Angular factors of the element's (lambda, zeta) grid for cos theta = cs:
-// Pw = 2^(1-zeta) (1 + lambda cs)^zeta and Dw = zeta lambda 2^(1-zeta)
-// (1 + lambda cs)^(zeta-1), the derivative of Pw with respect to cs, through
-// the hardware log2/exp2 (exp2(-inf) = 0); base^(zeta-1) = 1 for zeta = 1
-// also at base = 0 (collinear neighbours).
+// Pw = 2^(1-zeta) (1 + lambda cs)^zeta through the hardware log2/exp2
+// (exp2(-inf) = 0), and Dw = zeta lambda 2^(1-zeta) (1 + lambda cs)^(zeta-1),
+// the derivative of Pw with respect to cs, as zeta lambda Pw / base.  For
+// zeta = 1, Dw = lambda, also at base = 0 (collinear neighbours); for
+// zeta > 1 at base = 0 both are 0 (the reciprocal is taken as 0 there).
 template <bool WANT_D>
 __device__ __forceinline__ void ang_powers( const AngK &K, float cs, float *Pw, float *Dw )
 {
-    const float lbp = __builtin_amdgcn_logf( fmaxf( 1.0f + cs, 0.0f ) );
-    const float lbm = __builtin_amdgcn_logf( fmaxf( 1.0f - cs, 0.0f ) );
+    const float bp = fmaxf( 1.0f + cs, 0.0f ), bm = fmaxf( 1.0f - cs, 0.0f );
+    const float lbp = __builtin_amdgcn_logf( bp );
+    const float lbm = __builtin_amdgcn_logf( bm );
+    float ibp = 0.0f, ibm = 0.0f;
+    if ( WANT_D )
+    {
+        ibp = bp > 0.0f ? __builtin_amdgcn_rcpf( bp ) : 0.0f;
+        ibm = bm > 0.0f ? __builtin_amdgcn_rcpf( bm ) : 0.0f;
+    }
 #pragma unroll
     for ( int pp = 0; pp < AP_MAX; ++pp )
     {
@@ -392,12 +400,11 @@ __device__ __forceinline__ void ang_powers( const AngK &K, float cs, float *Pw, 
             Dw[pp] = 0.0f;
         if ( pp < K.nap )
         {
-            const float lb = ( K.lampos >> pp ) & 1u ? lbp : lbm;
+            const bool pos = ( K.lampos >> pp ) & 1u;
+            const float lb = pos ? lbp : lbm;
             Pw[pp] = __builtin_amdgcn_exp2f( K.zeta[pp] * lb + K.lpref[pp] );
             if ( WANT_D )
-                Dw[pp] = K.zl[pp] * __builtin_amdgcn_exp2f(
-                                        ( K.zm1[pp] != 0.f ? K.zm1[pp] * lb : 0.f ) +
-                                        K.lpref[pp] );
+                Dw[pp] = K.zm1[pp] != 0.f ? K.zl[pp] * Pw[pp] * ( pos ? ibp : ibm ) : K.zl[pp];
         }
     }
 }
@@ -521,19 +528,48 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
                                              unsigned *buf, Body body )
 {
     const int lane = threadIdx.x & 63;
+    // candidate pairs, 64 per test: with n <= 64 rows a and n - 2 - a side
+    // by side (n - 1 - a and a + 1 entries: n in all), else the flat pair
+    // index through pair_of
+    const bool rows = n <= 64;
     const int np = n * ( n - 1 ) / 2;
-    for ( int p0 = 0; p0 < np; )
+    const int nit = rows ? n / 2 : ( np + 63 ) / 64; // rows: pairs of rows
+    for ( int it = 0; it < nit; )
     {
         int cnt = 0;
-        while ( cnt <= TCAP - 64 && p0 < np )
+        while ( cnt <= TCAP - 64 && it < nit )
         {
-            const int p = p0 + lane;
             bool ok = false;
             unsigned code = 0u;
-            if ( p < np )
+            int a = 0, q = 0;
+            bool have;
+            if ( rows )
             {
-                int a, q;
-                pair_of( p, n, a, q );
+                // rows a1 < a2 hold n - 1 - a1 and n - 1 - a2 = a1 + 1
+                // entries; the middle row of an even n comes alone
+                const int a1 = it, a2 = n - 2 - it;
+                const int len1 = n - 1 - a1, len2 = a2 > a1 ? n - 1 - a2 : 0;
+                have = lane < len1 + len2;
+                if ( lane < len1 )
+                {
+                    a = a1;
+                    q = a1 + 1 + lane;
+                }
+                else
+                {
+                    a = a2;
+                    q = a2 + 1 + ( lane - len1 );
+                }
+            }
+            else
+            {
+                const int p = it * 64 + lane;
+                have = p < np;
+                if ( have )
+                    pair_of( p, n, a, q );
+            }
+            if ( have )
+            {
                 const float4 A = L.d[a], B = L.d[q];
                 const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
                 ok = cx * cx + cy * cy + cz * cz < rc2;
@@ -545,7 +581,7 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
                               (unsigned)( m >> 32 ),
                               __builtin_amdgcn_mbcnt_lo( (unsigned)m, 0u ) )] = code;
             cnt += (int)__popcll( m );
-            p0 += 64;
+            ++it;
         }
         __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
         const int per = ( cnt + 63 ) >> 6;
