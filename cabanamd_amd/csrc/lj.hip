@@ -438,19 +438,23 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     const int nw = woff[t + 1] - woff[t];
     const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int a = lane % APW, sub = lane / APW;
+    // four sentinel entries: the word a lane past the tile's atoms walks
+    const unsigned long long sent4 = 0x0001000100010001ull * (unsigned)( H.segoff[18] * TILE_REC );
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
     for ( int w = wid; w < nw; w += TPB / 64 )
     {
+        // gw lanes per atom in this slab wave (G, or more in a part-full wave)
+        const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
+        const int a = lane % apw, sub = lane / apw;
         const int ta = w * APW + a;
-        const bool active = ta < nat;
+        const bool active = a < APW && ta < nat;
         const int i = tl + ta;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const double4 xi =
             active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
         const int ti = (int)xi.w;
         const int n = active ? counts[i] : 0;
-        int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
+        int nr = n > sub ? ( n - sub + gw - 1 ) / gw : 0;
 #pragma unroll
         for ( int o = 32; o > 0; o >>= 1 )
             nr = max( nr, __shfl_xor( nr, o, 64 ) );
@@ -495,20 +499,24 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             }
         }
         // list words three iterations ahead: under load the memory latency
-        // spans several iterations of all waves of a SIMD
-        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull;
-        if ( nr > 0 )
-            q0 = __builtin_nontemporal_load( row8 );
-        if ( nr > 4 )
-            q1 = __builtin_nontemporal_load( row8 + 64 );
-        if ( nr > 8 )
-            q2 = __builtin_nontemporal_load( row8 + 128 );
+        // spans several iterations of all waves of a SIMD.  Lanes past the
+        // tile's atoms read nothing and walk the sentinel record.
+        unsigned long long q0 = sent4, q1 = sent4, q2 = sent4;
+        if ( active )
+        {
+            if ( nr > 0 )
+                q0 = __builtin_nontemporal_load( row8 );
+            if ( nr > 4 )
+                q1 = __builtin_nontemporal_load( row8 + 64 );
+            if ( nr > 8 )
+                q2 = __builtin_nontemporal_load( row8 + 128 );
+        }
         for ( int r = 0; r < nr; r += 4 )
         {
             const unsigned long long cur = q0;
             q0 = q1;
             q1 = q2;
-            if ( r + 12 < nr )
+            if ( active && r + 12 < nr )
                 q2 = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 3 ) * 64 );
             const double4 p0 = fetch( cur, 0 );
             const double4 p1 = fetch( cur, 1 );
@@ -519,8 +527,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             body( p2 );
             body( p3 );
         }
-#pragma unroll
-        for ( int o = APW; o < 64; o <<= 1 )
+        for ( int o = apw; o < 64; o <<= 1 )
         {
             fx += __shfl_xor( fx, o, 64 );
             fy += __shfl_xor( fy, o, 64 );
@@ -590,19 +597,22 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
     const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
     const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int a = lane % APW, sub = lane / APW;
+    const unsigned long long sent4 = 0x0001000100010001ull * (unsigned)( H.segoff[18] * TILE_REC );
     double pe = 0.0, vir = 0.0;
     for ( int w = wid; w < nw; w += TPB / 64 )
     {
+        // gw lanes per atom in this slab wave (G, or more in a part-full wave)
+        const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
+        const int a = lane % apw, sub = lane / apw;
         const int ta = w * APW + a;
-        const bool active = ta < nat;
+        const bool active = a < APW && ta < nat;
         const int i = tl + ta;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const double4 xi =
             active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
         const int ti = (int)xi.w;
         const int n = active ? counts[i] : 0;
-        int nr = n > sub ? ( n - sub + G - 1 ) / G : 0;
+        int nr = n > sub ? ( n - sub + gw - 1 ) / gw : 0;
 #pragma unroll
         for ( int o = 32; o > 0; o >>= 1 )
             nr = max( nr, __shfl_xor( nr, o, 64 ) );
@@ -639,14 +649,15 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
         };
         for ( int r = 0; r < nr; r += 4 )
         {
-            const unsigned long long wd = __builtin_nontemporal_load( row8 + (size_t)( r / 4 ) * 64 );
+            // lanes past the tile's atoms read nothing and walk the sentinel
+            const unsigned long long wd =
+                active ? __builtin_nontemporal_load( row8 + (size_t)( r / 4 ) * 64 ) : sent4;
             body( (unsigned)( wd & 0xffffull ) );
             body( (unsigned)( ( wd >> 16 ) & 0xffffull ) );
             body( (unsigned)( ( wd >> 32 ) & 0xffffull ) );
             body( (unsigned)( wd >> 48 ) );
         }
-#pragma unroll
-        for ( int o = APW; o < 64; o <<= 1 )
+        for ( int o = apw; o < 64; o <<= 1 )
         {
             fx += __shfl_xor( fx, o, 64 );
             fy += __shfl_xor( fy, o, 64 );

@@ -16,9 +16,10 @@
 // the order the oracle uses (stencil columns, then cells, then atoms by
 // index), so lists and FP64 sums are unchanged.
 //
-// Slab of a tile: waves of APW = 64/G atoms; neighbour k of atom a of wave w
-// goes to row r = k / G, lane (k % G) * APW + a, and four consecutive rows of
-// a lane are packed in 8 bytes:
+// Slab of a tile: waves of APW = 64/G atoms; with gw lanes per atom in wave
+// w (G, or up to 8 in a part-full wave: wave_lanes), neighbour k of atom a
+// of the wave goes to row r = k / gw, lane (k % gw) * (64 / gw) + a, and
+// four consecutive rows of a lane are packed in 8 bytes:
 //   tslab[tile_base + ((w * R4 + r / 4) * 64 + lane) * 4 + r % 4],
 // R4 = ceil(ceil(cap / G) / 4), so one 8-byte load per lane feeds four pairs.
 #include "internal.h"
@@ -409,19 +410,32 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
     }
     __syncthreads();
     {
-        // slab layout: per slab wave, R4 groups of 64 eight-byte words; word
-        // (group g4, lane L) holds entries (4 g4 + j) G + L / APW, j = 0..3,
-        // of the wave's atom L % APW
-        const int nwords = nw * R4 * 64;
+        // slab layout: per slab wave (gw = wave_lanes lanes per atom, 64 / gw
+        // atoms), R4 groups of 64 eight-byte words; word (group g4, lane L)
+        // holds entries (4 g4 + j) gw + L / (64 / gw), j = 0..3, of the
+        // wave's atom L % (64 / gw).  Words of lanes past the tile's atoms
+        // are never read and not written; entries at or past cap are the
+        // sentinel.
+        const int nwords = nw * R4 * 64, nat = H.info[TI_NAT];
+        const unsigned short sent = (unsigned short)( sent_slot * TILE_REC );
         uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)woff[t] * wsz );
         for ( int w = threadIdx.x; w < nwords; w += blockDim.x )
         {
             const int sw = w / ( R4 * 64 ), rem = w - sw * ( R4 * 64 );
             const int g4 = rem >> 6, L = rem & 63;
-            const unsigned short *src =
-                chunk + ( sw * APW + L % APW ) * RS + g4 * 4 * G + L / APW;
-            dst[w] = make_uint2( (unsigned)src[0] | ( (unsigned)src[G] << 16 ),
-                                 (unsigned)src[2 * G] | ( (unsigned)src[3 * G] << 16 ) );
+            const int gw = wave_lanes( G, nat, sw ), apw = 64 / gw;
+            const int ta = sw * APW + L % apw;
+            if ( L % apw >= min( APW, nat - sw * APW ) )
+                continue;
+            const unsigned short *row = chunk + ta * RS;
+            unsigned e4[4];
+#pragma unroll
+            for ( int j = 0; j < 4; ++j )
+            {
+                const int e = ( 4 * g4 + j ) * gw + L / apw;
+                e4[j] = e < cap ? row[e] : sent;
+            }
+            dst[w] = make_uint2( e4[0] | ( e4[1] << 16 ), e4[2] | ( e4[3] << 16 ) );
         }
     }
 #pragma unroll
@@ -451,11 +465,12 @@ __global__ __launch_bounds__( TPB ) void k_tile_to_csr(
     {
         const int i = tl + ta;
         const int w = ta / APW, aw = ta % APW;
+        const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
         const int n = counts[i];
         const int64_t o = offsets[i];
         for ( int k = 0; k < n; ++k )
         {
-            const int r = k / G, lane = ( k % G ) * APW + aw;
+            const int r = k / gw, lane = ( k % gw ) * apw + aw;
             const int sj =
                 st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )] / TILE_REC;
             int seg = 0;

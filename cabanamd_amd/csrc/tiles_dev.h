@@ -85,6 +85,19 @@ static constexpr double TILE_FAR = 1.0e30;
 // offset inside 16 bits.
 static constexpr int TILE_REC = 24;
 
+// Lanes per atom of slab wave w of a tile with nat atoms, for tiled lists
+// built with G lanes per atom (64 / G atoms per full wave): G, doubled up to
+// 8 while the wave's atoms still fill the 64 lanes.  A tile's last wave is
+// usually part full; spreading its rows over more lanes shortens its walk.
+__device__ __forceinline__ int wave_lanes( int G, int nat, int w )
+{
+    const int na = min( 64 / G, nat - w * ( 64 / G ) );
+    int g = G;
+    while ( g < 8 && na * g * 2 <= 64 )
+        g *= 2;
+    return g;
+}
+
 // Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS as
 // 24-byte records, followed by the sentinel record; with WITH_TYPE the types
 // follow as ints, `cap` records on.
