@@ -753,13 +753,15 @@ class Batch:
     thermo_rate); systems join with add() and leave with remove() at the
     first run() that starts on a rebuild step."""
 
-    def __init__(self, params, precision=PREC_FP64, lanes=0, nnp_model=None):
+    def __init__(self, params, precision=PREC_FP64, lanes=0, nnp_model=None,
+                 graphs=True):
         self.params = params
         h = C.c_void_p()
         call("cmd_batch_create", C.byref(params), C.byref(h))
         self._h = h
         call("cmd_batch_set_precision", h, precision)
         call("cmd_batch_set_lanes", h, lanes)
+        call("cmd_batch_set_graphs", h, int(bool(graphs)))
         self._model = nnp_model
         if nnp_model is not None:
             call("cmd_batch_set_nnp_model", h, nnp_model.handle)
@@ -786,6 +788,12 @@ class Batch:
 
     def run(self, nsteps):
         call("cmd_batch_run", self._h, int(nsteps))
+
+    def graph_steps(self):
+        """Steps run by replays of the captured plain-step graph."""
+        n = C.c_longlong()
+        call("cmd_batch_graph_steps", self._h, C.byref(n))
+        return n.value
 
     def counts(self):
         a, c, d = C.c_int(), C.c_int(), C.c_int()

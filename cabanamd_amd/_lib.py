@@ -167,6 +167,8 @@ _PROTOS = {
     "cmd_batch_destroy": ([VP], None),
     "cmd_batch_set_precision": ([VP, C.c_int], None),
     "cmd_batch_set_lanes": ([VP, C.c_int], None),
+    "cmd_batch_set_graphs": ([VP, C.c_int], None),
+    "cmd_batch_graph_steps": ([VP, C.POINTER(C.c_longlong)], None),
     "cmd_batch_add": ([VP, C.c_int, DP, DP, IP, IP, DP, DP, IP], None),
     "cmd_batch_remove": ([VP, C.c_int], None),
     "cmd_batch_run": ([VP, C.c_int], None),

@@ -404,6 +404,21 @@ struct cmd_batch
     std::vector<double> hsums;
     int ncell = 0;
     double timers[4] = { 0, 0, 0, 0 }; // rebuild, steps, thermo, join/leave
+    // Two plain LJ steps (ghost refresh + fused step, into x_alt and back)
+    // captured as one HIP graph, one per position buffer the pair starts
+    // from (an odd number of eager steps between replays swaps them); a
+    // graph is valid while its epoch (bumped at every rebuild, where
+    // buffers, counts and lists change) is the batch's.
+    struct PairGraph
+    {
+        hipGraphExec_t exec = nullptr;
+        const void *x = nullptr; // S->x.p at capture
+        int epoch = -1;
+    };
+    bool graphs = true;
+    PairGraph pg[2];
+    int epoch = 0;
+    long long graph_steps = 0; // steps run by graph replays
 };
 
 namespace
@@ -523,6 +538,7 @@ cmd_status build_list( cmd_batch *B, int G )
 // atoms, ghost images, cell sort of all atoms, Verlet list.
 cmd_status rebuild( cmd_batch *B, int G )
 {
+    ++B->epoch; // captured steps hold this build's buffers and sizes
     const double t0 = now_s();
     cmd_system *S = B->S;
     const hipStream_t st = B->stream;
@@ -717,6 +733,7 @@ cmd_status autotune( cmd_batch *B )
 // bit-identical to what they had.
 cmd_status apply_changes( cmd_batch *B )
 {
+    ++B->epoch; // membership changes move every buffer
     if ( B->pending.empty() && B->leaving.empty() )
         return CMD_OK;
     const double t0 = now_s();
@@ -890,6 +907,9 @@ cmd_status cmd_batch_destroy( cmd_batch *b )
         return CMD_OK;
     if ( b->S && b->S->stream )
         (void)hipStreamSynchronize( b->S->stream );
+    for ( auto &g : b->pg )
+        if ( g.exec )
+            (void)hipGraphExecDestroy( g.exec );
     if ( b->F )
         cmd_force_destroy( b->F );
     if ( b->N )
@@ -924,6 +944,20 @@ cmd_status cmd_batch_set_nnp_model( cmd_batch *b, const cmd_nnp_model *m )
     }
     CMD_TRY( cmd_force_nnp_create( m, &b->F ) );
     return cmd_force_set_precision( b->F, b->prec );
+}
+
+cmd_status cmd_batch_set_graphs( cmd_batch *b, int on )
+{
+    CMD_REQUIRE( b, CMD_ERR_ARG, "batch is NULL" );
+    b->graphs = on != 0;
+    return CMD_OK;
+}
+
+cmd_status cmd_batch_graph_steps( const cmd_batch *b, long long *steps )
+{
+    CMD_REQUIRE( b && steps, CMD_ERR_ARG, "NULL argument" );
+    *steps = b->graph_steps;
+    return CMD_OK;
 }
 
 cmd_status cmd_batch_set_lanes( cmd_batch *b, int lanes )
@@ -997,6 +1031,46 @@ cmd_status cmd_batch_remove( cmd_batch *b, int handle )
 // Systems added or removed since the last call join or leave now when the
 // batch sits on a rebuild boundary (else at the first call that starts on
 // one).  Thermo rows are taken every thermo_rate steps and at the last step.
+// Capture two plain steps (refresh ghosts, fused step into x_alt; the
+// same from x_alt back into x) as one graph.  The capture runs nothing; the
+// host-side buffer swaps it makes are undone by the second.  Only kernels
+// are recorded (no copies, fills or host waits).
+static cmd_status capture_plain_pair( cmd_batch *b, const StepCtl &ctl,
+                                      cmd_batch::PairGraph &pg )
+{
+    cmd_system *S = b->S;
+    if ( pg.exec )
+    {
+        CMD_HIP( hipGraphExecDestroy( pg.exec ) );
+        pg.exec = nullptr;
+    }
+    pg.x = S->x.p;
+    CMD_HIP( hipStreamBeginCapture( b->stream, hipStreamCaptureModeThreadLocal ) );
+    cmd_status rc = CMD_OK;
+    for ( int k = 0; k < 2; ++k )
+    {
+        if ( rc == CMD_OK )
+            rc = refresh( b );
+        if ( rc == CMD_OK )
+            rc = lj_full( b->F, S, b->N, ctl, S->x_alt.p, nullptr, nullptr );
+        S->x.swap( S->x_alt );
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture( b->stream, &g );
+    if ( rc != CMD_OK )
+    {
+        if ( e == hipSuccess && g )
+            (void)hipGraphDestroy( g );
+        return rc;
+    }
+    CMD_HIP( e );
+    const hipError_t ei = hipGraphInstantiate( &pg.exec, g, nullptr, nullptr, 0 );
+    (void)hipGraphDestroy( g );
+    CMD_HIP( ei );
+    pg.epoch = b->epoch;
+    return CMD_OK;
+}
+
 cmd_status cmd_batch_run( cmd_batch *b, int nsteps )
 {
     CMD_REQUIRE( b && nsteps >= 0, CMD_ERR_ARG, "bad argument" );
@@ -1044,9 +1118,45 @@ cmd_status cmd_batch_run( cmd_batch *b, int nsteps )
     CMD_TRY( cmd_nve_initial( S ) );
     if ( s0 % p.every == 0 )
         CMD_TRY( wrap( b, S->x.p ) );
+    // a plain step: no rebuild, no thermo row, no wrap for the next rebuild,
+    // not the last of the call
+    auto plain = [&]( int t )
+    {
+        return t < s1 && t % p.every != 0 && ( t + 1 ) % p.every != 0 &&
+               !( p.thermo_rate > 0 && t % p.thermo_rate == 0 );
+    };
+    // parameters uploaded per launch for several types would be a host copy
+    const bool graphs = b->graphs && b->F->ntypes == 1;
     for ( int s = s0; s <= s1; ++s )
     {
         b->step = s;
+        if ( graphs && plain( s ) && plain( s + 1 ) )
+        {
+            StepCtl gc = ctl;
+            gc.do_final = 1;
+            gc.do_next_initial = 1;
+            gc.write_f = 0;
+            gc.thermo = 0;
+            gc.wrap_next = 0;
+            // the graph of the buffer the pair starts from; else capture
+            // over a slot of another epoch, or the other buffer's
+            cmd_batch::PairGraph *use = nullptr;
+            for ( auto &g : b->pg )
+                if ( g.exec && g.epoch == b->epoch && g.x == S->x.p )
+                    use = &g;
+            if ( !use )
+            {
+                use = &b->pg[0];
+                for ( auto &g : b->pg )
+                    if ( !g.exec || g.epoch != b->epoch )
+                        use = &g;
+                CMD_TRY( capture_plain_pair( b, gc, *use ) );
+            }
+            CMD_HIP( hipGraphLaunch( use->exec, st ) );
+            b->graph_steps += 2;
+            b->step = ++s; // the pair ends with x back in place
+            continue;
+        }
         if ( s % p.every == 0 )
             CMD_TRY( rebuild( b, G ) );
         else

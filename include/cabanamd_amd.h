@@ -400,6 +400,13 @@ cmd_status cmd_batch_set_nnp_model( cmd_batch *b, const cmd_nnp_model *m );
 /* Lanes per atom of the list layout (1, 2, 4, 8); 0 = time each layout on
  * the first systems and keep the fastest.  Fixed once systems run. */
 cmd_status cmd_batch_set_lanes( cmd_batch *b, int lanes );
+/* HIP graphs for the plain LJ steps between rebuilds (on by default): two
+ * consecutive steps (ghost refresh and the fused step, each way of the
+ * position ping-pong) are captured once per rebuild and replayed.  No
+ * reference counterpart; off runs the same kernels launched one by one.
+ * cmd_batch_graph_steps counts the steps that graph replays ran. */
+cmd_status cmd_batch_set_graphs( cmd_batch *b, int on );
+cmd_status cmd_batch_graph_steps( const cmd_batch *b, long long *steps );
 /* Queue a system: n atoms (x, v as n x 3; v may be NULL), types 0-based,
  * ids (NULL = 1..n) and its box [lo, hi), each side longer than twice the
  * ghost depth (cutoff + skin).  *handle names it in later calls. */

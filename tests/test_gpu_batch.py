@@ -147,3 +147,38 @@ def test_batch_autotuned_lanes_run(gpu):
         th = B.thermo(h)
         e = th[:, 3]
         assert np.abs(e - e[0]).max() < 1e-3 * abs(e[0])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_graph_replays_equal_eager(gpu, prec):
+    """The plain steps between rebuilds run as replays of a captured graph
+    (two steps per replay, captured again after every rebuild).  Runs of
+    uneven length, a system joining and one leaving, and host reads between
+    the runs: every system's x, v, f and thermo rows are bit-identical to the
+    same batch launched kernel by kernel, after every run."""
+    M = gpu
+    P = M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED
+    g = M.Batch(_params(M, thermo=10), P, lanes=2, graphs=True)
+    e = M.Batch(_params(M, thermo=10), P, lanes=2, graphs=False)
+    hg = [_add(M, g, s) for s in SYSTEMS[:3]]
+    he = [_add(M, e, s) for s in SYSTEMS[:3]]
+    for k, change in ((27, None), (13, "join"), (40, "leave"), (23, None)):
+        if change == "join":
+            hg.append(_add(M, g, SYSTEMS[3]))
+            he.append(_add(M, e, SYSTEMS[3]))
+        elif change == "leave":
+            g.remove(hg.pop(0))
+            e.remove(he.pop(0))
+        g.run(k)
+        e.run(k)
+        for a, b in zip(hg, he):
+            # a join takes effect at the next run that starts on a rebuild
+            assert g.info(a)["state"] == e.info(b)["state"]
+            if g.info(a)["state"] != "running":
+                continue
+            for u, w in zip(g.download(a), e.download(b)):
+                assert np.array_equal(u, w)
+            assert np.array_equal(g.thermo(a), e.thermo(b))
+    assert all(g.info(a)["state"] == "running" for a in hg)
+    assert g.graph_steps() > 40
+    assert e.graph_steps() == 0
