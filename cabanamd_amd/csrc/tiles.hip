@@ -556,7 +556,7 @@ cmd_status tiles_ghost_refresh( cmd_system *S )
     return CMD_OK;
 }
 
-static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[3] )
+static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[4] )
 {
     TileGrid &T = N->tg;
     T.g = S->bin_grid;
@@ -580,18 +580,18 @@ static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[3] )
     CMD_LAUNCH_CHECK();
     CMD_TRY( scan_exclusive( waves.p, N->grid_sort.start.p, T.ntiles, S->sc,
                              S->stream, nullptr ) );
+    // h[0..2] from the info kernel, h[3] = slab waves of all tiles (the
+    // scan's total), in one wait
     CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, 3 * sizeof( int ),
+                             hipMemcpyDeviceToHost, S->stream ) );
+    CMD_HIP( hipMemcpyAsync( h + 3, N->grid_sort.start.p + T.ntiles, sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
     return CMD_OK;
 }
 
-static size_t tile_slab_entries( cmd_neigh *N, hipStream_t st )
+static size_t tile_slab_entries( const cmd_neigh *N, int nw )
 {
-    int nw = 0;
-    (void)hipMemcpyAsync( &nw, N->grid_sort.start.p + N->tg.ntiles, sizeof( int ),
-                          hipMemcpyDeviceToHost, st );
-    (void)hipStreamSynchronize( st );
     const int G = N->lanes;
     const size_t R4 = (size_t)( ( ( N->cap + G - 1 ) / G + 3 ) / 4 );
     return std::max<size_t>( (size_t)nw * R4 * 256, 1 );
@@ -646,7 +646,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
     // tile height: the requested one, or the tallest whose neighbourhood and
     // rows fit in LDS
     int E = N->tile_e > 0 ? N->tile_e : 4;
-    int h[3] = { 0, 0, 0 };
+    int h[4] = { 0, 0, 0, 0 };
     for ( int attempt = 0; attempt < 4; ++attempt )
     {
         size_t lds = 0;
@@ -672,7 +672,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             E = std::max( 1, E * 2 / 3 );
         }
         N->max_tile_slots = h[0];
-        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, S->stream ) ) );
+        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[3] ) ) );
         CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, S->stream ) );
         const int slot_cap = h[0] + 1;
         switch ( G )
