@@ -205,63 +205,146 @@ __global__ void k_b_gather( int n, const int *__restrict__ perm, const double4 *
     xs[k] = p;
 }
 
-// Full Verlet list of owned atom i over its system's 27-cell stencil
+// Full Verlet list of the owned atoms over their system's 27-cell stencil
 // (neighbor_verlet.h:43-62; accept j != i and rsq <= rc^2), written in the
 // lane-interleaved slab layout of the single-system engine (neighbor.hip):
 // entry k of atom a of wave w at ((w R + k / G) 64) + (k % G) APW + a.
+// One wave per cell (cells of every system in one space): the cell's owned
+// atoms share its stencil, which is at most 9 contiguous runs of the
+// cell-sorted candidates (one per column, cells e-1 .. e+1, columns in
+// (x, y) order).  The wave loads the candidates once into registers, CHB
+// chunks of 64 at a time (lane = candidate), and tests the cell's atoms one
+// by one (position and row length wave-uniform); ballot + mbcnt place the
+// accepted candidates in each row in walk order.
+static constexpr int CHB = 8;
+
 template <int G>
-__global__ __launch_bounds__( TPB ) void k_b_neigh( int n_local, const double4 *__restrict__ x,
-                                                    const double4 *__restrict__ xs,
-                                                    const int *__restrict__ start,
-                                                    const int *__restrict__ sys,
-                                                    const BSys *__restrict__ T, double rc2,
-                                                    int cap, int *__restrict__ counts,
-                                                    int *__restrict__ slab,
-                                                    int *__restrict__ max_row )
+__global__ __launch_bounds__( 256 ) void k_b_neigh_cell(
+    int ncell, const double4 *__restrict__ x, const double4 *__restrict__ xs,
+    const int *__restrict__ lstart, const int *__restrict__ astart,
+    const int *__restrict__ sys, const BSys *__restrict__ T, double rc2, int cap,
+    int *__restrict__ counts, int *__restrict__ slab, int *__restrict__ max_row )
 {
     constexpr int APW = 64 / G;
-    const int i = blockIdx.x * TPB + (int)threadIdx.x;
-    int cnt = 0;
-    if ( i < n_local )
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * ( blockDim.x >> 6 ) + ( threadIdx.x >> 6 );
+    int mrow = 0;
+    const int c0 = c < ncell ? __builtin_amdgcn_readfirstlane( lstart[c] ) : 0;
+    const int c1 = c < ncell ? __builtin_amdgcn_readfirstlane( lstart[c + 1] ) : 0;
+    if ( c1 > c0 )
     {
-        const BSys &s = T[sys[i]];
-        const double4 pi = x[i];
-        const int a = cell_coord( pi.x, s.gmin[0], s.grd[0], s.gn[0], 1 );
-        const int b = cell_coord( pi.y, s.gmin[1], s.grd[1], s.gn[1], 1 );
-        const int e = cell_coord( pi.z, s.gmin[2], s.grd[2], s.gn[2], 1 );
-        const int e0 = e > 0 ? e - 1 : 0;
-        const int e1 = e < s.gn[2] - 1 ? e + 1 : s.gn[2] - 1;
-        const int R = ( cap + G - 1 ) / G;
-        int *out = slab + (size_t)( i / APW ) * R * 64 + i % APW;
-        for ( int aa = max( a - 1, 0 ); aa <= min( a + 1, s.gn[0] - 1 ); ++aa )
-            for ( int bb = max( b - 1, 0 ); bb <= min( b + 1, s.gn[1] - 1 ); ++bb )
+        const BSys &s = T[__builtin_amdgcn_readfirstlane( sys[c0] )];
+        const int lc = c - s.cell_base;
+        const int a = lc / ( s.gn[1] * s.gn[2] ), b = ( lc / s.gn[2] ) % s.gn[1],
+                  e = lc % s.gn[2];
+        const int e0 = e > 0 ? e - 1 : 0, e1 = e < s.gn[2] - 1 ? e + 1 : s.gn[2] - 1;
+        // column q = lane (< 9): its run [rs, rs + len) of candidates
+        int rs = 0, len = 0;
+        if ( lane < 9 )
+        {
+            const int aa = a + lane / 3 - 1, bb = b + lane % 3 - 1;
+            if ( aa >= 0 && aa < s.gn[0] && bb >= 0 && bb < s.gn[1] )
             {
                 const int base = s.cell_base + ( aa * s.gn[1] + bb ) * s.gn[2];
-                const int s1 = start[base + e1 + 1];
-                for ( int q = start[base + e0]; q < s1; ++q )
+                rs = astart[base + e0];
+                len = astart[base + e1 + 1] - rs;
+            }
+        }
+        int inc = len;
+#pragma unroll
+        for ( int o = 1; o < 16; o <<= 1 )
+        {
+            const int u = __shfl_up( inc, o, 64 );
+            if ( lane >= o )
+                inc += u;
+        }
+        // run q starts at candidate P[q] of the walk; candidate v of run q is
+        // xs[v + D[q]] with D[q] = rs_q - P[q]
+        int P[9], D[9];
+#pragma unroll
+        for ( int q = 0; q < 9; ++q )
+        {
+            const int iq = __builtin_amdgcn_readlane( inc, q );
+            const int lq = __builtin_amdgcn_readlane( len, q );
+            P[q] = iq - lq;
+            D[q] = __builtin_amdgcn_readlane( rs, q ) - P[q];
+        }
+        const int total = __builtin_amdgcn_readlane( inc, 8 );
+        const int R = ( cap + G - 1 ) / G;
+        for ( int a0 = c0; a0 < c1; a0 += 64 )
+        {
+            const int na = min( 64, c1 - a0 );
+            int cnt = 0; // row length of atom a0 + lane
+            for ( int v0 = 0; v0 < total; v0 += CHB * 64 )
+            {
+                double px[CHB], py[CHB], pz[CHB];
+                int pj[CHB];
+#pragma unroll
+                for ( int k = 0; k < CHB; ++k )
                 {
-                    const double4 pj = xs[q];
-                    const int j = (int)pj.w;
-                    const double dx = pi.x - pj.x;
-                    const double dy = pi.y - pj.y;
-                    const double dz = pi.z - pj.z;
-                    const double rsq = dx * dx + dy * dy + dz * dz;
-                    if ( j != i && rsq <= rc2 )
+                    const int v = v0 + k * 64 + lane;
+                    px[k] = 1.0e30;
+                    py[k] = 1.0e30;
+                    pz[k] = 1.0e30;
+                    pj[k] = -1;
+                    if ( v < total )
                     {
-                        if ( cnt < cap )
-                            out[( cnt / G ) * 64 + ( cnt % G ) * APW] = j;
-                        ++cnt;
+                        int d = D[0];
+#pragma unroll
+                        for ( int q = 1; q < 9; ++q )
+                            d = v >= P[q] ? D[q] : d;
+                        const double4 p = xs[v + d];
+                        px[k] = p.x;
+                        py[k] = p.y;
+                        pz[k] = p.z;
+                        pj[k] = (int)p.w;
                     }
                 }
+                const int nch = min( CHB, ( total - v0 + 63 ) >> 6 );
+                double4 pn = x[a0];
+                for ( int l = 0; l < na; ++l )
+                {
+                    const double4 pa = pn;
+                    pn = x[a0 + min( l + 1, na - 1 )];
+                    const int i = a0 + l;
+                    int *out = slab + (size_t)( i / APW ) * R * 64 + i % APW;
+                    unsigned k0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
+#pragma unroll
+                    for ( int k = 0; k < CHB; ++k )
+                    {
+                        if ( k < nch )
+                        {
+                            const double dx = pa.x - px[k];
+                            const double dy = pa.y - py[k];
+                            const double dz = pa.z - pz[k];
+                            const double rsq = dx * dx + dy * dy + dz * dz;
+                            const unsigned long long m =
+                                __builtin_amdgcn_ballot_w64( rsq <= rc2 ) &
+                                __builtin_amdgcn_ballot_w64( pj[k] != i );
+                            const unsigned kk = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)( m >> 32 ),
+                                __builtin_amdgcn_mbcnt_lo( (unsigned)m, k0 ) );
+                            if ( __builtin_amdgcn_inverse_ballot_w64( m ) && kk < (unsigned)cap )
+                                out[( kk / G ) * 64 + ( kk % G ) * APW] = pj[k];
+                            k0 += (unsigned)__popcll( m );
+                        }
+                    }
+                    if ( lane == l )
+                        cnt = (int)k0;
+                }
             }
-        counts[i] = cnt;
+            if ( lane < na )
+            {
+                counts[a0 + lane] = cnt;
+                mrow = max( mrow, cnt );
+            }
+        }
     }
-    int m = cnt;
 #pragma unroll
     for ( int o = 32; o > 0; o >>= 1 )
-        m = max( m, __shfl_xor( m, o, 64 ) );
-    if ( ( threadIdx.x & 63 ) == 0 && m > 0 )
-        atomicMax( max_row, m );
+        mrow = max( mrow, __shfl_xor( mrow, o, 64 ) );
+    if ( lane == 0 && mrow > 0 )
+        atomicMax( max_row, mrow );
 }
 
 // Per-atom thermo terms in FP64: pe_i = 1/2 sum_j shifted LJ over the full
@@ -502,10 +585,10 @@ cmd_status build_list( cmd_batch *B, int G )
         CMD_TRY( fill_i32( B->max_row.p, 0, 1, B->stream ) );
         if ( n > 0 )
         {
-#define CMD_BN( GG )                                                                      \
-    k_b_neigh<GG><<<nblk( n ), TPB, 0, B->stream>>>( n, S->x.p, B->xs.p, B->asort.start.p, \
-                                                     B->sys.p, B->tab.p, rc2, N->cap,      \
-                                                     N->counts.p, N->slab.p, B->max_row.p )
+#define CMD_BN( GG )                                                                   \
+    k_b_neigh_cell<GG><<<( B->ncell + 3 ) / 4, 256, 0, B->stream>>>(                    \
+        B->ncell, S->x.p, B->xs.p, B->lsort.start.p, B->asort.start.p, B->sys.p, B->tab.p, \
+        rc2, N->cap, N->counts.p, N->slab.p, B->max_row.p )
             switch ( G )
             {
             case 1: CMD_BN( 1 ); break;
