@@ -41,6 +41,11 @@
 #include <string>
 #include <vector>
 
+// The NNP kernels compute in FP32 against a tolerance, not bit for bit:
+// multiply-adds may fuse here (the library builds with contraction off for
+// the FP64 paths, whose sums must match the oracle exactly).
+#pragma clang fp contract( fast )
+
 namespace cmd
 {
 static constexpr int NNP_MAXEL = 4;
@@ -598,6 +603,10 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
     }
 }
 
+// NAG, NAP > 0: the angular (eta, rs) and (lambda, zeta) grid sizes of
+// every element, fixed at compile time (loops unrolled straight); 0: read
+// from the plan.
+template <int NAG, int NAP>
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     int n_local, const double4 *__restrict__ x, const int64_t *__restrict__ off,
     const int *__restrict__ idx, const NnpPlan *__restrict__ P,
@@ -615,7 +624,11 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     const int n = gather_neighbours( i, x, off, idx, rc, rc2, kpi, L, flag );
     const int e = __builtin_amdgcn_readfirstlane( (int)x[i].w );
     const NnpElemPlan &Q = P->el[e];
-    const AngK K = load_angk( Q, kst[wv] );
+    AngK K = load_angk( Q, kst[wv] );
+    if constexpr ( NAG > 0 )
+        K.nag = NAG;
+    if constexpr ( NAP > 0 )
+        K.nap = NAP;
     float *Gi = G + (size_t)i * NSF_LD;
     // radial: lanes over neighbours
     {
@@ -933,6 +946,7 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp_f32( int n_local,
 }
 
 // ---- forces
+template <int NAG, int NAP>
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     int n_local, int fcap, const double4 *__restrict__ x,
     const int64_t *__restrict__ off, const int *__restrict__ idx,
@@ -954,7 +968,11 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     const int n = gather_neighbours( i, x, off, idx, rc, rc2, kpi, L, flag );
     const int e = __builtin_amdgcn_readfirstlane( (int)x[i].w );
     const NnpElemPlan &Q = P->el[e];
-    const AngK K = load_angk( Q, kst[wv] );
+    AngK K = load_angk( Q, kst[wv] );
+    if constexpr ( NAG > 0 )
+        K.nag = NAG;
+    if constexpr ( NAP > 0 )
+        K.nap = NAP;
     for ( int k = lane; k < NSF_LD; k += 64 )
         sC[wv][k] = k < Q.nsf ? dEdG[(size_t)i * NSF_LD + k] : 0.f;
     for ( int a = lane; a < n; a += 64 )
@@ -1221,8 +1239,16 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
         return CMD_OK;
     CMD_TRY( fill_i32( reinterpret_cast<int *>( M->G.p ), 0, (size_t)n * NSF_LD, st ) );
     const int nbw = ( n + NNP_TPB / 64 - 1 ) / ( NNP_TPB / 64 );
-    k_nnp_sf<<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
-                                       M->dplan.p, M->G.p, M->flag.p );
+    // the angular grid of every element, for the kernels compiled for it
+    bool g58 = M->plan.nelem > 0;
+    for ( int e = 0; e < M->plan.nelem; ++e )
+        g58 = g58 && M->plan.el[e].nag == 5 && M->plan.el[e].nap == 8;
+    if ( g58 )
+        k_nnp_sf<5, 8><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
+                                                 M->dplan.p, M->G.p, M->flag.p );
+    else
+        k_nnp_sf<0, 0><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
+                                                 M->dplan.p, M->G.p, M->flag.p );
     CMD_LAUNCH_CHECK();
     if ( M->prec == CMD_PREC_MIXED )
     {
@@ -1245,9 +1271,14 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
                                                          M->Ei.p );
         CMD_LAUNCH_CHECK();
     }
-    k_nnp_force<<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p,
-                                          N->idx.p, M->dplan.p, M->dEdG.p,
-                                          S->f.p, M->flag.p );
+    if ( g58 )
+        k_nnp_force<5, 8><<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p,
+                                                    N->idx.p, M->dplan.p, M->dEdG.p,
+                                                    S->f.p, M->flag.p );
+    else
+        k_nnp_force<0, 0><<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p,
+                                                    N->idx.p, M->dplan.p, M->dEdG.p,
+                                                    S->f.p, M->flag.p );
     CMD_LAUNCH_CHECK();
     int fl = 0;
     CMD_HIP( hipMemcpyAsync( &fl, M->flag.p, sizeof( int ),
