@@ -634,7 +634,8 @@ def lattice_params(ncells=(20, 20, 20), rho=0.8442, temp=1.44, seed=87287,
 class CbnMD:
     """CbnMD<System, Neighbor> (cabanamd_impl.h:70-432): init() then run()."""
 
-    def __init__(self, params, precision=PREC_FP64, lanes=0, tile=0):
+    def __init__(self, params, precision=PREC_FP64, lanes=0, tile=0,
+                 fuse_ghosts=True):
         self.params = params
         h = C.c_void_p()
         call("cmd_md_create", C.byref(params), C.byref(h))
@@ -642,6 +643,7 @@ class CbnMD:
         call("cmd_md_set_precision", h, precision)
         call("cmd_md_set_lanes", h, lanes)
         call("cmd_md_set_tile", h, tile)
+        call("cmd_md_set_ghost_fusion", h, int(bool(fuse_ghosts)))
 
     def __del__(self):
         if getattr(self, "_h", None):

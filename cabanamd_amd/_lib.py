@@ -149,6 +149,7 @@ _PROTOS = {
     "cmd_md_set_precision": ([VP, C.c_int], None),
     "cmd_md_set_lanes": ([VP, C.c_int], None),
     "cmd_md_set_tile": ([VP, C.c_int], None),
+    "cmd_md_set_ghost_fusion": ([VP, C.c_int], None),
     "cmd_md_set_nnp_model": ([VP, VP], None),
     "cmd_md_set_initial_state": ([VP, C.c_int, DP, DP, IP, IP, DP, DP], None),
     "cmd_md_init": ([VP], None),

@@ -1338,6 +1338,10 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
                                                   H, s->send_idx.p, s->ghost_root.p,
                                                   s->ghost_code.p );
         CMD_LAUNCH_CHECK();
+        // the halo gathers positions and types only (comm_mpi_impl.h:346-347);
+        // ghost velocity rows are set to zero so that downloads are defined
+        for ( int c = 0; c < 3; ++c )
+            CMD_TRY( fill_f64( s->v.p + (size_t)c * s->cap + n, 0.0, ng, s->stream ) );
     }
     s->n_ghost = ng;
     return CMD_OK;

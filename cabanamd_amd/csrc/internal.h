@@ -228,6 +228,15 @@ struct cmd_system
     cmd::CellSort gsort;
     cmd::DevBuf<double4> xg;
     bool xg_valid = false;
+    // the fused step writes the next step's sorted ghosts into xg_alt: local
+    // atom i has images img_pk[gimg.start[i] .. gimg.start[i + 1]), each the
+    // sorted slot | shift code << IMG_CODE_SHIFT; the system's own ghost rows
+    // are then stale until the next refresh
+    cmd::DevBuf<double4> xg_alt;
+    cmd::CellSort gimg;
+    cmd::DevBuf<int> img_pk;
+    bool img_valid = false; // img_pk matches the current ghosts
+    bool ghosts_x_stale = false;
     cmd::Scratch sc;
 
     cmd::MassTable mass_table() const
@@ -306,6 +315,11 @@ namespace cmd
 {
 LJParams lj_params( const cmd_force *f );
 
+// A ghost image entry packs its sorted slot (below 2^26) and its 6-bit
+// shift code.
+constexpr int IMG_CODE_SHIFT = 26;
+constexpr int IMG_SLOT_MASK = ( 1 << IMG_CODE_SHIFT ) - 1;
+
 // Integration constants for the fused force epilogue.
 struct StepCtl
 {
@@ -316,6 +330,12 @@ struct StepCtl
     int wrap_next;        // wrap the drifted positions (next step rebuilds)
     int thermo;           // accumulate KE, PE and virial partials
     int write_f;          // store f
+    // when set (with do_next_initial and no wrap), the drifted position of
+    // every local atom is also written, shifted per its images' codes, to
+    // xg_next at its sorted ghost slots: the next step's ghost refresh
+    double4 *xg_next = nullptr;
+    const int *img_start = nullptr; // per local atom, into img_pk
+    const int *img_pk = nullptr;    // sorted slot | code << IMG_CODE_SHIFT
 };
 
 // LJ kernels (lj.hip)

@@ -262,7 +262,8 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                                              double *__restrict__ f,
                                              const MassTable &mt,
                                              const StepCtl &ctl, double &mv2,
-                                             const double *vin = nullptr )
+                                             const double *vin = nullptr,
+                                             int img_b = 0, int img_e = 0 )
 {
     if ( G >= 4 )
     {
@@ -297,11 +298,28 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                                 pc = q + ctl.L[c];
                         }
                         xn[c] = pc;
+                        for ( int m = img_b; m < img_e; ++m )
+                        {
+                            // the ghost refresh's shift of this component
+                            const unsigned e = (unsigned)ctl.img_pk[m];
+                            const unsigned code = e >> ( IMG_CODE_SHIFT + 2 * c );
+                            double g = pc;
+                            if ( code & 1 )
+                                g = g - ctl.L[c];
+                            if ( code & 2 )
+                                g = g + ctl.L[c];
+                            reinterpret_cast<double *>( ctl.xg_next + ( e & IMG_SLOT_MASK ) )[c] = g;
+                        }
                     }
                     v[c * scap + i] = vc;
                 }
                 else if ( ctl.do_next_initial )
+                {
                     xn[3] = xi.w;
+                    for ( int m = img_b; m < img_e; ++m )
+                        reinterpret_cast<double *>(
+                            ctl.xg_next + ( (unsigned)ctl.img_pk[m] & IMG_SLOT_MASK ) )[3] = xi.w;
+                }
             }
         }
     }
@@ -347,6 +365,26 @@ __device__ __forceinline__ void finish_atom( bool active, int s, int i,
                         p.z = cz + ctl.L[2];
                 }
                 x_next[i] = p;
+                for ( int m = img_b; m < img_e; ++m )
+                {
+                    // the ghost refresh's shifts (one per dimension)
+                    const unsigned e = (unsigned)ctl.img_pk[m];
+                    const unsigned code = e >> IMG_CODE_SHIFT;
+                    double4 g = p;
+                    if ( code & 1 )
+                        g.x = g.x - ctl.L[0];
+                    if ( code & 2 )
+                        g.x = g.x + ctl.L[0];
+                    if ( code & 4 )
+                        g.y = g.y - ctl.L[1];
+                    if ( code & 8 )
+                        g.y = g.y + ctl.L[1];
+                    if ( code & 16 )
+                        g.z = g.z - ctl.L[2];
+                    if ( code & 32 )
+                        g.z = g.z + ctl.L[2];
+                    ctl.xg_next[e & IMG_SLOT_MASK] = g;
+                }
             }
             v[i] = vx;
             v[scap + i] = vy;
@@ -482,8 +520,10 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             return make_double4( p[0], p[1], p[2],
                                  ONE_TYPE ? 0.0 : (double)s_t[off / TILE_REC] );
         };
-        // the velocities the epilogue needs, loaded before the pair loop
+        // the velocities and ghost image range the epilogue needs, loaded
+        // before the pair loop
         double vpre[3] = { 0.0, 0.0, 0.0 };
+        int img_b = 0, img_e = 0;
         if ( FUSED && active && ctl.do_final )
         {
             if ( G >= 4 )
@@ -496,6 +536,11 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
                 vpre[0] = v[i];
                 vpre[1] = v[scap + i];
                 vpre[2] = v[2 * scap + i];
+            }
+            if ( ctl.xg_next && ( G >= 4 ? sub < 4 : sub == 0 ) )
+            {
+                img_b = ctl.img_start[i];
+                img_e = ctl.img_start[i + 1];
             }
         }
         // list words three iterations ahead: under load the memory latency
@@ -534,7 +579,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             fz += __shfl_xor( fz, o, 64 );
         }
         finish_atom<G, FUSED>( active, sub, i, xi, fx, fy, fz, scap, x_next, v,
-                               f, mt, ctl, mv2, vpre );
+                               f, mt, ctl, mv2, vpre, img_b, img_e );
     }
     if ( FUSED && ctl.thermo )
         block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );

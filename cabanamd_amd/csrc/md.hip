@@ -44,6 +44,7 @@ struct cmd_md
     int prec = CMD_PREC_FP64;
     int lanes = 0;
     int tile = 0; // tile height of the tiled list (0 = default or tuned)
+    bool fuse_ghosts = true; // update_halo inside the fused step (tiled lists)
     // the layout tuned at init (on the starting lattice) is tuned again at
     // the first rebuild, when the atoms have left their sites
     bool retune = false;
@@ -261,6 +262,9 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
             CMD_TRY( wrap_local( S ) );
         md->end( k );
     }
+    // the last fused step wrote this step's sorted ghosts (update_halo fused
+    // into the step's epilogue, tiled full lists)
+    bool ghosts_next = false;
     for ( int s = s0; s <= s1; ++s )
     {
         const bool rb = ( s % p.every == 0 );
@@ -268,6 +272,11 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         {
             S->n_ghost = 0;
             CMD_TRY( rebuild( md ) );
+        }
+        else if ( ghosts_next )
+        {
+            S->xg.swap( S->xg_alt );
+            S->ghosts_x_stale = true; // the system's ghost rows lag until a refresh
         }
         else
         {
@@ -293,6 +302,16 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         CMD_TRY( md->partB.reserve( 4 * (size_t)( ( n + 255 ) / 256 + 1 ) ) );
         CMD_TRY( S->x_alt.reserve( S->cap ) );
         int nb_force = 0, nb_int = 0;
+        ctl.xg_next = nullptr;
+        if ( md->fuse_ghosts && !half && N->tiled && S->xg_valid && S->img_valid &&
+             S->n_ghost > 0 && ctl.do_next_initial && !ctl.wrap_next )
+        {
+            CMD_TRY( S->xg_alt.reserve( S->n_ghost ) );
+            ctl.xg_next = S->xg_alt.p;
+            ctl.img_start = S->gimg.start.p;
+            ctl.img_pk = S->img_pk.p;
+        }
+        ghosts_next = ctl.xg_next != nullptr;
         if ( !half )
         {
             int k = md->begin( PH_FORCE );
@@ -331,6 +350,10 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         if ( !last )
             S->x.swap( S->x_alt );
     }
+    // ghost rows of the system at the final positions (downloads, the next
+    // call's first step)
+    if ( S->ghosts_x_stale )
+        CMD_TRY( tiles_ghost_refresh( S ) );
     CMD_HIP( hipStreamSynchronize( st ) );
     md->timers[5] +=
         std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 )
@@ -405,6 +428,13 @@ cmd_status cmd_md_set_tile( cmd_md *md, int height )
     CMD_REQUIRE( height >= 0 && height <= 14, CMD_ERR_ARG,
                  "tile height must be 0 (choose) or 1..14 cells" );
     md->tile = height;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_set_ghost_fusion( cmd_md *md, int on )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    md->fuse_ghosts = on != 0;
     return CMD_OK;
 }
 

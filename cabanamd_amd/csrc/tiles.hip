@@ -526,6 +526,29 @@ __global__ void k_gather_ghosts( int n_local, int n_ghost,
         xg[k] = x[n_local + gperm[k]];
 }
 
+// Root of the ghost at each sorted slot: the key that groups the slots by
+// local atom (the images the fused step writes).
+__global__ void k_image_roots( int n_ghost, const int *__restrict__ gperm,
+                               const int *__restrict__ root, int *__restrict__ key )
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( k < n_ghost )
+        key[k] = root[gperm[k]];
+}
+
+// Image entries in root order: sorted slot and shift code in one word.
+__global__ void k_image_pack( int n_ghost, const int *__restrict__ slot,
+                              const int *__restrict__ gperm,
+                              const int *__restrict__ code, int *__restrict__ pk )
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( m < n_ghost )
+    {
+        const int k = slot[m];
+        pk[m] = (int)( (unsigned)k | ( (unsigned)code[gperm[k]] << IMG_CODE_SHIFT ) );
+    }
+}
+
 cmd_status tiles_prepare_ghosts( cmd_system *S )
 {
     S->xg_valid = false;
@@ -535,18 +558,41 @@ cmd_status tiles_prepare_ghosts( cmd_system *S )
     CMD_TRY( cell_sort( S->x.p + S->n_local, ng, S->bin_grid, S->gsort, S->sc,
                         S->stream ) );
     CMD_TRY( S->xg.reserve( std::max( ng, 1 ) ) );
+    CMD_TRY( S->xg_alt.reserve( std::max( ng, 1 ) ) );
     if ( ng > 0 )
     {
         k_gather_ghosts<<<( ng + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
             S->n_local, ng, S->gsort.perm.p, S->x.p, S->xg.p );
         CMD_LAUNCH_CHECK();
     }
+    // sorted ghost slots grouped by their local atom
+    CMD_TRY( S->gimg.cell_of.reserve( std::max( ng, 1 ) ) );
+    if ( ng > 0 )
+    {
+        k_image_roots<<<( ng + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
+            ng, S->gsort.perm.p, S->ghost_root.p, S->gimg.cell_of.p );
+        CMD_LAUNCH_CHECK();
+    }
+    CMD_TRY( cell_sort_ids( ng, std::max( S->n_local, 1 ), S->gimg, S->sc, S->stream ) );
+    S->img_valid = ng <= IMG_SLOT_MASK + 1;
+    if ( S->img_valid )
+    {
+        CMD_TRY( S->img_pk.reserve( std::max( ng, 1 ) ) );
+        if ( ng > 0 )
+        {
+            k_image_pack<<<( ng + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
+                ng, S->gimg.perm.p, S->gsort.perm.p, S->ghost_code.p, S->img_pk.p );
+            CMD_LAUNCH_CHECK();
+        }
+    }
     S->xg_valid = true;
+    S->ghosts_x_stale = false;
     return CMD_OK;
 }
 
 cmd_status tiles_ghost_refresh( cmd_system *S )
 {
+    S->ghosts_x_stale = false;
     if ( S->n_ghost == 0 )
         return CMD_OK;
     k_ghost_refresh_sorted<<<( S->n_ghost + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(

@@ -106,7 +106,10 @@ cmd_status cmd_system_upload( cmd_system *s, int n_local, int n_ghost,
                               const double *x, const double *v,
                               const double *f, const int *type,
                               const int *id );
-/* Download the first n = n_local + n_ghost atoms (any pointer may be NULL). */
+/* Download the first n = n_local + n_ghost atoms (any pointer may be NULL).
+ * Ghost rows made by cmd_comm_exchange_halo carry positions, types and ids,
+ * with zero velocities (the halo gathers x and type only,
+ * comm_mpi_impl.h:346-347). */
 cmd_status cmd_system_download( cmd_system *s, double *x, double *v,
                                 double *f, int *type, int *id );
 cmd_status cmd_system_counts( const cmd_system *s, int *n_global,
@@ -338,6 +341,11 @@ cmd_status cmd_md_set_lanes( cmd_md *md, int lanes );
 /* Tile height (cells) of the tiled list when lanes are fixed (no reference
  * counterpart; for tuning): 0 keeps the default. */
 cmd_status cmd_md_set_tile( cmd_md *md, int height );
+/* Ghost refresh fused into the step (no reference counterpart; on by
+ * default): on tiled full lists the fused step also writes the next step's
+ * sorted ghost positions, in place of a separate update_halo pass.  Off
+ * gives bit-identical results through the separate pass. */
+cmd_status cmd_md_set_ghost_fusion( cmd_md *md, int on );
 /* Before init (no reference counterpart): an NNP model to use instead of
  * loading pair_style nnp's directory, and an initial state (atoms and box)
  * to use instead of the deck's lattice and velocities. */
