@@ -470,56 +470,83 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     load_head( info, t, H );
     const char *s_byte = reinterpret_cast<const char *>( s_rec );
     const int *s_t = reinterpret_cast<const int *>( s_rec + 3 * slot_cap );
-    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
-    const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
-    const int nw = woff[t + 1] - woff[t];
+    const int wbase = woff[t];
+    const unsigned short *st = slab + (size_t)wbase * R4 * 256;
+    const int nw = woff[t + 1] - wbase;
     const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     // four sentinel entries: the word a lane past the tile's atoms walks
     const unsigned long long sent4 = 0x0001000100010001ull * (unsigned)( H.segoff[18] * TILE_REC );
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
-    for ( int w = wid; w < nw; w += TPB / 64 )
+    // What a slab wave reads from global memory before its pair loop: the
+    // atom, its row length and the first three list words.  The first
+    // wave's reads are issued before the block stages its records, so both
+    // sets of loads are in flight together.
+    struct WaveIn
+    {
+        double4 xi;
+        int n;
+        unsigned long long q0, q1, q2;
+    };
+    auto lane_map = [&]( int w, int &gw, int &apw, int &sub, int &i, bool &active )
     {
         // gw lanes per atom in this slab wave (G, or more in a part-full wave)
-        const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
-        const int a = lane % apw, sub = lane / apw;
+        gw = wave_lanes( G, nat, w );
+        apw = 64 / gw;
+        const int a = lane % apw;
+        sub = lane / apw;
         const int ta = w * APW + a;
-        const bool active = a < APW && ta < nat;
-        const int i = tl + ta;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        active = a < APW && ta < nat;
+        i = tl + ta;
+    };
+    auto row_of = [&]( int w ) -> const unsigned long long *
+    {
+        return reinterpret_cast<const unsigned long long *>(
+            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
+    };
+    auto prep = [&]( int w, WaveIn &P )
+    {
+        int gw, apw, sub, i;
+        bool active;
+        lane_map( w, gw, apw, sub, i, active );
+        // loaded unconditionally (lanes past the atoms read the tile's first
+        // atom) and replaced after staging, so no wait is needed here
+        const int ic = active ? i : tl;
+        P.xi = x[ic];
+        P.n = counts[ic];
+        // the first three words of every lane's column (R4 >= 1 words each;
+        // words past the row's end are loaded but not used)
+        const unsigned long long *row8 = row_of( w );
+        P.q0 = __builtin_nontemporal_load( row8 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        P.q1 = __builtin_nontemporal_load( row8 + (size_t)min( 1, R4 - 1 ) * 64 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        P.q2 = __builtin_nontemporal_load( row8 + (size_t)min( 2, R4 - 1 ) * 64 );
+        __builtin_amdgcn_sched_barrier( 0 );
+    };
+    WaveIn P;
+    if ( wid < nw )
+        prep( wid, P );
+    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
+    for ( int w = wid; w < nw; w += TPB / 64 )
+    {
+        if ( w != wid )
+            prep( w, P );
+        int gw, apw, sub, i;
+        bool active;
+        lane_map( w, gw, apw, sub, i, active );
         const double4 xi =
-            active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
+            active ? P.xi : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
         const int ti = (int)xi.w;
-        const int n = active ? counts[i] : 0;
+        const int n = active ? P.n : 0;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
         int nr = n > sub ? ( n - sub + gw - 1 ) / gw : 0;
 #pragma unroll
         for ( int o = 32; o > 0; o >>= 1 )
             nr = max( nr, __shfl_xor( nr, o, 64 ) );
         nr = __builtin_amdgcn_readfirstlane( nr );
-        const unsigned long long *row8 = reinterpret_cast<const unsigned long long *>(
-            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
-        auto body = [&]( const double4 &xj )
-        {
-            const double dx = xi.x - xj.x;
-            const double dy = xi.y - xj.y;
-            const double dz = xi.z - xj.z;
-            const double rsq = ( PREC == CMD_PREC_FP64 )
-                                   ? dx * dx + dy * dy + dz * dz
-                                   : __builtin_fma( dx, dx, __builtin_fma( dy, dy, dz * dz ) );
-            const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
-            const double fpair = lj_pair<PREC, ENERGY>( rsq, c, pe, vir );
-            accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
-        };
-        // four records per 8-byte word of byte offsets; all four LDS reads
-        // are issued before the arithmetic that consumes them
-        auto fetch = [&]( unsigned long long wd, int q ) -> double4
-        {
-            const unsigned off = (unsigned)( ( wd >> ( 16 * q ) ) & 0xffffull );
-            const double *p = reinterpret_cast<const double *>( s_byte + off );
-            return make_double4( p[0], p[1], p[2],
-                                 ONE_TYPE ? 0.0 : (double)s_t[off / TILE_REC] );
-        };
+        const unsigned long long *row8 = row_of( w );
         // the velocities and ghost image range the epilogue needs, loaded
         // before the pair loop
         double vpre[3] = { 0.0, 0.0, 0.0 };
@@ -543,26 +570,44 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
                 img_e = ctl.img_start[i + 1];
             }
         }
+        auto body = [&]( const double4 &xj )
+        {
+            const double dx = xi.x - xj.x;
+            const double dy = xi.y - xj.y;
+            const double dz = xi.z - xj.z;
+            const double rsq = ( PREC == CMD_PREC_FP64 )
+                                   ? dx * dx + dy * dy + dz * dz
+                                   : __builtin_fma( dx, dx, __builtin_fma( dy, dy, dz * dz ) );
+            const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, (int)xj.w );
+            const double fpair = lj_pair<PREC, ENERGY>( rsq, c, pe, vir );
+            accumulate<PREC>( fx, fy, fz, dx, dy, dz, fpair );
+        };
+        // four records per 8-byte word of byte offsets; all four LDS reads
+        // are issued before the arithmetic that consumes them
+        auto fetch = [&]( unsigned long long wd, int q ) -> double4
+        {
+            const unsigned off = (unsigned)( ( wd >> ( 16 * q ) ) & 0xffffull );
+            const double *p = reinterpret_cast<const double *>( s_byte + off );
+            return make_double4( p[0], p[1], p[2],
+                                 ONE_TYPE ? 0.0 : (double)s_t[off / TILE_REC] );
+        };
         // list words three iterations ahead: under load the memory latency
-        // spans several iterations of all waves of a SIMD.  Lanes past the
-        // tile's atoms read nothing and walk the sentinel record.
-        unsigned long long q0 = sent4, q1 = sent4, q2 = sent4;
-        if ( active )
+        // spans several iterations of all waves of a SIMD.  The loop is
+        // unrolled by three so each word stays in its register until used
+        // (a rotation through registers would wait for the newest load).
+        // Every lane loads from its own column of the wave's slab, which
+        // holds at least max(nq, 1) words for all 64 lanes; lanes past the
+        // tile's atoms use the sentinel word in place of what their column
+        // holds.  Loads are issued unconditionally (indexes past the last
+        // word re-read it), so the count of loads in flight is the same on
+        // every path and the wait before a use covers only that word.
+        const int nq = ( nr + 3 ) / 4;
+        const int klast = max( nq - 1, 0 );
+        auto word = [&]( int k ) -> unsigned long long
+        { return __builtin_nontemporal_load( row8 + (size_t)min( k, klast ) * 64 ); };
+        auto quad = [&]( unsigned long long wd )
         {
-            if ( nr > 0 )
-                q0 = __builtin_nontemporal_load( row8 );
-            if ( nr > 4 )
-                q1 = __builtin_nontemporal_load( row8 + 64 );
-            if ( nr > 8 )
-                q2 = __builtin_nontemporal_load( row8 + 128 );
-        }
-        for ( int r = 0; r < nr; r += 4 )
-        {
-            const unsigned long long cur = q0;
-            q0 = q1;
-            q1 = q2;
-            if ( active && r + 12 < nr )
-                q2 = __builtin_nontemporal_load( row8 + (size_t)( r / 4 + 3 ) * 64 );
+            const unsigned long long cur = active ? wd : sent4;
             const double4 p0 = fetch( cur, 0 );
             const double4 p1 = fetch( cur, 1 );
             const double4 p2 = fetch( cur, 2 );
@@ -571,6 +616,18 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
             body( p1 );
             body( p2 );
             body( p3 );
+        };
+        unsigned long long q0 = P.q0, q1 = P.q1, q2 = P.q2;
+        for ( int k = 0; k < nq; k += 3 )
+        {
+            quad( q0 );
+            q0 = word( k + 3 );
+            if ( k + 1 < nq )
+                quad( q1 );
+            q1 = word( k + 4 );
+            if ( k + 2 < nq )
+                quad( q2 );
+            q2 = word( k + 5 );
         }
         for ( int o = apw; o < 64; o <<= 1 )
         {

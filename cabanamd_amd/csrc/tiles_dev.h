@@ -20,12 +20,19 @@ enum
     TI_NST = 40    // staged cells per column
 };
 
+constexpr int TILE_STRIP = 4;
+
 __device__ __forceinline__ void tile_coords( const TileGrid &T, int t, int &a,
                                              int &b, int &e0, int &e1 )
 {
+    // columns in strips of TILE_STRIP rows of a, b outer and a inner within
+    // a strip: the 3 x 3 columns around a column run close together in
+    // time, so a staged column is mostly re-read from L2
     const int col = t / T.tpc, z = t % T.tpc;
-    a = 1 + col / T.nc1;
-    b = 1 + col % T.nc1;
+    const int strip = col / ( TILE_STRIP * T.nc1 ), r = col % ( TILE_STRIP * T.nc1 );
+    const int h = min( TILE_STRIP, T.nc0 - strip * TILE_STRIP );
+    a = 1 + strip * TILE_STRIP + r % h;
+    b = 1 + r / h;
     e0 = 1 + z * T.E;
     e1 = min( e0 + T.E, T.g.n[2] - 1 );
 }
@@ -108,7 +115,7 @@ __device__ inline void stage_records( const TileHead &H, const double4 *__restri
 {
     int *s_t = reinterpret_cast<int *>( s_rec + 3 * cap );
     const int total = H.segoff[18];
-    constexpr int B = 4; // slots per thread whose loads are in flight together
+    constexpr int B = 2; // slots per thread whose loads are in flight together
     int seg = 0;
     for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
     {
