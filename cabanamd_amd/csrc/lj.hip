@@ -749,15 +749,39 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
                 unsafeAtomicAdd( r + 2, -az );
             }
         };
-        for ( int r = 0; r < nr; r += 4 )
+        // list words three loads ahead, as in the full-list kernel: every
+        // lane loads from its own column of the wave's slab (indexes past
+        // the row's last word re-read it) and lanes past the tile's atoms
+        // walk the sentinel in place of what their column holds
+        const int nq = ( nr + 3 ) / 4;
+        const int klast = max( nq - 1, 0 );
+        auto word = [&]( int k ) -> unsigned long long
+        { return __builtin_nontemporal_load( row8 + (size_t)min( k, klast ) * 64 ); };
+        auto quad = [&]( unsigned long long w8 )
         {
-            // lanes past the tile's atoms read nothing and walk the sentinel
-            const unsigned long long wd =
-                active ? __builtin_nontemporal_load( row8 + (size_t)( r / 4 ) * 64 ) : sent4;
+            const unsigned long long wd = active ? w8 : sent4;
             body( (unsigned)( wd & 0xffffull ) );
             body( (unsigned)( ( wd >> 16 ) & 0xffffull ) );
             body( (unsigned)( ( wd >> 32 ) & 0xffffull ) );
             body( (unsigned)( wd >> 48 ) );
+        };
+        const unsigned long long q0i = word( 0 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        const unsigned long long q1i = word( 1 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        const unsigned long long q2i = word( 2 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        unsigned long long q0 = q0i, q1 = q1i, q2 = q2i;
+        for ( int k = 0; k < nq; k += 3 )
+        {
+            quad( q0 );
+            q0 = word( k + 3 );
+            if ( k + 1 < nq )
+                quad( q1 );
+            q1 = word( k + 4 );
+            if ( k + 2 < nq )
+                quad( q2 );
+            q2 = word( k + 5 );
         }
         for ( int o = apw; o < 64; o <<= 1 )
         {
