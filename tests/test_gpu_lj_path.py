@@ -537,3 +537,28 @@ def test_lj_two_types(gpu, prec, path):
     assert err < (1e-12 if prec == "fp64" else 1e-5), err
     eo = O.lj_energy(xg, tg, n, 2, lj1, lj2, cs, off, idx, half)
     assert abs(F.compute_energy(s, nl) - eo) <= 1e-10 * abs(eo)
+
+
+def test_ghost_rows_after_halo(gpu):
+    """Ghost rows made by the halo exchange: positions are the root's image
+    (comm_mpi_impl.h:346-368), types follow the root, velocities read as
+    zero (the halo gathers x and type only), both after init and after a
+    run whose steps wrote the sorted ghosts inside the fused step."""
+    M = gpu
+    kw = dict(ncells=(8, 8, 8), nsteps=40, thermo=10)
+    md = M.CbnMD(M.lattice_params(**kw), lanes=2)
+    md.init()
+    for k in (0, 33):
+        if k:
+            md.run(k)
+        x, v, f, t, i = md.arrays()
+        n = md.n_local
+        assert len(x) > n
+        assert np.array_equal(v[n:], np.zeros_like(v[n:]))
+        L = 8 * (4 / 0.8442) ** (1 / 3)  # 8 fcc cells at rho 0.8442
+        # every ghost is a local atom shifted by a whole number of box lengths
+        by_id = {int(a): r for r, a in enumerate(i[:n])}
+        src = np.array([by_id[int(a)] for a in i[n:]])
+        shift = (x[n:] - x[src]) / L
+        assert np.allclose(shift, np.round(shift), atol=1e-9)
+        assert np.array_equal(t[n:], t[src])
