@@ -439,17 +439,20 @@ def test_md_loop_fp64(gpu, half):
         assert np.abs(v[:n] - vo[:n]).max() < 1e-9
 
 
+@pytest.mark.parametrize("lanes", [2, 4])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_md_fused_ghost_refresh_bit_identical(gpu, prec):
+def test_md_fused_ghost_refresh_bit_identical(gpu, prec, lanes):
     """On tiled full lists the fused step writes the next step's sorted
     ghosts (update_halo, comm_mpi_impl.h:369-408, inside the step): over
     rebuilds, thermo steps and a split run, x, v, f (ghost rows included)
-    and thermo rows equal those of the separate refresh pass bit for bit."""
+    and thermo rows equal those of the separate refresh pass bit for bit,
+    with one lane finishing an atom (2 lanes) and one lane per component
+    (4 lanes)."""
     M = gpu
     kw = dict(ncells=(10, 9, 8), nsteps=150, thermo=10)
     P = M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED
-    a = M.CbnMD(M.lattice_params(**kw), precision=P, lanes=2, fuse_ghosts=True)
-    b = M.CbnMD(M.lattice_params(**kw), precision=P, lanes=2, fuse_ghosts=False)
+    a = M.CbnMD(M.lattice_params(**kw), precision=P, lanes=lanes, fuse_ghosts=True)
+    b = M.CbnMD(M.lattice_params(**kw), precision=P, lanes=lanes, fuse_ghosts=False)
     a.init()
     b.init()
     for k in (45, 105):
