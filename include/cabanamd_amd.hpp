@@ -344,6 +344,8 @@ class CbnMD
     }
     void set_precision( int prec ) { check( cmd_md_set_precision( h_.get(), prec ) ); }
     void set_lanes( int lanes ) { check( cmd_md_set_lanes( h_.get(), lanes ) ); }
+    void set_tile( int height ) { check( cmd_md_set_tile( h_.get(), height ) ); }
+    void set_ghost_fusion( bool on ) { check( cmd_md_set_ghost_fusion( h_.get(), on ? 1 : 0 ) ); }
     void init() { check( cmd_md_init( h_.get() ) ); }
     void run( int nsteps ) { check( cmd_md_run( h_.get(), nsteps ) ); }
     // rows of (step, T, PE/N, (PE+KE)/N, W)
