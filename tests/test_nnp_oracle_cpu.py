@@ -137,3 +137,81 @@ def test_reference_fixture_parses_and_runs():
     assert np.abs(f[:len(x)]).max() < 1e-9
     # scaled inputs of the crystal sit inside the training range's span
     assert np.isfinite(e)
+
+
+def _numpy_network(m, e, G):
+    """Independent forward pass of one element's network from raw symmetry
+    functions: min-max scaling smin + (smax - smin)(G - Gmin)/(Gmax - Gmin),
+    then h <- act(W h + b) per layer with W stored (out x in), tanh on the
+    hidden layers and identity on the output ('ttl')."""
+    h = np.asarray(G, float)
+    if m["scale"]:
+        h = m["smin"] + (m["smax"] - m["smin"]) * (h - m["gmin"][e]) / (
+            m["gmax"][e] - m["gmin"][e])
+    nl = len(m["W"][e])
+    for l in range(nl):
+        h = m["W"][e][l] @ h + m["b"][e][l]
+        if m["act"][l] == "t":
+            h = np.tanh(h)
+    assert h.shape == (1,)
+    return float(h[0])
+
+
+def _numpy_dE_dG(m, e, G, step=1e-6):
+    """dE/dG of the numpy network by central differences in each raw G."""
+    out = np.zeros(len(G))
+    for k in range(len(G)):
+        gp, gm = np.array(G, float), np.array(G, float)
+        gp[k] += step
+        gm[k] -= step
+        out[k] = (_numpy_network(m, e, gp) - _numpy_network(m, e, gm)) / (2 * step)
+    return out
+
+
+def test_network_forward_matches_numpy():
+    """The oracle's atomic energies E_i equal an independent numpy forward pass
+    (scaling, W h + b, tanh, tanh, linear) over the brute-force symmetry
+    functions, and its dE/dG equals the numpy network's gradient.  A
+    transposed W (non-square first layer), dropped scaling, a tanh on the
+    output layer or a missing bias all change E_i by far more than 1e-12."""
+    m = random_model(seed=21, hidden=(7, 5))
+    M = to_oracle(O, m)
+    x, t = cluster(16, box=4.5, seed=3)
+    off, idx = _brute_list(x, 4.5)
+    e, Ei, G, dG, f = M.compute(x, t, len(x), off, idx)
+    ref = sf_brute(m, x, t)
+    for i in range(len(x)):
+        n = len(ref[i])
+        want = _numpy_network(m, t[i], ref[i])
+        assert abs(Ei[i] - want) <= 1e-12 * max(1.0, abs(want)), (i, Ei[i], want)
+        d = _numpy_dE_dG(m, t[i], ref[i])
+        assert np.allclose(dG[i, :n], d, rtol=1e-6, atol=1e-8)
+    assert abs(e - Ei.sum()) <= 1e-12 * max(1.0, abs(e))
+    # the checks can fail: a network with the output layer through tanh
+    # differs from the oracle's linear output
+    bad = dict(m, act="ttt")
+    assert max(abs(_numpy_network(bad, t[i], ref[i]) - Ei[i])
+               for i in range(len(x))) > 1e-6
+
+
+def test_network_forward_matches_numpy_on_reference_fixture():
+    """The same independent forward pass on the input/nnp model (27-24-24-1,
+    scaled inputs) on a jittered fcc cluster of Ni at the in.nnp lattice
+    constant 3.52: E_i from the oracle equals numpy's."""
+    m = read_n2p2()
+    M = to_oracle(O, m)
+    rng = np.random.default_rng(8)
+    a = 3.52
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    x = np.array([(np.array([i, j, k]) + b) * a for i in range(2)
+                  for j in range(2) for k in range(2) for b in base])
+    x = x + rng.uniform(-0.1, 0.1, x.shape)
+    t = np.zeros(len(x), np.int32)
+    rc = max(s[7] for s in m["sf"][0])
+    off, idx = _brute_list(x, rc + 0.5)
+    e, Ei, G, dG, f = M.compute(x, t, len(x), off, idx)
+    ref = sf_brute(m, x, t)
+    for i in range(len(x)):
+        assert np.allclose(G[i, :27], ref[i], rtol=1e-10, atol=1e-12)
+        want = _numpy_network(m, 0, ref[i])
+        assert abs(Ei[i] - want) <= 1e-11 * max(1.0, abs(want)), (i, Ei[i], want)

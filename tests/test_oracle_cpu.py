@@ -246,6 +246,56 @@ def test_lj_benchmark_lattice_energy():
     assert abs(pe_shifted + pairs_per_atom * erc - (-6.7733681)) < 5e-8
 
 
+def _lj_pair_virial_closed_form(r, eps, sigma, rc):
+    """r . F(r) of one LJ pair: r (-dE/dr) = 48 eps (s/r)^12 - 24 eps (s/r)^6
+    inside the cutoff (strict), 0 outside."""
+    if r >= rc:
+        return 0.0
+    s6 = (sigma / r) ** 6
+    return 48.0 * eps * s6 * s6 - 24.0 * eps * s6
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_lj_virial_two_atom_closed_form(half):
+    """orc_lj_virial = sum over pairs of r_ij . F_ij: a single pair at
+    distances on both sides of the minimum (sign change at 2^(1/6) sigma),
+    next to the cutoff and beyond it; the full list sums each pair from
+    both sides with weight 1/2, the half list once."""
+    lj1, lj2, cutsq = O.lj_coeff(1, [(1, 1, 1.0, 1.0, 2.5)])
+    t = np.zeros(2, np.int32)
+    for r in (0.93, 1.05, 2.0 ** (1.0 / 6.0), 1.37, 2.1, 2.4999, 2.7):
+        d = np.array([0.6, -0.48, 0.64])  # a unit direction off the axes
+        x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0]]) + np.outer([0, r], d)
+        off, idx = O.neigh_brute(x, 2, 2.8, half=half)
+        w = O.lj_virial(x, t, 2, 1, lj1, lj2, cutsq, off, idx, half)
+        ref = _lj_pair_virial_closed_form(r, 1.0, 1.0, 2.5)
+        assert abs(w - ref) <= 1e-12 * max(1.0, abs(ref)), (r, w, ref)
+    # repulsive inside the minimum, attractive outside it
+    assert _lj_pair_virial_closed_form(1.0, 1.0, 1.0, 2.5) > 0
+    assert _lj_pair_virial_closed_form(1.5, 1.0, 1.0, 2.5) < 0
+
+
+def test_lj_virial_mixed_types_sums_pairs():
+    """Three atoms of two types with per-pair eps, sigma and cutoff: W is the
+    sum of the three pairs' closed forms (a transposed type index or a
+    dropped pair changes it)."""
+    coeffs = [(1, 1, 1.0, 1.0, 2.5), (1, 2, 0.7, 1.2, 2.2), (2, 2, 1.4, 0.9, 2.0)]
+    lj1, lj2, cutsq = O.lj_coeff(2, coeffs)
+    x = np.array([[0.0, 0.0, 0.0], [1.1, 0.2, -0.1], [0.3, 1.25, 0.4]])
+    t = np.array([0, 1, 1], np.int32)
+    table = {(0, 0): coeffs[0], (0, 1): coeffs[1], (1, 0): coeffs[1],
+             (1, 1): coeffs[2]}
+    ref = 0.0
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        _, _, eps, sig, rc = table[(t[i], t[j])]
+        ref += _lj_pair_virial_closed_form(np.linalg.norm(x[i] - x[j]), eps,
+                                           sig, rc)
+    for half in (False, True):
+        off, idx = O.neigh_brute(x, 3, 2.8, half=half)
+        w = O.lj_virial(x, t, 3, 2, lj1, lj2, cutsq, off, idx, half)
+        assert abs(w - ref) <= 1e-12 * abs(ref), (half, w, ref)
+
+
 def test_half_list_force_equals_full_after_fold():
     p, d = _setup((4, 4, 4), half=False)
     md_full = O.MD(p)
