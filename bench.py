@@ -3,8 +3,8 @@
 Workload (the reference's LJ melt at scale): fcc lattice at reduced density
 0.8442, 80x80x80 unit cells = 2,048,000 atoms, periodic cube, eps = sigma =
 mass = 1, cutoff 2.5, skin 0.3, full Verlet list rebuilt every 20 steps,
-velocities from the seeded loop-geom generator at T = 1.44 (zero net
-momentum), dt = 0.005, NVE, thermo every 100 steps.  A "step" is one MD
+Gaussian velocities from the reference's seeded generator at T = 1.44
+(zero net momentum), dt = 0.005, NVE, thermo every 100 steps.  A "step" is one MD
 timestep of the whole system (integrate, list maintenance, force, thermo).
 
 Prints ONE JSON line.  value = atom-steps/s of the headline variant
@@ -55,13 +55,21 @@ def _force_error(md):
                 max_over_rms=float(np.sqrt((d ** 2).sum(axis=1)).max() / rms))
 
 
+def _drift(e):
+    """max |E - E_0| / |E_0| over the thermo rows, None with fewer than two."""
+    if len(e) < 2:
+        return None
+    return float(abs(e - e[0]).max() / abs(e[0]))
+
+
 def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     p = M.lattice_params(ncells=ncells, half=half, thermo=100,
-                         nsteps=steps)
+                         nsteps=steps, gaussian=True)
     md = M.CbnMD(p, precision=prec, lanes=lanes)
     t0 = time.time()
     md.init()
     t_init = time.time() - t0
+    g, tile, tune_s = md.tune_info()
     n = md.n_local
     if warmup:
         md.run(warmup)
@@ -74,38 +82,89 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     launches, mean_ms = md.kernel_stats()
     ferr = _force_error(md) if accuracy else None
     th = md.thermo()
-    e = th[:, 3]
     out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
                atom_steps_per_s=n * steps / dt, kernel_launches=launches,
-               kernel_mean_ms=mean_ms, init_s=t_init,
-               lanes=md.force.lanes,
-               energy_drift=float(abs(e[-1] - e[0]) / abs(e[0])),
+               kernel_mean_ms=mean_ms, init_s=t_init, tune_s=tune_s,
+               lanes=g, tile=tile,
+               energy_drift=_drift(th[:, 3]), thermo_rows=len(th),
                force_error_vs_fp64_oracle=ferr,
                timers=md.timers())
     del md
     return out
 
 
-def _cpu_baseline(ncells, steps):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
+def _host_threads():
     threads = os.cpu_count() or 1
     try:
         threads = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    threads = int(os.environ.get("OMP_NUM_THREADS", threads))
-    md = O.MD(O.lj_params(ncells=ncells, thermo=100, nsteps=steps))
+    return int(os.environ.get("OMP_NUM_THREADS", threads))
+
+
+def _oracle_run(ncells, steps, thermo=100, half=False):
+    """The FP64 OpenMP oracle's MD loop on the LJ workload with Gaussian
+    velocities: (atom-steps/s of the step loop, thermo rows)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    md = O.MD(O.lj_params(ncells=ncells, thermo=thermo, nsteps=steps,
+                          gaussian=True, half=half))
     md.init()
     t0 = time.perf_counter()
     md.run(steps)
     dt = time.perf_counter() - t0
-    n = md.n_local
-    return dict(value=n * steps / dt, unit="atom-steps/s", cores=threads,
+    return md.n_local * steps / dt, dt, md.thermo()
+
+
+def _cpu_baseline(ncells, steps):
+    rate, dt, _ = _oracle_run(ncells, steps)
+    n = 4 * ncells[0] * ncells[1] * ncells[2]
+    return dict(value=rate, unit="atom-steps/s", cores=_host_threads(),
                 kind="port",
                 sample=(f"FP64 OpenMP oracle (restatement of the reference "
                         f"algorithm), same workload, {n} atoms, {steps} "
-                        f"steps incl. one list rebuild, {dt:.2f} s"))
+                        f"steps incl. {steps // 20} list rebuilds, "
+                        f"{dt:.2f} s"))
+
+
+def _c1_case(M, cpu=True, drift_steps=1000):
+    """The reference's CPU-runnable LJ case: fcc rho 0.8442, 20^3 cells =
+    32,000 atoms, full list rebuilt every 20 steps, Gaussian velocities at
+    T 1.44, dt 0.005, 100 NVE steps, FP64, thermo every 100.  Timed on the
+    GPU (FP64) and with the FP64 oracle on the host's cores; then the
+    total-energy drift over drift_steps steps (thermo every 100) of the
+    engine in both precisions against the oracle's drift on the same deck."""
+    import numpy as np
+    nc = (20, 20, 20)
+    out = {}
+    for name, prec in (("fp64", M.PREC_FP64), ("mixed", M.PREC_MIXED)):
+        p = M.lattice_params(ncells=nc, thermo=100, nsteps=100, gaussian=True)
+        md = M.CbnMD(p, precision=prec)
+        md.init()
+        M._lib.call("cmd_device_synchronize")
+        t0 = time.perf_counter()
+        md.run(100)
+        M._lib.call("cmd_device_synchronize")
+        dt = time.perf_counter() - t0
+        md.run(drift_steps - 100)
+        e = md.thermo()[:, 3]
+        out[name] = dict(n=md.n_local, steps=100, seconds=dt,
+                         atom_steps_per_s=md.n_local * 100 / dt,
+                         ms_per_step=10.0 * dt, lanes=md.tune_info()[0],
+                         drift_steps=drift_steps, energy_drift=_drift(e),
+                         e_final=float(e[-1]))
+        del md
+    if cpu:
+        rate, dt, _ = _oracle_run(nc, 100)
+        out["cpu_oracle"] = dict(atom_steps_per_s=rate, seconds=dt,
+                                 cores=_host_threads(), kind="port")
+        _, _, th = _oracle_run(nc, drift_steps)
+        e = th[:, 3]
+        out["cpu_oracle"].update(drift_steps=drift_steps,
+                                 energy_drift=_drift(e), e_final=float(e[-1]))
+        for name in ("fp64", "mixed"):
+            out[name]["e_final_minus_oracle"] = out[name]["e_final"] - e[-1]
+    return out
 
 
 NNP_SIZES = {3000: (10, 15, 20), 24000: (20, 30, 40), 192000: (40, 60, 80)}
@@ -406,7 +465,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cells", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--c1", type=int, default=1,
+                    help="time the 32,000-atom case and its 1000-step drift")
     ap.add_argument("--variants", default="mixed,fp64,half")
     ap.add_argument("--nnp", default="3000,24000,192000",
                     help="NNP system sizes to time ('' to skip)")
@@ -463,6 +524,15 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+    c1 = None
+    if args.c1:
+        c1 = _c1_case(M, cpu=not args.no_cpu)
+        print(f"# C1 32,000 atoms FP64: {c1['fp64']['atom_steps_per_s']:.4e} "
+              f"atom-steps/s GPU"
+              + (f", {c1['cpu_oracle']['atom_steps_per_s']:.4e} CPU oracle; "
+                 f"1000-step drift GPU {c1['fp64']['energy_drift']:.3e} "
+                 f"oracle {c1['cpu_oracle']['energy_drift']:.3e}"
+                 if 'cpu_oracle' in c1 else ""), file=sys.stderr, flush=True)
     nnp = {}
     for sz in [int(v) for v in args.nnp.split(",") if v]:
         for prec in (M.PREC_MIXED, M.PREC_FP64):
@@ -505,7 +575,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-pair/f64" if head == "mixed" else "f64",
-        "data": "synthetic (seeded fcc lattice + loop-geom velocities)",
+        "data": "synthetic (seeded fcc lattice + Gaussian velocities)",
         "config": {
             "workload": ("LJ fcc rho 0.8442, 80^3 cells = 2,048,000 atoms, "
                          "rc 2.5, skin 0.3, full Verlet list rebuilt every 20"
@@ -525,6 +595,7 @@ def main():
                       f"{h['kernel_mean_ms']:.4f} ms, "
                       f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},
         "cpu_baseline": cpu,
+        "c1_32k": c1,
         "nnp": nnp,
         "batch": batch,
         "serve": serve,

@@ -747,6 +747,12 @@ class CbnMD:
         call("cmd_md_kernel_stats", self._h, C.byref(n), C.byref(ms))
         return n.value, ms.value
 
+    def tune_info(self):
+        """(lanes per atom, tile height or 0, seconds init spent choosing)."""
+        g, e, s = C.c_int(), C.c_int(), C.c_double()
+        call("cmd_md_tune_info", self._h, C.byref(g), C.byref(e), C.byref(s))
+        return g.value, e.value, s.value
+
 
 class Batch:
     """Many independent LJ systems stepped together (no reference
@@ -787,6 +793,9 @@ class Batch:
 
     def remove(self, handle):
         call("cmd_batch_remove", self._h, int(handle))
+
+    def set_precision(self, prec):
+        call("cmd_batch_set_precision", self._h, int(prec))
 
     def run(self, nsteps):
         call("cmd_batch_run", self._h, int(nsteps))

@@ -161,6 +161,7 @@ _PROTOS = {
     "cmd_md_timers": ([VP, DP], None),
     "cmd_md_profile": ([VP, C.c_int], None),
     "cmd_md_kernel_stats": ([VP, IP, DP], None),
+    "cmd_md_tune_info": ([VP, IP, IP, DP], None),
     "cmd_md_dump": ([VP, C.c_char_p, C.c_int], None),
     "cmd_md_check_correctness": ([VP, C.c_char_p, C.c_int, C.c_char_p, DP],
                                  None),

@@ -1010,6 +1010,8 @@ cmd_status cmd_batch_set_precision( cmd_batch *b, int prec )
     CMD_REQUIRE( b, CMD_ERR_ARG, "batch is NULL" );
     CMD_REQUIRE( prec == CMD_PREC_FP64 || prec == CMD_PREC_MIXED, CMD_ERR_ARG,
                  "unknown precision %d", prec );
+    if ( prec != b->prec )
+        ++b->epoch; // captured steps launch the other precision's kernel
     b->prec = prec;
     return b->F ? cmd_force_set_precision( b->F, prec ) : CMD_OK;
 }

@@ -309,6 +309,7 @@ struct cmd_force
     int lanes = 4;
     bool tuned = false;
     cmd::DevBuf<LJParams> dparams; // device copy for multi-type kernels
+    bool dparams_dirty = true;     // coefficients changed since the copy
 };
 
 namespace cmd

@@ -963,16 +963,22 @@ __global__ __launch_bounds__( TPB ) void k_lj_energy_csr(
 }
 
 // --------------------------------------------------------------- dispatch
+// The device copy of the multi-type tables, uploaded once after the
+// coefficients change (the copy waits for the stream, so the host table it
+// reads can go); launches between changes reuse it without a host wait.
 static cmd_status upload_params( const cmd_force *F, hipStream_t st,
                                  LJParams **dptr )
 {
     cmd_force *mf = const_cast<cmd_force *>( F );
-    CMD_TRY( mf->dparams.reserve( 1 ) );
-    static thread_local LJParams P;
-    P = lj_params( F );
-    CMD_HIP( hipMemcpyAsync( mf->dparams.p, &P, sizeof( P ),
-                             hipMemcpyHostToDevice, st ) );
-    CMD_HIP( hipStreamSynchronize( st ) );
+    if ( mf->dparams_dirty || !mf->dparams.p )
+    {
+        CMD_TRY( mf->dparams.reserve( 1 ) );
+        const LJParams P = lj_params( F );
+        CMD_HIP( hipMemcpyAsync( mf->dparams.p, &P, sizeof( P ),
+                                 hipMemcpyHostToDevice, st ) );
+        CMD_HIP( hipStreamSynchronize( st ) );
+        mf->dparams_dirty = false;
+    }
     *dptr = mf->dparams.p;
     return CMD_OK;
 }
@@ -1535,6 +1541,7 @@ cmd_status cmd_force_lj_init_coeff( cmd_force *f, int i, int j, double eps,
     f->lj1[b] = f->lj1[a];
     f->lj2[b] = f->lj2[a];
     f->cutsq[b] = f->cutsq[a];
+    f->dparams_dirty = true;
     return CMD_OK;
 }
 

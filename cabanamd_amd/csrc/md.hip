@@ -45,9 +45,9 @@ struct cmd_md
     int lanes = 0;
     int tile = 0; // tile height of the tiled list (0 = default or tuned)
     bool fuse_ghosts = true; // update_halo inside the fused step (tiled lists)
-    // the layout tuned at init (on the starting lattice) is tuned again at
-    // the first rebuild, when the atoms have left their sites
+    // set for one rebuild: the list is built by timing every layout
     bool retune = false;
+    double tune_s = 0.0; // host seconds spent choosing the layout at init
     std::vector<double> thermo; // rows of 5: step, T, PE/N, (PE+KE)/N, W
     cmd_nnp_model *nnp_model = nullptr; // injected NNP model (owned)
     bool custom_state = false;          // initial state given by the caller
@@ -375,6 +375,54 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
     return CMD_OK;
 }
 
+// Choose lanes per atom and tile height (autotune_lanes) on a short scratch
+// run from the initial state, then put the initial state back.  Timed on the
+// perfect lattice, layouts with more lanes per atom measure within a few
+// percent of fewer lanes but fall behind once the atoms leave their sites
+// (melted LJ, 2,048,000 atoms: 2 lanes 0.253 ms, 4 lanes 0.270 ms per fused
+// step), so the layouts are timed at the scratch run's list rebuild.  The
+// run proper then starts from the uploaded state exactly as an untuned run
+// would, and no tuning falls inside CbnMD::run's timed steps.
+static cmd_status tune_layout( cmd_md *md, int n, const std::vector<double> &x,
+                               const std::vector<double> &v, const std::vector<int> &type,
+                               const std::vector<int> &id )
+{
+    cmd_system *S = md->S;
+    const auto t0 = std::chrono::steady_clock::now();
+    CMD_TRY( cmd_comm_exchange( S ) );
+    CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut, 1, 1, 0, 1,
+                                 nullptr ) );
+    CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
+    md->N->lanes = 2; // the tiled layout for the scratch steps
+    md->N->tile_e = 0;
+    CMD_TRY( cmd_neigh_build( md->N, S ) );
+    md->F->lanes = md->N->lanes;
+    CMD_TRY( cmd_force_compute( md->F, S, md->N ) );
+    // the scratch steps end on a rebuild step, where the layouts are timed
+    const int k = std::min( md->p.every, 20 );
+    const int every = md->p.every, thermo = md->p.thermo_rate;
+    md->p.every = k;
+    md->p.thermo_rate = 0;
+    md->retune = true;
+    cmd_status st = run_steps( md, k );
+    md->p.every = every;
+    md->p.thermo_rate = thermo;
+    md->retune = false;
+    CMD_TRY( st );
+    md->F->lanes = md->N->lanes;
+    // back to the initial state; the caller rebuilds in the chosen layout
+    md->step = 0;
+    md->thermo.clear();
+    for ( double &t : md->timers )
+        t = 0.0;
+    md->prof_launches = 0;
+    md->prof_ms = 0.0;
+    CMD_TRY( cmd_system_upload( S, n, 0, x.data(), v.data(), nullptr, type.data(), id.data() ) );
+    CMD_HIP( hipStreamSynchronize( S->stream ) );
+    md->tune_s = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
+    return CMD_OK;
+}
+
 } // namespace cmd
 
 using namespace cmd;
@@ -578,18 +626,14 @@ cmd_status cmd_md_init( cmd_md *md )
         CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
     else if ( !nnp && p.half )
         CMD_TRY( cmd_neigh_set_tile( md->N, 6 ) ); // measured best for half lists
+    if ( md->lanes == 0 && !p.half && !nnp )
+        CMD_TRY( tune_layout( md, n, x, v, type, id ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
     CMD_TRY( cmd_comm_exchange( S ) );
     CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut,
                                  md->neigh_cut, 1, 1, 0, 1, nullptr ) );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
-    if ( md->lanes == 0 && !p.half && !nnp )
-    {
-        CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
-        md->retune = true;
-    }
-    else
-        CMD_TRY( cmd_neigh_build( md->N, S ) );
+    CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->F->lanes = md->N->lanes;
     CMD_TRY( cmd_force_compute( md->F, S, md->N ) );
     if ( p.half || nnp )
@@ -855,6 +899,19 @@ cmd_status cmd_md_profile( cmd_md *md, int enable )
     md->prof_ms = 0.0;
     for ( int k = 0; k < 6; ++k )
         md->timers[k] = 0.0;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile, double *seconds )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( md->inited, CMD_ERR_STATE, "tune info before init" );
+    if ( lanes )
+        *lanes = md->N->lanes;
+    if ( tile )
+        *tile = md->N->tiled ? md->N->tg.E : 0;
+    if ( seconds )
+        *seconds = md->tune_s;
     return CMD_OK;
 }
 

@@ -711,7 +711,13 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             // slab wave of the tile
             lds = tile_planes_bytes( h[0] + 1 ) + 64 * sizeof( unsigned short ) +
                   (size_t)std::max( h[2], 1 ) * wbytes;
-            if ( h[0] <= MAX_SLOTS && lds <= hard )
+            // the force kernels' records on the same tiles (lj.hip): a
+            // position record and, with several types, a type per slot;
+            // the half list adds a reaction record and an energy weight
+            const size_t slot_bytes = TILE_REC + sizeof( int );
+            const size_t force_lds =
+                (size_t)( h[0] + 1 ) * slot_bytes * ( N->half ? 2 : 1 ) + 8;
+            if ( h[0] <= MAX_SLOTS && lds <= hard && force_lds <= hard )
                 break;
             if ( E == 1 )
                 return CMD_OK; // too dense for LDS tiles: global path

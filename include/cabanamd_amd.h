@@ -384,6 +384,11 @@ cmd_status cmd_md_timers( const cmd_md *md, double out[6] );
 cmd_status cmd_md_profile( cmd_md *md, int enable );
 cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
                                 double *mean_ms );
+/* The list layout init chose (lanes per atom, tile height in cells, 0 for
+ * the global slab) and the host seconds init spent choosing it; no
+ * reference counterpart (the reference's layout is fixed at compile time,
+ * neighbor_verlet.h:30-32). */
+cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile, double *seconds );
 
 /* ---- batched ensemble -------------------------------------------------------
  * No reference counterpart: CabanaMD steps one system per process.  A batch

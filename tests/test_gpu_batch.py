@@ -182,3 +182,32 @@ def test_batch_graph_replays_equal_eager(gpu, prec):
     assert all(g.info(a)["state"] == "running" for a in hg)
     assert g.graph_steps() > 40
     assert e.graph_steps() == 0
+
+
+def test_batch_precision_switch_with_graphs(gpu):
+    """Switching precision between runs takes effect at once, also for the
+    steps that replay captured graphs: a batch with graphs and one launched
+    kernel by kernel, both switched FP64 -> mixed -> FP64 mid-way between
+    rebuilds, stay bit-identical, and differ from a batch that stays FP64."""
+    M = gpu
+    g = M.Batch(_params(M, thermo=10), M.PREC_FP64, lanes=2, graphs=True)
+    e = M.Batch(_params(M, thermo=10), M.PREC_FP64, lanes=2, graphs=False)
+    s = M.Batch(_params(M, thermo=10), M.PREC_FP64, lanes=2, graphs=True)
+    hg = [_add(M, g, sp) for sp in SYSTEMS[:2]]
+    he = [_add(M, e, sp) for sp in SYSTEMS[:2]]
+    hs = [_add(M, s, sp) for sp in SYSTEMS[:2]]
+    for k, prec in ((25, None), (12, M.PREC_MIXED), (14, M.PREC_FP64), (9, None)):
+        if prec is not None:
+            g.set_precision(prec)
+            e.set_precision(prec)
+        g.run(k)
+        e.run(k)
+        s.run(k)
+        for a, b in zip(hg, he):
+            for u, w in zip(g.download(a), e.download(b)):
+                assert np.array_equal(u, w)
+            assert np.array_equal(g.thermo(a), e.thermo(b))
+    assert g.graph_steps() > 20
+    xg = g.download(hg[0])[0]
+    xs = s.download(hs[0])[0]
+    assert not np.array_equal(xg, xs)

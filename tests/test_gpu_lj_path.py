@@ -565,3 +565,34 @@ def test_ghost_rows_after_halo(gpu):
         shift = (x[n:] - x[src]) / L
         assert np.allclose(shift, np.round(shift), atol=1e-9)
         assert np.array_equal(t[n:], t[src])
+
+
+def test_init_tuning_restores_initial_state(gpu):
+    """Init chooses the list layout on a scratch run (20 steps, layouts
+    timed at its list rebuild), then uploads the initial state again: x and
+    v after init equal an untuned init's, the step-0 thermo row agrees and
+    the run starts at step 0."""
+    M = gpu
+    kw = dict(ncells=(12, 12, 12), nsteps=40, thermo=20)
+    a = M.CbnMD(M.lattice_params(**kw), precision=M.PREC_FP64)
+    a.init()
+    b = M.CbnMD(M.lattice_params(**kw), precision=M.PREC_FP64, lanes=1)
+    b.init()
+    g, tile, secs = a.tune_info()
+    print(f"tuned layout: lanes {g}, tile {tile}, {secs:.3f} s", flush=True)
+    assert g in (1, 2, 4, 8) and secs > 0.0
+    xa, va, _, _, ia = a.arrays()
+    xb, vb, _, _, ib = b.arrays()
+    n = a.n_local
+    assert n == b.n_local
+    assert np.array_equal(ia[:n], ib[:n])
+    assert np.array_equal(xa[:n], xb[:n])
+    assert np.array_equal(va[:n], vb[:n])
+    ta, tb = a.thermo(), b.thermo()
+    assert ta.shape == tb.shape == (1, 5)
+    assert np.allclose(ta, tb, rtol=1e-12, atol=1e-14)
+    a.run(40)
+    b.run(40)
+    ta, tb = a.thermo(), b.thermo()
+    assert np.array_equal(ta[:, 0], [0, 20, 40])
+    assert np.allclose(ta[:, :4], tb[:, :4], rtol=1e-10, atol=1e-13)
