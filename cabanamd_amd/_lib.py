@@ -16,6 +16,7 @@ LP = C.POINTER(C.c_int64)
 VP = C.c_void_p
 
 CMD_OK = 0
+CMD_ERR_CAPACITY = 5
 CMD_ERR_NO_DEVICE = 7
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -136,6 +137,7 @@ _PROTOS = {
                                 IP, IP], None),
     "cmd_thermo_temperature": ([VP, DP], None),
     "cmd_thermo_kinetic": ([VP, DP], None),
+    "cmd_thermo_kinetic_noalloc": ([VP, DP], None),
     "cmd_params_default": ([C.POINTER(Params)], None),
     "cmd_params_parse_deck": ([C.c_char_p, C.c_int, C.POINTER(Params)], None),
     "cmd_lattice_create": ([C.POINTER(Params), C.POINTER(Domain), IP, DP, DP,

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 
 namespace cmd
 {
@@ -889,6 +890,36 @@ static cmd_status sum_mv2( cmd_system *S, double *out )
     return CMD_OK;
 }
 
+// ------------------------------------------------ no-allocation thermo
+// Scratch of cmd_thermo_kinetic_noalloc: four partial sums per 256-atom
+// block, then the four reduced sums.  Each device holds its own copy for the
+// life of the process (1 MiB + 32 B); host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+static constexpr int NOALLOC_MAX_BLOCKS = 32768; // 8,388,608 local atoms
+static __device__ __align__( 256 ) double scratch[4 * NOALLOC_MAX_BLOCKS + 4];
+
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+static cmd_status noalloc_scratch( double **out )
+{
+    int dev = 0;
+    CMD_HIP( hipGetDevice( &dev ) );
+    CMD_REQUIRE( dev >= 0 && dev < MAX_DEVICES, CMD_ERR_ARG, "device %d beyond %d", dev,
+                 MAX_DEVICES );
+    std::lock_guard<std::mutex> lock( g_scratch_mu );
+    if ( !g_scratch_ptr[dev] )
+    {
+        void *ptr = nullptr;
+        CMD_HIP( hipGetSymbolAddress( &ptr, HIP_SYMBOL( scratch ) ) );
+        g_scratch_ptr[dev] = static_cast<double *>( ptr );
+    }
+    *out = g_scratch_ptr[dev];
+    return CMD_OK;
+}
+
 // Streaming copy used to calibrate byte counters and achievable bandwidth.
 __global__ void k_copy_probe( const double4 *__restrict__ a,
                               double4 *__restrict__ b, size_t n )
@@ -1464,6 +1495,33 @@ cmd_status cmd_thermo_kinetic( cmd_system *s, double *ke )
     double mv2 = 0.0;
     CMD_TRY( sum_mv2( s, &mv2 ) );
     *ke = mv2 * ( 0.5 * s->mvv2e );
+    return CMD_OK;
+}
+
+// cmd_thermo_kinetic with its partial sums in the file-scope `scratch`
+// array instead of the system's lazily grown buffers: the same kernels in
+// the same order (identical bits), no allocation.  Every slot the reduction
+// reads is written first by the same call, so nothing needs clearing; calls
+// on one device share the array, so overlapping calls race.
+cmd_status cmd_thermo_kinetic_noalloc( cmd_system *s, double *ke )
+{
+    CMD_REQUIRE( s && ke, CMD_ERR_ARG, "NULL argument" );
+    const int n = s->n_local;
+    const int nb = nblocks_for( n );
+    CMD_REQUIRE( nb <= NOALLOC_MAX_BLOCKS, CMD_ERR_CAPACITY,
+                 "cmd_thermo_kinetic_noalloc takes at most %d local atoms (%d given)",
+                 NOALLOC_MAX_BLOCKS * TPB, n );
+    double *scr = nullptr;
+    CMD_TRY( noalloc_scratch( &scr ) );
+    double *res = scr + 4 * (size_t)NOALLOC_MAX_BLOCKS;
+    k_mv2_partials<<<nb, TPB, 0, s->stream>>>( n, s->cap, s->x.p, s->v.p, s->mass_table(),
+                                              scr );
+    CMD_LAUNCH_CHECK();
+    CMD_TRY( reduce_partials( scr, nb, res, s->stream ) );
+    double h[4];
+    CMD_HIP( hipMemcpyAsync( h, res, sizeof( h ), hipMemcpyDeviceToHost, s->stream ) );
+    CMD_HIP( hipStreamSynchronize( s->stream ) );
+    *ke = h[0] * ( 0.5 * s->mvv2e );
     return CMD_OK;
 }
 

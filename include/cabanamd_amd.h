@@ -263,6 +263,13 @@ cmd_status cmd_nnp_lattice_create( int nx, int ny, int nz, double s,
  * property_kine_impl.h:55-76). */
 cmd_status cmd_thermo_temperature( cmd_system *s, double *T );
 cmd_status cmd_thermo_kinetic( cmd_system *s, double *ke );
+/* No reference counterpart: cmd_thermo_kinetic without any device
+ * allocation, for callers that cannot allocate (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Its partial sums live in
+ * a fixed per-device array (1 MiB); at most 8,388,608 local atoms, more is
+ * CMD_ERR_CAPACITY.  Same kernels and result bits as cmd_thermo_kinetic;
+ * calls that overlap on one device race on the array. */
+cmd_status cmd_thermo_kinetic_noalloc( cmd_system *s, double *ke );
 
 /* ---- run parameters and input deck ----------------------------------------
  * InputFile<System> (inputFile.h:150-275; inputFile_impl.h:125-534). */
