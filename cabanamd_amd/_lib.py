@@ -21,6 +21,7 @@ CMD_ERR_NO_DEVICE = 7
 PREC_FP64 = 0
 PREC_MIXED = 1
 MAX_TYPES = 8
+MAX_REGIONS = 16
 
 
 class CmdError(RuntimeError):
@@ -61,7 +62,14 @@ class Params(C.Structure):
                 ("coeff_cut", C.c_double * 16),
                 ("force_type", C.c_int), ("nnp_dir", C.c_char * 256),
                 ("atom_style_charge", C.c_int), ("data_in", C.c_char * 256),
-                ("data_out", C.c_char * 256)]
+                ("data_out", C.c_char * 256),
+                ("nregions", C.c_int),
+                ("region_name", (C.c_char * 32) * MAX_REGIONS),
+                ("region_block", (C.c_double * 6) * MAX_REGIONS),
+                ("region_type", C.c_int * MAX_REGIONS),
+                ("type_temp", C.c_double * MAX_TYPES),
+                ("type_seed", C.c_int * MAX_TYPES),
+                ("vacuum_rate", C.c_double)]
 
 
 # name -> argtypes (restype is always c_int = cmd_status unless listed)

@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <unordered_map>
+#include <cstdint>
 
 #include "../../include/cabanamd_amd.h"
 
@@ -102,6 +104,65 @@ class VelocityRng
     }
 };
 
+// rand() of the C library as a fresh reference process sees it (never
+// seeded, so seed 1).  glibc's generator is additive feedback of degree 31
+// and separation 3 (TYPE_3 of random_r.c), restated from its published
+// description: r_0 = seed; r_i = 16807 r_{i-1} mod (2^31 - 1) for
+// 0 < i < 31; r_i = r_{i-31} for 31 <= i < 34; then r_i = r_{i-31} + r_{i-3}
+// (mod 2^32) and the k-th call returns r_{k+344} >> 1.  The engine keeps its
+// own copy, restarted for every lattice, so its types never depend on other
+// users of rand() in the process.
+class CRand
+{
+    uint32_t r_[34];
+    int i_ = 0; // index of the next term, modulo 34
+
+    uint32_t step()
+    {
+        const uint32_t v = r_[( i_ + 34 - 31 ) % 34] + r_[( i_ + 34 - 3 ) % 34];
+        r_[i_] = v;
+        i_ = ( i_ + 1 ) % 34;
+        return v;
+    }
+
+  public:
+    explicit CRand( uint32_t seed = 1 )
+    {
+        int32_t r = (int32_t)( seed == 0 ? 1 : seed );
+        r_[0] = (uint32_t)r;
+        for ( int i = 1; i < 31; ++i )
+        {
+            r = (int32_t)( ( 16807LL * r ) % 2147483647LL );
+            if ( r < 0 )
+                r += 2147483647;
+            r_[i] = (uint32_t)r;
+        }
+        for ( int i = 31; i < 34; ++i )
+            r_[i] = r_[i - 31];
+        i_ = 0; // r_34 overwrites the slot of r_0
+        for ( int i = 34; i < 344; ++i )
+            step();
+    }
+    int next() { return (int)( step() >> 1 ); }
+};
+
+// The deck's named region blocks in the iteration order of the reference's
+// std::unordered_map<std::string, Block> (inputFile.h:172-173), which
+// decides both which region 'create_atoms T box' names (the map's first
+// entry, inputFile_impl.h:287-292) and the order get_regions lists the
+// regions of a site in (inputFile.h:203-216).  The same container type,
+// filled with the same names in the same order, iterates in the same order.
+std::vector<int> region_order( const cmd_params *p )
+{
+    std::unordered_map<std::string, int> m;
+    for ( int k = 0; k < p->nregions; ++k )
+        m[std::string( p->region_name[k] )] = k;
+    std::vector<int> order;
+    for ( const auto &kv : m )
+        order.push_back( kv.second );
+    return order;
+}
+
 // Lattice loop bounds and the half-open site test (inputFile_impl.h:562-588,
 // 598-606, 692-699).
 struct SiteWalk
@@ -135,12 +196,14 @@ inline bool site_inside( const cmd_domain *d, const SiteWalk &w, double x,
            ( y < w.maxv[1] ) && ( z < w.maxv[2] );
 }
 
-template <class Emit>
-int walk_sites( const cmd_params *p, const cmd_domain *d, Emit emit )
+// Every lattice site inside the box, in the reference's loop order, passed
+// to visit(x, y, z, sc) (sc: the simple cubic lattice, whose sites take a
+// random type instead of a region's).
+template <class Visit>
+void walk_sites( const cmd_params *p, const cmd_domain *d, Visit visit )
 {
     const double a = p->lattice_constant;
     const SiteWalk w = site_walk( p, d );
-    int n = 0;
     if ( p->lattice_style == 0 )
     {
         // simple cubic (inputFile_impl.h:590-655)
@@ -154,11 +217,11 @@ int walk_sites( const cmd_params *p, const cmd_domain *d, Emit emit )
                 {
                     const double x = a * ( ix + p->lattice_offset[0] );
                     if ( site_inside( d, w, x, y, z ) )
-                        emit( n++, x, y, z );
+                        visit( x, y, z, true );
                 }
             }
         }
-        return n;
+        return;
     }
     // face-centred cubic, 4-site basis (inputFile_impl.h:658-792)
     const double base[4][3] = { { 0.0, 0.0, 0.0 },
@@ -178,8 +241,52 @@ int walk_sites( const cmd_params *p, const cmd_domain *d, Emit emit )
                     const double y = a * ( 1.0 * iy + basis[b][1] );
                     const double z = a * ( 1.0 * iz + basis[b][2] );
                     if ( site_inside( d, w, x, y, z ) )
-                        emit( n++, x, y, z );
+                        visit( x, y, z, false );
                 }
+}
+
+// Atom creation over the sites (inputFile_impl.h:658-792): an fcc site
+// becomes an atom when it lies in a region that create_atoms gave a type;
+// of several such regions, rand() picks one in the map's iteration order
+// (one draw per atom).  sc sites are all created with type rand() % ntypes.
+// emit(k, x, y, z, type0) gets the atom's index and 0-based type.
+template <class Emit>
+int create_sites( const cmd_params *p, const cmd_domain *d, bool draw, Emit emit )
+{
+    const double a = p->lattice_constant;
+    const std::vector<int> order = region_order( p );
+    CRand crand;
+    std::vector<int> hits;
+    int n = 0;
+    walk_sites( p, d,
+                [&]( double x, double y, double z, bool sc )
+                {
+                    if ( sc )
+                    {
+                        const int t = draw ? crand.next() % p->ntypes : 0;
+                        emit( n++, x, y, z, t );
+                        return;
+                    }
+                    int type1 = 1; // the single block of type 1
+                    if ( p->nregions > 0 )
+                    {
+                        hits.clear();
+                        for ( int k : order )
+                        {
+                            const double *b = p->region_block[k];
+                            if ( p->region_type[k] > 0 && x >= a * b[0] && y >= a * b[2] &&
+                                 z >= a * b[4] && x < a * b[1] && y < a * b[3] && z < a * b[5] )
+                                hits.push_back( k );
+                        }
+                        if ( hits.empty() )
+                            return;
+                        if ( draw )
+                            type1 = p->region_type[hits[crand.next() % (int)hits.size()]];
+                    }
+                    else if ( draw )
+                        crand.next();
+                    emit( n++, x, y, z, type1 - 1 );
+                } );
     return n;
 }
 
@@ -221,6 +328,7 @@ cmd_status cmd_params_default( cmd_params *p )
     p->mvv2e = 1.0;
     p->boltz = 1.0;
     p->force_type = 0;
+    p->vacuum_rate = 1.0;
     return CMD_OK;
 }
 
@@ -241,6 +349,8 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
     cmd_params_default( p );
     bool timestepflag = false;
     bool have_region = false;
+    // the reference's regions map (inputFile.h:172): id -> slot of p
+    std::unordered_map<std::string, int> regions;
     std::string line;
     int lineno = 0;
     while ( std::getline( in, line ) )
@@ -335,18 +445,36 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
             }
             else if ( key == "region" )
             {
+                // inputFile_impl.h:241-266: regions[id] = block (a second
+                // definition of an id replaces its block), and the bounds
+                // of all blocks
                 if ( !need( 9 ) || w[2] != "block" )
                     return bad( "'region' command only supports 'block' option "
                                 "in CabanaMD" );
-                if ( have_region )
-                    return bad( "only one region is supported by this engine" );
+                if ( w[1].size() >= sizeof( p->region_name[0] ) )
+                    return bad( "region id too long" );
+                auto it = regions.find( w[1] );
+                int k = 0;
+                if ( it == regions.end() )
+                {
+                    if ( p->nregions >= CMD_MAX_REGIONS )
+                        return bad( "too many regions" );
+                    k = p->nregions++;
+                    regions[w[1]] = k;
+                    std::snprintf( p->region_name[k], sizeof( p->region_name[k] ), "%s",
+                                   w[1].c_str() );
+                }
+                else
+                    k = it->second;
+                for ( int c = 0; c < 6; ++c )
+                    p->region_block[k][c] = std::stod( w[3 + c] );
+                for ( int c = 0; c < 3; ++c )
+                {
+                    const double lo = p->region_block[k][2 * c], hi = p->region_block[k][2 * c + 1];
+                    p->region_lo[c] = have_region ? std::min( p->region_lo[c], lo ) : lo;
+                    p->region_hi[c] = have_region ? std::max( p->region_hi[c], hi ) : hi;
+                }
                 have_region = true;
-                p->region_lo[0] = std::stod( w[3] );
-                p->region_hi[0] = std::stod( w[4] );
-                p->region_lo[1] = std::stod( w[5] );
-                p->region_hi[1] = std::stod( w[6] );
-                p->region_lo[2] = std::stod( w[7] );
-                p->region_hi[2] = std::stod( w[8] );
             }
             else if ( key == "create_box" )
             {
@@ -358,12 +486,29 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
             }
             else if ( key == "create_atoms" )
             {
+                // inputFile_impl.h:271-297: regions_to_type[id] = type; 'box'
+                // names the first region of the map
                 if ( !need( 3 ) || ( w[2] != "box" && w[2] != "region" ) )
                     return bad( "'create_atoms' command only supports 'region' "
                                 "option in CabanaMD" );
-                if ( std::stoi( w[1] ) != 1 )
-                    return bad( "create_atoms supports atom type 1 only in this "
-                                "engine" );
+                const int t = std::stoi( w[1] );
+                if ( t < 1 || t > CMD_MAX_TYPES )
+                    return bad( "create_atoms type outside [1,8]" );
+                int k = -1;
+                if ( w[2] == "region" )
+                {
+                    if ( !need( 4 ) || regions.find( w[3] ) == regions.end() )
+                        return bad( "region '" + ( need( 4 ) ? w[3] : std::string() ) +
+                                    "' is not defined" );
+                    k = regions[w[3]];
+                }
+                else
+                {
+                    if ( regions.empty() )
+                        return bad( "create_atoms box before any region" );
+                    k = regions.begin()->second;
+                }
+                p->region_type[k] = t;
             }
             else if ( key == "mass" )
             {
@@ -441,12 +586,21 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
                 if ( !need( 5 ) || w[2] != "create" )
                     return bad( "'velocity' command can only be used with option "
                                 "'create' in CabanaMD" );
+                // type_to_temperature[type] (inputFile_impl.h:396-410);
+                // 'all' is keyed as type 1
                 const int t = ( w[1] == "all" ) ? 1 : std::stoi( w[1] );
-                if ( t != 1 )
-                    return bad( "velocity create is supported for 'all' (type 1) "
-                                "only in this engine" );
-                p->temp_target = std::stod( w[3] );
-                p->temp_seed = std::stoi( w[4] );
+                if ( t < 1 || t > CMD_MAX_TYPES )
+                    return bad( "velocity type outside [1,8]" );
+                if ( t == 1 )
+                {
+                    p->temp_target = std::stod( w[3] );
+                    p->temp_seed = std::stoi( w[4] );
+                }
+                else
+                {
+                    p->type_temp[t - 1] = std::stod( w[3] );
+                    p->type_seed[t - 1] = std::stoi( w[4] );
+                }
                 for ( size_t k = 5; k + 1 < w.size(); ++k )
                     if ( w[k] == "dist" && w[k + 1] == "gaussian" )
                         p->vel_gaussian = 1;
@@ -539,9 +693,10 @@ cmd_status cmd_params_parse_deck( const char *path, int force_iteration,
     return CMD_OK;
 }
 
-// create_lattice (inputFile_impl.h:536-865): sites, ids and types, then
-// velocities (loop-geom uniform or gaussian), zero net momentum and a
-// rescale to the target temperature.  Atoms are type 0 (create_atoms 1).
+// create_lattice (inputFile_impl.h:536-865): sites, ids and types (by
+// region), then velocities (loop-geom uniform or gaussian, seeded per
+// type), zero net momentum and a rescale of each atom to its type's target
+// temperature.
 cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
                                double *x, double *v, int *type, int *id )
 {
@@ -555,31 +710,42 @@ cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
         cmd::set_error( "lattice constant must be positive" );
         return CMD_ERR_ARG;
     }
-    const int count = walk_sites( p, d, []( int, double, double, double ) {} );
+    const int count =
+        create_sites( p, d, false, []( int, double, double, double, int ) {} );
     *n = count;
     if ( !x )
         return CMD_OK;
-    walk_sites( p, d,
-                [&]( int k, double xs, double ys, double zs )
-                {
-                    x[3 * k + 0] = xs;
-                    x[3 * k + 1] = ys;
-                    x[3 * k + 2] = zs;
-                    if ( type )
-                        type[k] = 0;
-                    if ( id )
-                        id[k] = k + 1;
-                } );
+    std::vector<int> types( count, 0 );
+    create_sites( p, d, true,
+                  [&]( int k, double xs, double ys, double zs, int t )
+                  {
+                      x[3 * k + 0] = xs;
+                      x[3 * k + 1] = ys;
+                      x[3 * k + 2] = zs;
+                      types[k] = t;
+                      if ( type )
+                          type[k] = t;
+                      if ( id )
+                          id[k] = k + 1;
+                  } );
+    for ( int t : types )
+        if ( t < 0 || t >= p->ntypes )
+        {
+            cmd::set_error( "create_atoms type %d outside create_box's %d types", t + 1,
+                            p->ntypes );
+            return CMD_ERR_ARG;
+        }
     if ( !v )
         return CMD_OK;
+    // type_to_temperature: type 1 from temp_target/temp_seed, others per type
+    auto vel_seed = [&]( int t ) { return t == 0 ? p->temp_seed : p->type_seed[t]; };
+    auto vel_temp = [&]( int t ) { return t == 0 ? p->temp_target : p->type_temp[t]; };
     double total_mass = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     VelocityRng rng;
     for ( int i = 0; i < count; ++i )
     {
-        const int t = type ? type[i] : 0;
-        // type_to_temperature holds the 'velocity all' entry under key 1
-        const int seed_base = ( t + 1 == 1 ) ? p->temp_seed : 0;
-        rng.reset( seed_base, &x[3 * i] );
+        const int t = types[i];
+        rng.reset( vel_seed( t ), &x[3 * i] );
         const double m = p->mass[t];
         double vx, vy, vz;
         if ( p->vel_gaussian )
@@ -616,7 +782,7 @@ cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
     double comp = 0.0;
     for ( int i = 0; i < count; ++i )
     {
-        const int t = type ? type[i] : 0;
+        const int t = types[i];
         const double term = ( v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
                               v[3 * i + 2] * v[3 * i + 2] ) *
                             p->mass[t];
@@ -632,9 +798,7 @@ cmd_status cmd_lattice_create( const cmd_params *p, const cmd_domain *d, int *n,
     const double T = mv2 * ( p->mvv2e / ( 1.0 * dof * p->boltz ) );
     for ( int i = 0; i < count; ++i )
     {
-        const int t = type ? type[i] : 0;
-        const double target = ( t + 1 == 1 ) ? p->temp_target : 0.0;
-        const double s = std::sqrt( target / T );
+        const double s = std::sqrt( vel_temp( types[i] ) / T );
         v[3 * i + 0] *= s;
         v[3 * i + 1] *= s;
         v[3 * i + 2] *= s;

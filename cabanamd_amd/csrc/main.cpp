@@ -15,6 +15,7 @@
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <array>
 #include <string>
 
 namespace
@@ -26,6 +27,7 @@ struct Options
     int precision = CMD_PREC_FP64;
     int lanes = 0; // 0: autotune
     int dump_rate = 0, check_rate = 0; // --dumpbinary / --correctness
+    double vacuum_rate = 1.0;          // --vacuum
     std::string dump_path, ref_path, check_file;
 };
 
@@ -43,7 +45,8 @@ void usage( std::ostream &o )
          "  --lanes 0|1|2|4|8          lanes per atom, 0 = autotune\n"
          "  --dumpbinary N PATH        snapshot every N steps to PATH/output.*\n"
          "  --correctness N PATH FILE  compare with PATH/output.* every N steps,\n"
-         "                             differences appended to FILE\n";
+         "                             differences appended to FILE\n"
+         "  --vacuum N                 scale the box's high corner by N > 1\n";
 }
 
 [[noreturn]] void fail( const std::string &msg )
@@ -130,6 +133,13 @@ Options parse( int argc, char **argv )
         }
         else if ( a == "--lanes" )
             o.lanes = std::stoi( value( i ) );
+        else if ( a == "--vacuum" )
+        {
+            // inputCL.cpp:243-250
+            o.vacuum_rate = std::atof( value( i ).c_str() );
+            if ( o.vacuum_rate <= 1.0 )
+                fail( "Vacuum multiplier must be bigger than 1.0" );
+        }
         else
             fail( "Unknown commandline option: " + a );
     }
@@ -182,11 +192,14 @@ int main( int argc, char **argv )
         check( cmd_device_init( 0 ) );
         InputFile input;
         input.read_file( o.input.c_str(), o.force_iteration );
+        input.params.vacuum_rate = o.vacuum_rate;
         const cmd_params &p = input.params;
         CbnMD md( p );
         md.set_precision( o.precision );
         md.set_lanes( o.lanes );
         md.init();
+        // phase times (T_Force, T_Neigh, T_Comm, T_Int) from HIP events
+        check( cmd_md_profile( md.handle(), 1 ) );
         const int thermo = p.thermo_rate > 0 ? p.thermo_rate : p.nsteps;
         auto rows = md.thermo();
         int N = 0;
@@ -196,11 +209,18 @@ int main( int argc, char **argv )
             int nl, ng;
             check( cmd_system_counts( s, &N, &nl, &ng ) );
         }
-        out << "\n#Timestep Temperature PotE ETot Time Atomsteps/s\n";
+        // print_summary (cabanamd_impl.h:647-668): step, then T, PE/N and
+        // E/N fixed with 6 digits, time fixed with 2, the rate scientific
+        // with 2
+        auto row = [&]( int step, const std::array<double, 5> &r, double time, double rate )
+        {
+            out << step << "\t" << fmt( "%.6f", r[1] ) << "\t" << fmt( "%.6f", r[2] ) << "\t"
+                << fmt( "%.6f", r[3] ) << "\t" << fmt( "%.2f", time ) << "\t"
+                << fmt( "%.2e", rate ) << "\n";
+        };
+        out << "\n#Timestep Temperature PotE ETot Time Atomsteps/s \n";
         if ( !rows.empty() )
-            out << 0 << "\t" << fmt( "%.6f", rows[0][1] ) << "\t"
-                << fmt( "%.6f", rows[0][2] ) << "\t" << fmt( "%.6f", rows[0][3] )
-                << "\t" << fmt( "%.2f", 0.0 ) << "\t" << fmt( "%e", 0.0 ) << "\n";
+            row( 0, rows[0], 0.0, 0.0 );
         // dump_binary / check_correctness at step 0 and every rate steps
         // (cabanamd_impl.h:245-251, 392-396)
         auto at_step = [&]( int step )
@@ -217,7 +237,7 @@ int main( int argc, char **argv )
         at_step( 0 );
         using clock = std::chrono::steady_clock;
         const auto t0 = clock::now();
-        double last = 0.0;
+        double last = 0.0, last_row = 0.0;
         int done = 0;
         while ( done < p.nsteps )
         {
@@ -234,26 +254,29 @@ int main( int argc, char **argv )
             rows = md.thermo();
             if ( done % thermo == 0 && !rows.empty() )
             {
-                const auto &r = rows.back();
-                out << done << "\t" << fmt( "%.6f", r[1] ) << "\t"
-                    << fmt( "%.6f", r[2] ) << "\t" << fmt( "%.6f", r[3] ) << "\t"
-                    << fmt( "%.2f", now ) << "\t"
-                    << fmt( "%e", 1.0 * N * chunk / ( now - last ) ) << "\n";
+                // rate over the steps since the last row (cabanamd_impl.h:367-368)
+                row( done, rows.back(), now, 1.0 * N * thermo / ( now - last_row ) );
+                last_row = now;
             }
             last = now;
         }
+        // the final block (cabanamd_impl.h:400-421): one process, no load
+        // balancer (T_lb 0), everything not in a phase is T_Other
         const double time = last;
         const auto t = md.timers();
         const double other = std::max( 0.0, time - t[0] - t[1] - t[2] - t[3] );
+        const double ph[6] = { t[0], t[1], t[2], t[3], 0.0, other };
         out << "\n#Procs Atoms | Time T_Force T_Neigh T_Comm T_Int T_lb T_Other |\n"
-            << 1 << " " << N << " | " << fmt( "%.2f", time ) << " "
-            << fmt( "%.2f", t[0] ) << " " << fmt( "%.2f", t[1] ) << " "
-            << fmt( "%.2f", t[2] ) << " " << fmt( "%.2f", t[3] ) << " "
-            << fmt( "%.2f", 0.0 ) << " " << fmt( "%.2f", other )
-            << " | PERFORMANCE\n";
+            << 1 << " " << N << " | " << fmt( "%.2f", time );
+        for ( double v : ph )
+            out << " " << fmt( "%.2f", v );
+        out << " | PERFORMANCE\n" << 1 << " " << N << " | " << fmt( "%.2f", 1.0 );
+        for ( double v : ph )
+            out << " " << fmt( "%.2f", time > 0 ? v / time : 0.0 );
+        out << " | FRACTION\n\n";
         const double sps = time > 0 ? p.nsteps / time : 0.0;
-        out << "\n#Steps/s Atomsteps/s Atomsteps/(proc*s)\n" << fmt( "%e", sps )
-            << " " << fmt( "%e", sps * N ) << " " << fmt( "%e", sps * N ) << "\n";
+        out << "#Steps/s Atomsteps/s Atomsteps/(proc*s)\n" << fmt( "%.2e", sps ) << " "
+            << fmt( "%.2e", sps * N ) << " " << fmt( "%.2e", sps * N ) << "\n";
     }
     catch ( const std::exception &e )
     {

@@ -531,6 +531,10 @@ cmd_status cmd_md_init( cmd_md *md )
     {
         low[k] = p.lattice_constant * p.region_lo[k];
         high[k] = p.lattice_constant * p.region_hi[k];
+        // --vacuum (inputFile_impl.h:553-559): the box grows, the lattice
+        // keeps the regions' bounds
+        if ( p.vacuum_rate > 1.0 )
+            high[k] *= p.vacuum_rate;
     }
     if ( !md->custom_state && p.data_in[0] )
     {

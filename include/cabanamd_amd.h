@@ -42,7 +42,8 @@ typedef enum cmd_status
 
 enum
 {
-    CMD_MAX_TYPES = 8
+    CMD_MAX_TYPES = 8,
+    CMD_MAX_REGIONS = 16
 };
 
 /* Force-kernel precision.  FP64: pair maths in double (the reference's).
@@ -300,6 +301,20 @@ typedef struct cmd_params
     int atom_style_charge; /* atom_style charge (else atomic) */
     char data_in[256];     /* read_data file ("" = lattice) */
     char data_out[256];    /* write_data file at the end of each run */
+    /* Named region blocks in deck order (inputFile_impl.h:241-266) and the
+     * create_atoms type of each (1-based, 0 = no atoms; :271-297).  With no
+     * named region, region_lo/hi is one block of type 1.  region_lo/hi
+     * always hold the bounding box of the blocks. */
+    int nregions;
+    char region_name[CMD_MAX_REGIONS][32];
+    double region_block[CMD_MAX_REGIONS][6]; /* xlo xhi ylo yhi zlo zhi */
+    int region_type[CMD_MAX_REGIONS];
+    /* velocity create per type (type_to_temperature, :396-410): type 1 (or
+     * 'all') is temp_target/temp_seed; type k > 1 is type_temp[k-1] and
+     * type_seed[k-1]; a type never named keeps temperature 0, seed 0. */
+    double type_temp[CMD_MAX_TYPES];
+    int type_seed[CMD_MAX_TYPES];
+    double vacuum_rate; /* --vacuum: box high corner scaled (inputCL.cpp:243-250); 1 = off */
 } cmd_params;
 
 /* Parse a LAMMPS-subset deck (inputFile_impl.h:141-533).  Unknown or

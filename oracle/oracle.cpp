@@ -16,7 +16,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdint>
 #include <limits>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -231,23 +234,119 @@ int lattice_walk( const orc_params *p, const orc_domain *d, Visit visit )
 }
 } // namespace
 
+/* glibc rand() from seed 1: r[0] = 1, r[i] = 16807 r[i-1] mod (2^31-1)
+ * (i < 31), r[i] = r[i-31] (31 <= i < 34), r[i] = r[i-31] + r[i-3] mod 2^32
+ * afterwards; call k returns r[344 + k] >> 1.  Kept as one growing array. */
+struct GlibcRand
+{
+    std::vector<uint32_t> r;
+    size_t k = 0;
+    GlibcRand()
+    {
+        r.resize( 344 );
+        r[0] = 1;
+        for ( int i = 1; i < 31; ++i )
+            r[i] = (uint32_t)( ( 16807ull * r[i - 1] ) % 2147483647ull );
+        for ( int i = 31; i < 34; ++i )
+            r[i] = r[i - 31];
+        for ( int i = 34; i < 344; ++i )
+            r[i] = r[i - 31] + r[i - 3];
+    }
+    int operator()()
+    {
+        const size_t i = 344 + k++;
+        r.push_back( r[i - 31] + r[i - 3] );
+        return (int)( r[i] >> 1 );
+    }
+};
+
+void orc_crand( int n, int *out )
+{
+    GlibcRand g;
+    for ( int i = 0; i < n; ++i )
+        out[i] = g();
+}
+
+/* Types of the lattice atoms (inputFile_impl.h:651, 755-777): sc sites draw
+ * rand() % ntypes; an fcc site is created when a region with a
+ * create_atoms type holds it, the type of one such region picked by
+ * rand() % count among them in the regions map's iteration order
+ * (std::unordered_map<std::string, Block>, inputFile.h:172). */
+struct LatticeTyping
+{
+    const orc_params *p;
+    std::vector<int> order; /* region slots in map iteration order */
+    GlibcRand g;
+    explicit LatticeTyping( const orc_params *pp ) : p( pp )
+    {
+        std::unordered_map<std::string, int> m;
+        for ( int k = 0; k < p->nregions; ++k )
+            m[p->region_name[k]] = k;
+        for ( auto it = m.begin(); it != m.end(); ++it )
+            order.push_back( it->second );
+    }
+    bool in_block( int k, double xt, double yt, double zt ) const
+    {
+        const double a = p->lattice_constant;
+        const double *b = p->region_block[k];
+        return xt >= a * b[0] && yt >= a * b[2] && zt >= a * b[4] && xt < a * b[1] &&
+               yt < a * b[3] && zt < a * b[5];
+    }
+    /* -1: no atom at this site; else the 0-based type (draws when asked) */
+    int type_at( double xt, double yt, double zt, bool draw )
+    {
+        if ( p->lattice_style == 0 )
+            return draw ? g() % p->ntypes : 0;
+        if ( p->nregions == 0 )
+        {
+            if ( draw )
+                g();
+            return 0;
+        }
+        std::vector<int> ids;
+        for ( int k : order )
+            if ( p->region_type[k] != 0 && in_block( k, xt, yt, zt ) )
+                ids.push_back( k );
+        if ( ids.empty() )
+            return -1;
+        if ( !draw )
+            return 0;
+        return p->region_type[ids[g() % (int)ids.size()]] - 1;
+    }
+};
+
 int orc_lattice_count( const orc_params *p, const orc_domain *d )
 {
-    return lattice_walk( p, d, []( int, double, double, double ) {} );
+    LatticeTyping ty( p );
+    int n = 0;
+    lattice_walk( p, d,
+                  [&]( int, double xt, double yt, double zt )
+                  {
+                      if ( ty.type_at( xt, yt, zt, false ) >= 0 )
+                          ++n;
+                  } );
+    return n;
 }
 
 int orc_lattice_fill( const orc_params *p, const orc_domain *d, double *x,
                       int *type, int *id )
 {
-    return lattice_walk( p, d,
-                         [&]( int n, double xt, double yt, double zt )
-                         {
-                             x[3 * n + 0] = xt;
-                             x[3 * n + 1] = yt;
-                             x[3 * n + 2] = zt;
-                             type[n] = 0;
-                             id[n] = n + 1;
-                         } );
+    LatticeTyping ty( p );
+    int n = 0;
+    lattice_walk( p, d,
+                  [&]( int, double xt, double yt, double zt )
+                  {
+                      const int t = ty.type_at( xt, yt, zt, true );
+                      if ( t < 0 )
+                          return;
+                      x[3 * n + 0] = xt;
+                      x[3 * n + 1] = yt;
+                      x[3 * n + 2] = zt;
+                      type[n] = t;
+                      id[n] = n + 1;
+                      ++n;
+                  } );
+    return n;
 }
 
 /* Velocities: inputFile_impl.h:795-865.  type_to_temperature holds only the
@@ -260,7 +359,7 @@ void orc_velocity_create( const orc_params *p, int n, const double *x,
     for ( int i = 0; i < n; ++i )
     {
         int key = type[i] + 1;
-        int seed_base = ( key == 1 ) ? p->temp_seed : 0;
+        int seed_base = ( key == 1 ) ? p->temp_seed : p->type_seed[key - 1];
         int seed = orc_rng_reset( seed_base, &x[3 * i] );
         double mass_i = p->mass[type[i]];
         double vx, vy, vz;
@@ -294,7 +393,7 @@ void orc_velocity_create( const orc_params *p, int n, const double *x,
     double T = orc_temperature( n, n, v, type, p->mass, p->mvv2e, p->boltz );
     for ( int i = 0; i < n; ++i )
     {
-        double target = ( type[i] + 1 == 1 ) ? p->temp_target : 0.0;
+        double target = ( type[i] + 1 == 1 ) ? p->temp_target : p->type_temp[type[i]];
         double s = std::sqrt( target / T );
         v[3 * i + 0] *= s;
         v[3 * i + 1] *= s;

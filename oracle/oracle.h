@@ -67,7 +67,23 @@ typedef struct orc_params
     int ncoeff;
     int coeff_i[16], coeff_j[16]; /* 1-based type ids as in pair_coeff */
     double coeff_eps[16], coeff_sigma[16], coeff_cut[16];
+    /* named region blocks in deck order and their create_atoms types
+     * (1-based, 0 none); nregions == 0: region_lo/hi is one block of type 1
+     * (inputFile_impl.h:241-297) */
+    int nregions;
+    char region_name[16][32];
+    double region_block[16][6]; /* xlo xhi ylo yhi zlo zhi, lattice units */
+    int region_type[16];
+    /* velocity create of types 2..8 (type 1: temp_target/temp_seed),
+     * inputFile_impl.h:396-410 */
+    double type_temp[8];
+    int type_seed[8];
 } orc_params;
+
+/* The first n values of the C library's rand() after srand(1) (glibc's
+ * TYPE_3 additive feedback generator), which the reference's lattice draws
+ * types with (inputFile_impl.h:651,766). */
+void orc_crand( int n, int *out );
 
 /* Count atoms the lattice would create (first pass), then fill. */
 int orc_lattice_count( const orc_params *p, const orc_domain *d );

@@ -46,7 +46,13 @@ class Params(C.Structure):
                 ("coeff_i", C.c_int * 16), ("coeff_j", C.c_int * 16),
                 ("coeff_eps", C.c_double * 16),
                 ("coeff_sigma", C.c_double * 16),
-                ("coeff_cut", C.c_double * 16)]
+                ("coeff_cut", C.c_double * 16),
+                ("nregions", C.c_int),
+                ("region_name", (C.c_char * 32) * 16),
+                ("region_block", (C.c_double * 6) * 16),
+                ("region_type", C.c_int * 16),
+                ("type_temp", C.c_double * 8),
+                ("type_seed", C.c_int * 8)]
 
 
 def lib():
@@ -141,6 +147,26 @@ def create_domain(low, high, ghost_cutoff):
     lib().orc_create_domain(_d3(low), _d3(high), C.c_double(ghost_cutoff),
                             C.byref(d))
     return d
+
+
+def crand(n):
+    """The first n values of the C library's rand() after srand(1)."""
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_crand(C.c_int(n), ip(out))
+    return out
+
+
+def params_from(src):
+    """An oracle Params holding the fields of an engine cmd_params (a deck
+    parsed by the engine) that the oracle has."""
+    p = Params()
+    names = {f[0] for f in Params._fields_}
+    for f in src._fields_:
+        if f[0] in names:
+            C.memmove(C.addressof(p) + getattr(Params, f[0]).offset,
+                      C.addressof(src) + getattr(type(src), f[0]).offset,
+                      C.sizeof(f[1]))
+    return p
 
 
 def rng_reset(ibase, coord):

@@ -27,6 +27,7 @@ def test_help():
     (["--neigh-type", "TREE_2D"], "Unknown commandline option"),
     (["--bogus"], "Unknown commandline option: --bogus"),
     ([], "No input file"),
+    (["-il", "in.lj", "--vacuum", "1.0"], "Vacuum multiplier must be bigger than 1.0"),
 ])
 def test_bad_options(args, msg):
     r = _run(*args)
