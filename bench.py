@@ -69,7 +69,7 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     t0 = time.time()
     md.init()
     t_init = time.time() - t0
-    g, tile, tune_s = md.tune_info()
+    g, tile, tune_s, width = md.tune_info()
     n = md.n_local
     if warmup:
         md.run(warmup)
@@ -85,7 +85,7 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
                atom_steps_per_s=n * steps / dt, kernel_launches=launches,
                kernel_mean_ms=mean_ms, init_s=t_init, tune_s=tune_s,
-               lanes=g, tile=tile,
+               lanes=g, tile=f"{width}x{width}x{tile}",
                energy_drift=_drift(th[:, 3]), thermo_rows=len(th),
                force_error_vs_fp64_oracle=ferr,
                timers=md.timers())

@@ -211,13 +211,14 @@ class NeighborVerlet:
     """NeighborVerlet<System, Full|Half, CSR> (neighbor_verlet.h:19-70)."""
 
     def __init__(self, neigh_cut, half_neigh, max_neigh_guess=50, lanes=1,
-                 tile=0):
+                 tile=0, tile_width=0):
         h = C.c_void_p()
         call("cmd_neigh_create", neigh_cut, int(half_neigh),
              int(max_neigh_guess), C.byref(h))
         self._h = h
         call("cmd_neigh_set_lanes", h, int(lanes))
         call("cmd_neigh_set_tile", h, int(tile))
+        call("cmd_neigh_set_tile_width", h, int(tile_width))
         self.neigh_cut = neigh_cut
         self.half_neigh = bool(half_neigh)
 
@@ -251,9 +252,11 @@ class NeighborVerlet:
 
     def layout(self):
         """Layout of the last build: tiled (bool), tile height, lanes."""
-        t, e, g = C.c_int(), C.c_int(), C.c_int()
+        t, e, g, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         call("cmd_neigh_layout", self._h, C.byref(t), C.byref(e), C.byref(g))
-        return dict(tiled=bool(t.value), tile_height=e.value, lanes=g.value)
+        call("cmd_neigh_tile_shape", self._h, C.byref(w), None)
+        return dict(tiled=bool(t.value), tile_height=e.value, lanes=g.value,
+                    tile_width=w.value)
 
     @property
     def max_neigh_guess(self):
@@ -635,7 +638,7 @@ class CbnMD:
     """CbnMD<System, Neighbor> (cabanamd_impl.h:70-432): init() then run()."""
 
     def __init__(self, params, precision=PREC_FP64, lanes=0, tile=0,
-                 fuse_ghosts=True):
+                 fuse_ghosts=True, tile_width=0):
         self.params = params
         h = C.c_void_p()
         call("cmd_md_create", C.byref(params), C.byref(h))
@@ -643,6 +646,7 @@ class CbnMD:
         call("cmd_md_set_precision", h, precision)
         call("cmd_md_set_lanes", h, lanes)
         call("cmd_md_set_tile", h, tile)
+        call("cmd_md_set_tile_width", h, tile_width)
         call("cmd_md_set_ghost_fusion", h, int(bool(fuse_ghosts)))
 
     def __del__(self):
@@ -748,10 +752,12 @@ class CbnMD:
         return n.value, ms.value
 
     def tune_info(self):
-        """(lanes per atom, tile height or 0, seconds init spent choosing)."""
-        g, e, s = C.c_int(), C.c_int(), C.c_double()
-        call("cmd_md_tune_info", self._h, C.byref(g), C.byref(e), C.byref(s))
-        return g.value, e.value, s.value
+        """(lanes per atom, tile height or 0, seconds init spent choosing,
+        tile width or 0)."""
+        g, w, e, s = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        call("cmd_md_tune_info", self._h, C.byref(g), C.byref(w), C.byref(e),
+             C.byref(s))
+        return g.value, e.value, s.value, w.value
 
 
 class Batch:

@@ -105,6 +105,8 @@ _PROTOS = {
     "cmd_neigh_destroy": ([VP], None),
     "cmd_neigh_set_lanes": ([VP, C.c_int], None),
     "cmd_neigh_set_tile": ([VP, C.c_int], None),
+    "cmd_neigh_set_tile_width": ([VP, C.c_int], None),
+    "cmd_neigh_tile_shape": ([VP, IP, IP], None),
     "cmd_neigh_build": ([VP, VP], None),
     "cmd_neigh_info": ([VP, IP, LP, IP, IP], None),
     "cmd_neigh_layout": ([VP, IP, IP, IP], None),
@@ -159,6 +161,7 @@ _PROTOS = {
     "cmd_md_set_precision": ([VP, C.c_int], None),
     "cmd_md_set_lanes": ([VP, C.c_int], None),
     "cmd_md_set_tile": ([VP, C.c_int], None),
+    "cmd_md_set_tile_width": ([VP, C.c_int], None),
     "cmd_md_set_ghost_fusion": ([VP, C.c_int], None),
     "cmd_md_set_nnp_model": ([VP, VP], None),
     "cmd_md_set_initial_state": ([VP, C.c_int, DP, DP, IP, IP, DP, DP], None),
@@ -171,7 +174,7 @@ _PROTOS = {
     "cmd_md_timers": ([VP, DP], None),
     "cmd_md_profile": ([VP, C.c_int], None),
     "cmd_md_kernel_stats": ([VP, IP, DP], None),
-    "cmd_md_tune_info": ([VP, IP, IP, DP], None),
+    "cmd_md_tune_info": ([VP, IP, IP, IP, DP], None),
     "cmd_md_dump": ([VP, C.c_char_p, C.c_int], None),
     "cmd_md_check_correctness": ([VP, C.c_char_p, C.c_int, C.c_char_p, DP],
                                  None),

@@ -171,8 +171,11 @@ struct TileGrid
 {
     CellGrid g;
     int nc0, nc1; // interior columns: a in [1, n0-2], b in [1, n1-2]
+    int W;        // tile width: W x W interior columns per tile (1 or 2)
+    int nt0, nt1; // tile columns: ceil(nc0 / W) x ceil(nc1 / W)
     int E, tpc;   // tile height in cells, tiles per column
     int ntiles;
+    int bw;       // waves per block of the tiled force kernels
 };
 
 // Counting sort of n positions into grid cells, stable by index.  Produces
@@ -275,6 +278,10 @@ struct cmd_neigh
     bool allow_tiles = true;
     bool tiled = false;          // the last build produced the tiled layout
     int tile_e = 0;              // tile height in cells (0 = choose)
+    int tile_w = 0;              // tile width in columns (0 = choose)
+    int tile_bw = 0;             // waves per force-kernel block (0 = one per slab wave, <= 8)
+    int tile_blocks = 0;         // force-kernel blocks per CU of the last tiled build
+    int occ_blocks = 0;          // keep at least this many (0: any), set by the autotuner
     cmd::TileGrid tg{};
     cmd::DevBuf<int> tile_info;  // TI_WORDS ints per tile
     cmd::DevBuf<int64_t> tile_off; // slab offset of each tile (16-bit units)

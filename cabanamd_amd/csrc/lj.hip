@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 namespace cmd
 {
@@ -144,10 +145,21 @@ __device__ __forceinline__ void accumulate( double &fx, double &fy, double &fz,
     }
 }
 
+// Sum of four values over the block into out[0..3]; red holds 4 x 8
+// doubles of LDS (up to 8 waves per block).
+__device__ void block_sum4_in( double a, double b, double c, double d,
+                               double *__restrict__ out, double ( *red )[8] );
+
 __device__ void block_sum4( double a, double b, double c, double d,
                             double *__restrict__ out )
 {
-    __shared__ double red[4][8]; // up to 8 waves per block
+    __shared__ double red[4][8];
+    block_sum4_in( a, b, c, d, out, red );
+}
+
+__device__ void block_sum4_in( double a, double b, double c, double d,
+                               double *__restrict__ out, double ( *red )[8] )
+{
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for ( int o = 32; o > 0; o >>= 1 )
@@ -453,8 +465,10 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
 // the tile's sentinel slot (a position far outside any cutoff), so all lanes
 // run the same number of iterations with no per-pair bounds test; inactive
 // lanes sit at another far point.  A padded pair adds exact zeros.
+// LDS keeps the tiled kernels at three or four waves per SIMD, so the
+// compiler may use registers for more loads and pair chains in flight.
 template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY>
-__global__ __launch_bounds__( TPB ) void k_tile_force(
+__global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 ) ) ) void k_tile_force(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
     int lcap, int scap, int slot_cap, const double4 *__restrict__ x,
     const double4 *__restrict__ xg, double4 *__restrict__ x_next,
@@ -470,17 +484,19 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
     // staging waves issue first: a block's tile head and records are on
     // the critical path of all its waves (1 % faster than equal priority)
     __builtin_amdgcn_s_setprio( 2 );
-    load_head( info, t, H );
+    load_head( info, t, T.W, H );
     const char *s_byte = reinterpret_cast<const char *>( s_rec );
     const int *s_t = reinterpret_cast<const int *>( s_rec + 3 * slot_cap );
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
     const int wbase = woff[t];
     const unsigned short *st = slab + (size_t)wbase * R4 * 256;
     const int nw = woff[t + 1] - wbase;
-    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
+    const int nat = H.info[TI_NAT];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nwb = blockDim.x >> 6;
     // four sentinel entries: the word a lane past the tile's atoms walks
-    const unsigned long long sent4 = 0x0001000100010001ull * (unsigned)( H.segoff[18] * TILE_REC );
+    const unsigned long long sent4 =
+        0x0001000100010001ull * (unsigned)( H.segoff[H.nseg] * TILE_REC );
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
     // What a slab wave reads from global memory before its pair loop: the
     // atom, its row length and the first three list words.  The first
@@ -501,7 +517,8 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         sub = lane / apw;
         const int ta = w * APW + a;
         active = a < APW && ta < nat;
-        i = tl + ta;
+        // lanes past the atoms take the tile's first atom (nat >= 1 here)
+        i = tile_atom( H, T.W, active ? ta : 0 );
     };
     auto row_of = [&]( int w ) -> const unsigned long long *
     {
@@ -515,9 +532,8 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         lane_map( w, gw, apw, sub, i, active );
         // loaded unconditionally (lanes past the atoms read the tile's first
         // atom) and replaced after staging, so no wait is needed here
-        const int ic = active ? i : tl;
-        P.xi = x[ic];
-        P.n = counts[ic];
+        P.xi = x[i];
+        P.n = counts[i];
         // the first three words of every lane's column (R4 >= 1 words each;
         // words past the row's end are loaded but not used)
         const unsigned long long *row8 = row_of( w );
@@ -533,7 +549,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         prep( wid, P );
     stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
     __builtin_amdgcn_s_setprio( 0 );
-    for ( int w = wid; w < nw; w += TPB / 64 )
+    for ( int w = wid; w < nw; w += nwb )
     {
         if ( w != wid )
             prep( w, P );
@@ -642,8 +658,14 @@ __global__ __launch_bounds__( TPB ) void k_tile_force(
         finish_atom<G, FUSED>( active, sub, i, xi, fx, fy, fz, scap, x_next, v,
                                f, mt, ctl, mv2, vpre, img_b, img_e );
     }
-    if ( FUSED && ctl.thermo )
-        block_sum4( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x );
+    if ( FUSED && ENERGY )
+    {
+        // the staged records are dead once every wave is past its rows:
+        // their first 256 bytes hold the block sums (no static LDS)
+        __syncthreads();
+        block_sum4_in( mv2, 0.5 * pe, 0.5 * vir, 0.0, partial + 4 * blockIdx.x,
+                       reinterpret_cast<double( * )[8]>( s_rec ) );
+    }
 }
 
 // LDS bytes before the reaction records of the tiled half-list kernel: the
@@ -665,7 +687,7 @@ __host__ __device__ constexpr size_t half_acc_offset( int slot_cap, bool one_typ
 // slots go to their ghost rows, which update_force folds.  Energy weights
 // follow compute_energy_half: 1 for a local j, 1/2 for a ghost j.
 template <int G, int PREC, bool ONE_TYPE, bool ENERGY>
-__global__ __launch_bounds__( TPB ) void k_tile_half(
+__global__ __launch_bounds__( 512 ) void k_tile_half(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
     int lcap, int scap, int slot_cap, int n_local, const double4 *__restrict__ x,
     const double4 *__restrict__ xg, const int *__restrict__ gperm,
@@ -677,13 +699,13 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
     __shared__ TileHead H;
     constexpr int APW = 64 / G;
     const int t = xcd_tile( blockIdx.x, gridDim.x );
-    load_head( info, t, H );
+    load_head( info, t, T.W, H );
     char *s_byte = reinterpret_cast<char *>( s_rec );
     const int *s_t = reinterpret_cast<const int *>( s_rec + 3 * slot_cap );
     // reaction records, then (energy only) a ghost flag per slot
     char *s_acc = s_byte + half_acc_offset( slot_cap, ONE_TYPE );
     int *s_gh = reinterpret_cast<int *>( s_acc + (size_t)slot_cap * TILE_REC );
-    const int total = H.segoff[18];
+    const int total = H.segoff[H.nseg];
     for ( int k = threadIdx.x; k < 3 * slot_cap; k += blockDim.x )
         reinterpret_cast<double *>( s_acc )[k] = 0.0;
     if ( ENERGY )
@@ -691,7 +713,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
         int seg = 0;
         for ( int k = threadIdx.x; k < slot_cap; k += blockDim.x )
         {
-            while ( seg < 18 && k >= H.segoff[seg + 1] )
+            while ( seg < H.nseg && k >= H.segoff[seg + 1] )
                 ++seg;
             s_gh[k] = ( seg & 1 ) && k < total;
         }
@@ -700,19 +722,27 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
     const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
     const int nw = woff[t + 1] - woff[t];
-    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
-    const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
+    const int nat = H.info[TI_NAT];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const unsigned long long sent4 = 0x0001000100010001ull * (unsigned)( H.segoff[18] * TILE_REC );
+    const int nwb = blockDim.x >> 6;
+    const unsigned long long sent4 =
+        0x0001000100010001ull * (unsigned)( H.segoff[H.nseg] * TILE_REC );
     double pe = 0.0, vir = 0.0;
-    for ( int w = wid; w < nw; w += TPB / 64 )
+    for ( int w = wid; w < nw; w += nwb )
     {
         // gw lanes per atom in this slab wave (G, or more in a part-full wave)
         const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
         const int a = lane % apw, sub = lane / apw;
         const int ta = w * APW + a;
         const bool active = a < APW && ta < nat;
-        const int i = tl + ta;
+        // the atom, its atom column and the LDS slot it is staged at (its
+        // staged column's local run)
+        int ac = 0;
+        while ( ac + 1 < T.W * T.W && ta >= H.aoff[ac + 1] )
+            ++ac;
+        const int i = H.info[TI_AL + ac] + ( ta - H.aoff[ac] );
+        const int qc = atom_col_q( T.W, ac );
+        const int islot = H.segoff[2 * qc] + ( i - H.info[TI_LB + qc] );
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const double4 xi =
             active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
@@ -795,7 +825,7 @@ __global__ __launch_bounds__( TPB ) void k_tile_half(
         }
         if ( active && sub == 0 )
         {
-            double *r = reinterpret_cast<double *>( s_acc + ( si0 + i ) * TILE_REC );
+            double *r = reinterpret_cast<double *>( s_acc + islot * TILE_REC );
             unsafeAtomicAdd( r, fx );
             unsafeAtomicAdd( r + 1, fy );
             unsafeAtomicAdd( r + 2, fz );
@@ -1054,8 +1084,10 @@ static void launch_tile_k( cmd_neigh *N, cmd_system *S, double4 *xn,
 {
     // staged records plus the sentinel (and an int type per slot)
     const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
-    const size_t lds = (size_t)slot_cap * ( TILE_REC + ( ONE ? 0 : sizeof( int ) ) );
-    k_tile_force<G, PREC, ONE, FUSED, ENERGY><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+    // at least the 4 x 8 doubles of the thermo block sums, which reuse it
+    const size_t lds = std::max<size_t>(
+        (size_t)slot_cap * ( TILE_REC + ( ONE ? 0 : sizeof( int ) ) ), 32 * sizeof( double ) );
+    k_tile_force<G, PREC, ONE, FUSED, ENERGY><<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
         N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
         S->x.p, S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt,
         ctl, partial );
@@ -1223,7 +1255,7 @@ static void launch_tile_half_k( const cmd_neigh *N, cmd_system *S, const LJParam
     const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
     const size_t lds = half_acc_offset( slot_cap, ONE ) + (size_t)slot_cap * TILE_REC +
                        ( ENERGY ? (size_t)slot_cap * sizeof( int ) : 0 );
-    k_tile_half<G, PREC, ONE, ENERGY><<<N->tg.ntiles, TPB, lds, S->stream>>>(
+    k_tile_half<G, PREC, ONE, ENERGY><<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
         N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
         S->n_local, S->x.p, S->xg.p, S->gsort.perm.p, S->f.p, N->counts.p,
         N->tslab.p, Pd, c, partial );
@@ -1389,17 +1421,33 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
 }
 
 // Build the list in each layout -- lanes per atom 1, 2, 4, 8 and, for the
-// LDS-tiled layout, tile heights of 4 and 6 cells -- time the force kernel
-// on it and keep the fastest; the list is left built in that layout.
+// LDS-tiled layout, tiles of 2 x 2 columns 2 to 4 cells tall and of one
+// column 4 to 8 cells tall -- time the force kernel on it and keep the
+// fastest; the list is left built in that layout, and later builds keep at
+// least its force-kernel blocks per CU.
 cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
 {
     struct Cand
     {
-        int G, E;
+        int G, W, E, bw;
     };
-    // lanes per atom x tile height
-    const Cand cands[] = { { 1, 0 }, { 2, 4 }, { 2, 6 }, { 2, 8 }, { 4, 4 },
-                           { 4, 6 }, { 4, 8 }, { 8, 4 }, { 8, 6 } };
+    // lanes per atom x tile width x tile height x waves per block (0: one
+    // per slab wave of the largest tile)
+    static const Cand all[] = { { 1, 0, 0, 0 }, { 2, 2, 2, 4 }, { 2, 2, 2, 2 }, { 2, 2, 3, 4 },
+                                { 2, 2, 3, 2 }, { 2, 2, 3, 8 }, { 2, 2, 4, 4 }, { 2, 2, 4, 2 },
+                                { 2, 1, 4, 4 }, { 2, 1, 6, 4 }, { 2, 1, 6, 2 }, { 4, 2, 3, 4 },
+                                { 4, 1, 6, 4 }, { 8, 2, 3, 4 } };
+    // CMD_TUNE=G,W,E,bw limits the search to one layout (for measurements)
+    std::vector<Cand> cv( std::begin( all ), std::end( all ) );
+    if ( const char *e = std::getenv( "CMD_TUNE" ) )
+    {
+        Cand c{};
+        if ( std::sscanf( e, "%d,%d,%d,%d", &c.G, &c.W, &c.E, &c.bw ) == 4 )
+            cv.assign( 1, c );
+    }
+    const Cand *cands = cv.data();
+    const size_t ncand = cv.size();
+    N->occ_blocks = 0;
     // time the fused step the run loop launches (force, both half-kicks and
     // the drift into x_alt); the velocities it kicks are restored after
     StepCtl ctl{};
@@ -1417,15 +1465,17 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     CMD_HIP( hipEventCreate( &e0 ) );
     CMD_HIP( hipEventCreate( &e1 ) );
     float best = 1e30f;
-    Cand bestc = { 1, 0 };
-    float tms[sizeof( cands ) / sizeof( cands[0] )];
+    Cand bestc = { 1, 0, 0, 0 };
+    std::vector<float> tms( ncand, 1e30f );
     cmd_status st = CMD_OK;
-    for ( size_t ci = 0; ci < sizeof( cands ) / sizeof( cands[0] ); ++ci )
+    for ( size_t ci = 0; ci < ncand; ++ci )
     {
         const Cand &cd = cands[ci];
         tms[ci] = 1e30f;
         N->lanes = cd.G;
+        N->tile_w = cd.W;
         N->tile_e = cd.E;
+        N->tile_bw = cd.bw;
         st = neigh_build( N, S );
         if ( st != CMD_OK )
             break;
@@ -1445,8 +1495,9 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
         float ms = 0.0f;
         (void)hipEventElapsedTime( &ms, e0, e1 );
         if ( std::getenv( "CMD_VERBOSE" ) )
-            std::fprintf( stderr, "# autotune lanes=%d tile=%d: %.4f ms/step\n", cd.G, cd.E,
-                          ms / reps );
+            std::fprintf( stderr, "# autotune lanes=%d tile=%dx%dx%d waves/block=%d: %.4f ms/step\n",
+                          cd.G, N->tiled ? N->tg.W : 0, N->tiled ? N->tg.W : 0,
+                          N->tiled ? N->tg.E : 0, N->tiled ? N->tg.bw : 0, ms / reps );
         tms[ci] = ms;
         if ( ms < best )
         {
@@ -1459,7 +1510,7 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     // structure disorders (melted LJ, 2,048,000 atoms: 2 lanes 0.253 ms,
     // 4 lanes 0.270 ms per fused step); so among candidates within 3 % of
     // the fastest the fewest lanes win, the faster of those first.
-    for ( size_t ci = 0; ci < sizeof( cands ) / sizeof( cands[0] ); ++ci )
+    for ( size_t ci = 0; ci < ncand; ++ci )
     {
         const Cand &cd = cands[ci];
         if ( tms[ci] > 1.03f * best )
@@ -1468,7 +1519,7 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
         {
             bestc = cd;
             // the fastest candidate with these lanes
-            for ( size_t cj = 0; cj < sizeof( cands ) / sizeof( cands[0] ); ++cj )
+            for ( size_t cj = 0; cj < ncand; ++cj )
                 if ( cands[cj].G == cd.G && tms[cj] < tms[ci] )
                     bestc = cands[cj];
         }
@@ -1480,8 +1531,12 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     F->lanes = bestc.G;
     F->tuned = true;
     N->lanes = bestc.G;
+    N->tile_w = bestc.W;
     N->tile_e = bestc.E;
-    return neigh_build( N, S );
+    N->tile_bw = bestc.bw;
+    CMD_TRY( neigh_build( N, S ) );
+    N->occ_blocks = N->tiled ? N->tile_blocks : 0;
+    return CMD_OK;
 }
 
 cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N )

@@ -43,7 +43,8 @@ struct cmd_md
     bool inited = false;
     int prec = CMD_PREC_FP64;
     int lanes = 0;
-    int tile = 0; // tile height of the tiled list (0 = default or tuned)
+    int tile = 0;   // tile height of the tiled list (0 = default or tuned)
+    int tile_w = 0; // tile width of the tiled list (0 = default or tuned)
     bool fuse_ghosts = true; // update_halo inside the fused step (tiled lists)
     // set for one rebuild: the list is built by timing every layout
     bool retune = false;
@@ -394,7 +395,9 @@ static cmd_status tune_layout( cmd_md *md, int n, const std::vector<double> &x,
                                  nullptr ) );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
     md->N->lanes = 2; // the tiled layout for the scratch steps
+    md->N->tile_w = 0;
     md->N->tile_e = 0;
+    md->N->tile_bw = 0;
     CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->F->lanes = md->N->lanes;
     CMD_TRY( cmd_force_compute( md->F, S, md->N ) );
@@ -476,6 +479,16 @@ cmd_status cmd_md_set_tile( cmd_md *md, int height )
     CMD_REQUIRE( height >= 0 && height <= 14, CMD_ERR_ARG,
                  "tile height must be 0 (choose) or 1..14 cells" );
     md->tile = height;
+    return CMD_OK;
+}
+
+cmd_status cmd_md_set_tile_width( cmd_md *md, int width )
+{
+    CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
+    CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "the tile width is set before init" );
+    CMD_REQUIRE( width >= 0 && width <= 2, CMD_ERR_ARG,
+                 "tile width must be 0 (choose), 1 or 2 columns" );
+    md->tile_w = width;
     return CMD_OK;
 }
 
@@ -628,8 +641,8 @@ cmd_status cmd_md_init( cmd_md *md )
         CMD_TRY( cmd_neigh_set_lanes( md->N, 2 ) ); // tiled half list
     if ( !nnp && md->tile > 0 )
         CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
-    else if ( !nnp && p.half )
-        CMD_TRY( cmd_neigh_set_tile( md->N, 6 ) ); // measured best for half lists
+    if ( !nnp && md->tile_w > 0 )
+        CMD_TRY( cmd_neigh_set_tile_width( md->N, md->tile_w ) );
     if ( md->lanes == 0 && !p.half && !nnp )
         CMD_TRY( tune_layout( md, n, x, v, type, id ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force
@@ -906,12 +919,15 @@ cmd_status cmd_md_profile( cmd_md *md, int enable )
     return CMD_OK;
 }
 
-cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile, double *seconds )
+cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile_width, int *tile,
+                             double *seconds )
 {
     CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
     CMD_REQUIRE( md->inited, CMD_ERR_STATE, "tune info before init" );
     if ( lanes )
         *lanes = md->N->lanes;
+    if ( tile_width )
+        *tile_width = md->N->tiled ? md->N->tg.W : 0;
     if ( tile )
         *tile = md->N->tiled ? md->N->tg.E : 0;
     if ( seconds )

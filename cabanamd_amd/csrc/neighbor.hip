@@ -514,6 +514,30 @@ cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
     return CMD_OK;
 }
 
+cmd_status cmd_neigh_set_tile_width( cmd_neigh *n, int width )
+{
+    CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    CMD_REQUIRE( width >= 0 && width <= 2, CMD_ERR_ARG,
+                 "tile width must be 0 (choose), 1 or 2 columns" );
+    if ( width != n->tile_w )
+    {
+        n->tile_w = width;
+        n->n_rows = 0; // layout changed: must rebuild
+        n->csr_valid = false;
+    }
+    return CMD_OK;
+}
+
+cmd_status cmd_neigh_tile_shape( const cmd_neigh *n, int *width, int *height )
+{
+    CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
+    if ( width )
+        *width = n->tiled ? n->tg.W : 0;
+    if ( height )
+        *height = n->tiled ? n->tg.E : 0;
+    return CMD_OK;
+}
+
 cmd_status cmd_neigh_layout( const cmd_neigh *n, int *tiled, int *tile_height,
                              int *lanes )
 {

@@ -4,24 +4,29 @@
 // forces as ForceLJ::compute_force_full (force_lj_cabana_neigh_impl.h:
 // 151-203); only the work decomposition and the list encoding differ.
 //
-// A tile is a run of E cells [e0, e1) of one interior column (a, b) of the
-// binning grid (cells >= the neighbour radius).  Its atoms are the contiguous
-// local range of those cells, and every neighbour candidate lies in the 27-cell
-// stencils of those cells: 9 columns (a+-1, b+-1) x cells [e0-1, e1].  Per
-// column that is one contiguous range of local atoms (locals are cell sorted
-// by create_binning) and one contiguous range of ghosts (ghosts are cell
-// sorted here).  A block stages those 18 ranges of positions in LDS and the
-// list stores, for each neighbour, the 16-bit byte offset of its LDS record
-// (slot * TILE_REC).  Rows are walked in
-// the order the oracle uses (stencil columns, then cells, then atoms by
-// index), so lists and FP64 sums are unchanged.
+// A tile is W x W interior columns (W = 1 or 2) of the binning grid (cells
+// >= the neighbour radius) over E cells [e0, e1).  Its atoms are the local
+// atoms of those cells, one contiguous run per atom column (locals are cell
+// sorted by create_binning), and every neighbour candidate lies in the
+// (W + 2) x (W + 2) staged columns around them, cells [e0 - 1, e1]: per
+// staged column one contiguous range of local atoms and one of ghosts
+// (ghosts are cell sorted here).  The force block stages those ranges of
+// positions in LDS and the list stores, for each neighbour, the 16-bit byte
+// offset of its LDS record (slot * TILE_REC).  Rows are walked in the order
+// the oracle uses (stencil columns, then cells, then atoms by index), so
+// lists and FP64 sums are unchanged.  Wider tiles stage fewer records per
+// atom (2 x 2 x 3 cells: 6.9 per atom against 12 for 1 x 1 x 6) and give a
+// block more atoms for the same LDS.
 //
-// Slab of a tile: waves of APW = 64/G atoms; with gw lanes per atom in wave
-// w (G, or up to 8 in a part-full wave: wave_lanes), neighbour k of atom a
-// of the wave goes to row r = k / gw, lane (k % gw) * (64 / gw) + a, and
-// four consecutive rows of a lane are packed in 8 bytes:
+// Slab of a tile: the tile's atoms in atom-column order, in waves of
+// APW = 64/G atoms; with gw lanes per atom in wave w (G, or up to 8 in a
+// part-full wave: wave_lanes), neighbour k of atom a of the wave goes to row
+// r = k / gw, lane (k % gw) * (64 / gw) + a, and four consecutive rows of a
+// lane are packed in 8 bytes:
 //   tslab[tile_base + ((w * R4 + r / 4) * 64 + lane) * 4 + r % 4],
 // R4 = ceil(ceil(cap / G) / 4), so one 8-byte load per lane feeds four pairs.
+// Every 8-byte word belongs to one atom, so the rows of different atom
+// columns of a wave are written by different blocks without overlap.
 #include "internal.h"
 #include "tiles_dev.h"
 
@@ -35,45 +40,75 @@ namespace cmd
 static constexpr int TPB = 256;
 static constexpr int BTPB = 512;       // list build block
 static constexpr int EMAX = 16;
-static constexpr int MAX_SLOTS = 1792; // 42 KB of 24-byte records per block
+static constexpr int MAX_SLOTS = 2720; // largest staged neighbourhood (65 KB of records)
 static_assert( ( MAX_SLOTS + 1 ) * TILE_REC + 16 < 65536,
                "record offsets must fit the 16-bit list entries" );
-// Per-tile runs, atom range and staged count; waves per tile for the slab.
-// slots[0] gets the largest staged count, slots[1] the number of local atoms
-// the tiles cover (all of them only when every local atom sits in an
-// interior cell of the grid), slots[2] the most slab waves of one tile.
+
+// Per-tile runs, atom columns and staged count; slab waves per tile.
+// slots[0] gets the largest staged count of a tile, slots[1] the number of
+// local atoms the tiles cover (all of them only when every local atom sits
+// in an interior cell of the grid), slots[2] the most slab waves of one
+// tile, slots[3] the largest staged count of one atom column's 3 x 3
+// columns (the list build's neighbourhood), slots[4] the most atoms of one
+// atom column (the list build's rows).
 __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
                              const int *__restrict__ gs, int apw,
                              int *__restrict__ info, int *__restrict__ waves,
                              int *__restrict__ slots )
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = 0, nat = 0, nwv = 0;
+    int total = 0, nat = 0, nwv = 0, bmax = 0, cmax = 0;
     if ( t < T.ntiles )
     {
-        int a, b, e0, e1;
-        tile_coords( T, t, a, b, e0, e1 );
+        int a0, b0, e0, e1;
+        tile_coords( T, t, a0, b0, e0, e1 );
         const int ea = max( e0 - 1, 0 ), eb = min( e1 + 1, T.g.n[2] );
+        const int S = T.W + 2;
         int *I = info + (size_t)t * TI_WORDS;
-        int q = 0;
-        for ( int da = -1; da <= 1; ++da )
-            for ( int db = -1; db <= 1; ++db )
+        int len[TILE_QMAX];
+        for ( int q = 0; q < TILE_QMAX; ++q )
+        {
+            const int a = a0 - 1 + q / S, b = b0 - 1 + q % S;
+            int lb = 0, ll = 0, gb = 0, gl = 0;
+            if ( q < S * S && a < T.g.n[0] && b < T.g.n[1] )
             {
-                const int c = col_base( T, a + da, b + db );
-                I[TI_LB + q] = ls[c + ea];
-                I[TI_LLEN + q] = ls[c + eb] - ls[c + ea];
-                I[TI_GB + q] = gs[c + ea];
-                I[TI_GLEN + q] = gs[c + eb] - gs[c + ea];
-                total += I[TI_LLEN + q] + I[TI_GLEN + q];
-                ++q;
+                const int c = col_base( T, a, b );
+                lb = ls[c + ea];
+                ll = ls[c + eb] - lb;
+                gb = gs[c + ea];
+                gl = gs[c + eb] - gb;
             }
-        const int cc = col_base( T, a, b );
-        I[TI_TL] = ls[cc + e0];
-        I[TI_NAT] = ls[cc + e1] - ls[cc + e0];
+            I[TI_LB + q] = lb;
+            I[TI_LLEN + q] = ll;
+            I[TI_GB + q] = gb;
+            I[TI_GLEN + q] = gl;
+            len[q] = ll + gl;
+            total += ll + gl;
+        }
+        for ( int c = 0; c < TILE_CMAX; ++c )
+        {
+            const int ca = c / T.W, cb = c % T.W;
+            int al = 0, an = 0;
+            if ( c < T.W * T.W && a0 + ca <= T.g.n[0] - 2 && b0 + cb <= T.g.n[1] - 2 )
+            {
+                const int cc = col_base( T, a0 + ca, b0 + cb );
+                al = ls[cc + e0];
+                an = ls[cc + e1] - al;
+                int s9 = 0;
+                for ( int da = 0; da < 3; ++da )
+                    for ( int db = 0; db < 3; ++db )
+                        s9 += len[( ca + da ) * S + cb + db];
+                bmax = max( bmax, s9 );
+            }
+            I[TI_AL + c] = al;
+            I[TI_AN + c] = an;
+            nat += an;
+            cmax = max( cmax, an );
+        }
+        I[TI_NAT] = nat;
         I[TI_TOTAL] = total;
         I[TI_EA] = ea;
         I[TI_NST] = eb - ea;
-        nat = I[TI_NAT];
         nwv = ( nat + apw - 1 ) / apw;
         waves[t] = nwv;
     }
@@ -84,6 +119,8 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
         m = max( m, __shfl_xor( m, o, 64 ) );
         nat += __shfl_xor( nat, o, 64 );
         nwv = max( nwv, __shfl_xor( nwv, o, 64 ) );
+        bmax = max( bmax, __shfl_xor( bmax, o, 64 ) );
+        cmax = max( cmax, __shfl_xor( cmax, o, 64 ) );
     }
     if ( ( threadIdx.x & 63 ) == 0 )
     {
@@ -93,38 +130,30 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
             atomicAdd( &slots[1], nat );
         if ( nwv > 0 )
             atomicMax( &slots[2], nwv );
+        if ( bmax > 0 )
+            atomicMax( &slots[3], bmax );
+        if ( cmax > 0 )
+            atomicMax( &slots[4], cmax );
     }
 }
 
-template <bool HALF>
-__device__ __forceinline__ bool tile_accept( const double4 &pi, const double4 &pj,
-                                             double rc2 )
-{
-    const double dx = pi.x - pj.x;
-    const double dy = pi.y - pj.y;
-    const double dz = pi.z - pj.z;
-    const double rsq = dx * dx + dy * dy + dz * dz;
-    bool ok = rsq <= rc2;
-    if ( HALF )
-        ok = ok && ( ( pj.x > pi.x ) |
-                     ( ( pj.x == pi.x ) & ( ( pj.y > pi.y ) |
-                                            ( ( pj.y == pi.y ) & ( pj.z > pi.z ) ) ) ) );
-    return ok;
-}
-
-// List build, one wave per cell of the tile.  All atoms of a cell share the
-// same 27-cell stencil, so the wave loads the stencil's candidates into
-// registers, BCH chunks of 64 at a time (lane = candidate, positions from
-// LDS planes), and tests every atom of the cell against them in turn, the
-// atom's position and row length being wave-uniform (scalar loads and
-// registers); ballot + mbcnt place each accepted candidate in the atom's
-// row, so rows come out in walk order.  The
-// stencil is a list of contiguous LDS ranges (per column: the locals of its
-// three cells, or locals and ghosts cell by cell where ghosts are present),
-// concatenated so that chunks run across range ends.  Rows are assembled in
-// LDS one after another (pre-filled with the sentinel slot), so an entry's
-// address is the row start plus its index, and are transposed into the slab
-// layout on the way out.
+// List build, one block per atom column of a tile and one wave per cell of
+// that column.  All atoms of a cell share the same 27-cell stencil, so the
+// wave loads the stencil's candidates into registers, BCH chunks of 64 at a
+// time (lane = candidate, positions from LDS planes), and tests every atom
+// of the cell against them in turn, the atom's position and row length being
+// wave-uniform (scalar loads and registers); ballot + mbcnt place each
+// accepted candidate in the atom's row, so rows come out in walk order.  The
+// block stages the 3 x 3 columns around its atom column (a subset of the
+// tile's staged columns, the same cell range) as coordinate planes; each
+// candidate carries the byte offset of its record in the force tile's
+// staging, which is what the row stores.  The stencil is a list of
+// contiguous LDS ranges (per column: the locals of its three cells, or
+// locals and ghosts cell by cell where ghosts are present), concatenated so
+// that chunks run across range ends.  Rows are assembled in LDS one after
+// another (pre-filled with the sentinel), so an entry's address is the row
+// start plus its index, and are transposed into the slab layout on the way
+// out.
 static constexpr int MAXR = 9 * 3 * 2; // stencil ranges of one cell
 static constexpr int BCH = 10;         // candidate chunks of 64 held in registers
 
@@ -142,23 +171,16 @@ __host__ __device__ constexpr int tile_row_stride( int R4, int G )
     return R4 * 4 * G + 2;
 }
 
-// One atom against NC register chunks of candidates: every chunk's test
-// first (independent chains, one mask each), then the row entries in chunk
-// order.  A lane that does not accept writes its own slot of a trash area
-// instead, so the stores need no branch.  Returns the atom's new row length.
-template <int NC, int G, bool HALF>
-__device__ __forceinline__ unsigned build_row( const double ( &px )[BCH], const double ( &py )[BCH],
-                                               const double ( &pz )[BCH], const int ( &po )[BCH],
-                                               const double4 pa, int vs, unsigned short *row,
-                                               unsigned k0, double rc2, int cap,
-                                               unsigned short *trash )
+// The test masks of one atom against NC register chunks of candidates.
+template <int NC, bool HALF>
+__device__ __forceinline__ void test_chunks( const double ( &px )[BCH], const double ( &py )[BCH],
+                                             const double ( &pz )[BCH], const double4 pa, int vs,
+                                             double rc2, unsigned long long ( &m )[NC] )
 {
-    const int lane = threadIdx.x & 63;
     // the atom itself: candidate vs of this pass (chunk vs >> 6, lane
     // vs & 63; outside the pass when the chunk is not in 0 .. NC - 1)
     const int cself = vs >> 6;
     const unsigned long long notself = ~( 1ull << ( vs & 63 ) );
-    unsigned long long m[NC];
 #pragma unroll
     for ( int c = 0; c < NC; ++c )
     {
@@ -177,6 +199,18 @@ __device__ __forceinline__ unsigned build_row( const double ( &px )[BCH], const 
             m[c] &= __builtin_amdgcn_ballot_w64( up );
         }
     }
+}
+
+// The accepted candidates of one atom's masks appended to its row in chunk
+// order; a lane that does not accept writes its own slot of a trash area
+// instead, so the stores need no branch.  Returns the new row length.
+template <int NC>
+__device__ __forceinline__ unsigned place_row( const int ( &po )[BCH],
+                                               const unsigned long long ( &m )[NC],
+                                               unsigned short *row, unsigned k0, int cap,
+                                               unsigned short *trash )
+{
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for ( int c = 0; c < NC; ++c )
     {
@@ -191,10 +225,10 @@ __device__ __forceinline__ unsigned build_row( const double ( &px )[BCH], const 
 }
 
 // The atoms a0 .. a0 + na - 1 of a cell (candidates vs0 + l of this pass,
-// LDS rows row0 + l * RS) against NC register chunks of candidates, one
-// atom at a time: its
-// position and row length are wave-uniform (scalar loads and registers).
-// Lane l of cnt holds atom a0 + l's row length.
+// LDS rows row0 + l * RS) against NC register chunks of candidates, two
+// atoms at a time: both atoms' tests first (independent chains), then their
+// rows.  An atom's position and row length are wave-uniform (scalar loads
+// and registers).  Lane l of cnt holds atom a0 + l's row length.
 template <int NC, int G, bool HALF>
 __device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const double ( &py )[BCH],
                                              const double ( &pz )[BCH], const int ( &po )[BCH],
@@ -203,21 +237,36 @@ __device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const do
                                              int cap, unsigned short *trash, int &cnt )
 {
     const int lane = threadIdx.x & 63;
-    double4 pn = x[a0];
-    for ( int l = 0; l < na; ++l )
+    int l = 0;
+    for ( ; l + 1 < na; l += 2 )
     {
-        const double4 pa = pn;
-        pn = x[a0 + min( l + 1, na - 1 )];
-        unsigned k0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
-        k0 = build_row<NC, G, HALF>( px, py, pz, po, pa, vs0 + l, row0 + l * RS, k0,
-                                     rc2, cap, trash );
+        const double4 pa = x[a0 + l], pb = x[a0 + l + 1];
+        const unsigned ka0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
+        const unsigned kb0 = (unsigned)__builtin_amdgcn_readlane( cnt, l + 1 );
+        unsigned long long ma[NC], mb[NC];
+        test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
+        test_chunks<NC, HALF>( px, py, pz, pb, vs0 + l + 1, rc2, mb );
+        const unsigned ka = place_row<NC>( po, ma, row0 + l * RS, ka0, cap, trash );
+        const unsigned kb = place_row<NC>( po, mb, row0 + ( l + 1 ) * RS, kb0, cap, trash );
+        cnt = lane == l ? (int)ka : ( lane == l + 1 ? (int)kb : cnt );
+    }
+    if ( l < na )
+    {
+        const double4 pa = x[a0 + l];
+        unsigned long long ma[NC];
+        test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
+        const unsigned ka = place_row<NC>(
+            po, ma, row0 + l * RS, (unsigned)__builtin_amdgcn_readlane( cnt, l ), cap, trash );
         if ( lane == l )
-            cnt = (int)k0;
+            cnt = (int)ka;
     }
 }
 
+// LDS keeps the build at about three waves per SIMD, so the compiler may
+// spend registers on overlapping the candidate chunks' independent tests
+// instead of on occupancy it cannot reach.
 template <int G, bool HALF>
-__global__ __launch_bounds__( BTPB ) void k_tile_build(
+__global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 ) ) ) void k_tile_build(
     TileGrid T, const int *__restrict__ ls, const int *__restrict__ gs,
     const int *__restrict__ info, const int *__restrict__ woff,
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2,
@@ -226,53 +275,113 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
 {
     extern __shared__ double s_x[];
     __shared__ TileHead H;
+    __shared__ int s_bseg[19];                // the block's 9-column segment offsets
+    __shared__ int s_dl[9], s_dg[9];          // force slot - build slot, per column
     __shared__ int s_lcs[9][EMAX + 4], s_gcs[9][EMAX + 4];
     __shared__ int s_rs[BTPB / 64][MAXR];     // range starts (LDS slots)
+    __shared__ int s_rf[BTPB / 64][MAXR];     // range starts in the force tile's slots
     __shared__ int s_rp[BTPB / 64][MAXR + 1]; // prefix of range lengths
     constexpr int APW = 64 / G;
-    const int t = xcd_tile( blockIdx.x, gridDim.x );
-    load_head( info, t, H );
-    int a, b, e0, e1;
-    tile_coords( T, t, a, b, e0, e1 );
+    const int W = T.W, S = W + 2, NA = W * W;
+    const int sub = xcd_tile( blockIdx.x, gridDim.x );
+    const int t = sub / NA, c = sub % NA, ca = c / W, cb = c % W;
+    load_head( info, t, W, H );
+    const int nat_c = H.info[TI_AN + c];
+    if ( nat_c == 0 )
+        return; // block-uniform: an atom column outside the grid, or empty
+    int a0, b0, e0, e1;
+    tile_coords( T, t, a0, b0, e0, e1 );
+    const int a = a0 + ca, b = b0 + cb;
     const int ea = H.info[TI_EA], nst = H.info[TI_NST];
+    // the block's 3 x 3 columns: build column qb = da * 3 + db is staged
+    // column (ca + da) * S + cb + db of the tile
+    if ( threadIdx.x == 0 )
+    {
+        int off = 0;
+        for ( int qb = 0; qb < 9; ++qb )
+        {
+            const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
+            s_bseg[2 * qb] = off;
+            s_dl[qb] = H.segoff[2 * qf] - off;
+            off += H.info[TI_LLEN + qf];
+            s_bseg[2 * qb + 1] = off;
+            s_dg[qb] = H.segoff[2 * qf + 1] - off;
+            off += H.info[TI_GLEN + qf];
+        }
+        s_bseg[18] = off;
+    }
+    __syncthreads();
     for ( int k = threadIdx.x; k < 9 * ( nst + 1 ); k += blockDim.x )
     {
-        const int q = k / ( nst + 1 ), ce = k % ( nst + 1 );
-        const int c = col_base( T, a + q / 3 - 1, b + q % 3 - 1 ) + ea + ce;
-        s_lcs[q][ce] = H.segoff[2 * q] + ( ls[c] - H.info[TI_LB + q] );
-        s_gcs[q][ce] = H.segoff[2 * q + 1] + ( gs[c] - H.info[TI_GB + q] );
+        const int qb = k / ( nst + 1 ), ce = k % ( nst + 1 );
+        const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
+        const int cl = col_base( T, a + qb / 3 - 1, b + qb % 3 - 1 ) + ea + ce;
+        s_lcs[qb][ce] = s_bseg[2 * qb] + ( ls[cl] - H.info[TI_LB + qf] );
+        s_gcs[qb][ce] = s_bseg[2 * qb + 1] + ( gs[cl] - H.info[TI_GB + qf] );
     }
-    stage_planes<false>( H, x, xg, s_x, slot_cap );
-    const double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
+    // coordinate planes of the 3 x 3 columns' runs, then the sentinel
+    double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
+    const int btotal = s_bseg[18];
+    {
+        constexpr int B = 4;
+        int seg = 0;
+        for ( int k0 = threadIdx.x; k0 < btotal; k0 += B * blockDim.x )
+        {
+            double4 p[B];
+#pragma unroll
+            for ( int u = 0; u < B; ++u )
+            {
+                const int k = min( k0 + u * (int)blockDim.x, btotal - 1 );
+                while ( k >= s_bseg[seg + 1] )
+                    ++seg;
+                const int qb = seg >> 1, off = k - s_bseg[seg];
+                const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
+                p[u] = ( seg & 1 ) ? xg[H.info[TI_GB + qf] + off] : x[H.info[TI_LB + qf] + off];
+            }
+#pragma unroll
+            for ( int u = 0; u < B; ++u )
+            {
+                const int k = min( k0 + u * (int)blockDim.x, btotal );
+                s_x[k] = p[u].x;
+                s_y[k] = p[u].y;
+                s_z[k] = p[u].z;
+            }
+        }
+        __syncthreads();
+        if ( threadIdx.x == 0 )
+        {
+            s_x[btotal] = TILE_FAR;
+            s_y[btotal] = TILE_FAR;
+            s_z[btotal] = TILE_FAR;
+        }
+    }
     // after the planes (rounded to 16 bytes): a 64-entry trash area, then
-    // the rows
+    // the rows of this atom column's atoms
     unsigned short *trash = reinterpret_cast<unsigned short *>(
         reinterpret_cast<char *>( s_x ) + tile_planes_bytes( slot_cap ) );
     unsigned short *chunk = trash + 64;
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     const int wsz = R4 * 256; // slab entries of one slab wave
-    // LDS rows: one per atom of the tile's slab waves, entries in order;
     // the stride is an odd number of words, which spreads the copy-out's
     // strided reads over the banks
     const int RS = tile_row_stride( R4, G );
-    const int nw = woff[t + 1] - woff[t];
-    const int tl = H.info[TI_TL];
-    const int si0 = H.segoff[8] - H.info[TI_LB + 4]; // LDS slot of local i = si0 + i
+    const int al = H.info[TI_AL + c];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
-    const int sent_slot = H.segoff[18];
-    // rows start as the sentinel record (the tile's slot TOTAL)
+    const int fsent = H.segoff[H.nseg]; // the force tile's sentinel slot
+    // rows start as the force tile's sentinel record
     {
-        const unsigned sent = (unsigned)( sent_slot * TILE_REC );
+        const unsigned sent = (unsigned)( fsent * TILE_REC );
         const unsigned sent2 = sent | ( sent << 16 );
         const uint4 fill = make_uint4( sent2, sent2, sent2, sent2 );
-        const int n16 = nw * APW * RS / 8;
+        const int n16 = ( nat_c * RS + 7 ) / 8;
         for ( int k = threadIdx.x; k < n16; k += blockDim.x )
             reinterpret_cast<uint4 *>( chunk )[k] = fill;
     }
     __syncthreads();
     int mrow = 0;
-    int *rs = s_rs[wv], *rp = s_rp[wv];
+    int *rs = s_rs[wv], *rf = s_rf[wv], *rp = s_rp[wv];
+    const int bsent = btotal;
     for ( int e = e0 + wv; e < e1; e += nwv )
     {
         const int ce = e - ea;
@@ -302,6 +411,7 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
             if ( !gh )
             {
                 rs[off] = s_lcs[lane][elo];
+                rf[off] = s_lcs[lane][elo] + s_dl[lane];
                 rp[off + 1] = s_lcs[lane][ehi + 1] - s_lcs[lane][elo];
             }
             else
@@ -309,8 +419,10 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
                 for ( int c2 = elo; c2 <= ehi; ++c2 )
                 {
                     rs[off] = s_lcs[lane][c2];
+                    rf[off] = s_lcs[lane][c2] + s_dl[lane];
                     rp[off + 1] = s_lcs[lane][c2 + 1] - s_lcs[lane][c2];
                     rs[off + 1] = s_gcs[lane][c2];
+                    rf[off + 1] = s_gcs[lane][c2] + s_dg[lane];
                     rp[off + 2] = s_gcs[lane][c2 + 1] - s_gcs[lane][c2];
                     off += 2;
                 }
@@ -344,22 +456,22 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
         const int cc = col_base( T, a, b ) + e;
         const int c0 = __builtin_amdgcn_readfirstlane( ls[cc] );
         const int c1 = __builtin_amdgcn_readfirstlane( ls[cc + 1] );
-        for ( int a0 = c0; a0 < c1; a0 += 64 )
+        for ( int i0 = c0; i0 < c1; i0 += 64 )
         {
-            const int na = min( 64, c1 - a0 );
-            int cnt = 0; // row length of atom a0 + lane
+            const int na = min( 64, c1 - i0 );
+            int cnt = 0; // row length of atom i0 + lane
             for ( int v0 = 0; v0 < total; v0 += BCH * 64 )
             {
                 // BCH chunks of candidates in registers (lane = candidate);
                 // past the end: the sentinel position, which no test accepts
                 double px[BCH], py[BCH], pz[BCH];
-                int po[BCH]; // record byte offset, the value a row stores
+                int po[BCH]; // the force tile's record byte offset, the value a row stores
 #pragma unroll
-                for ( int c = 0; c < BCH; ++c )
+                for ( int ch = 0; ch < BCH; ++ch )
                 {
-                    const int v = v0 + c * 64 + lane;
-                    int sj = sent_slot;
-                    if ( v0 + c * 64 < total )
+                    const int v = v0 + ch * 64 + lane;
+                    int sj = bsent, sf = fsent;
+                    if ( v0 + ch * 64 < total )
                     {
                         // range holding candidate v: binary search over the
                         // prefix of range lengths
@@ -373,35 +485,38 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
                                 hi = mid;
                         }
                         if ( v < total )
+                        {
                             sj = rs[lo] + ( v - rp[lo] );
+                            sf = rf[lo] + ( v - rp[lo] );
+                        }
                     }
-                    px[c] = s_x[sj];
-                    py[c] = s_y[sj];
-                    pz[c] = s_z[sj];
-                    po[c] = sj * TILE_REC;
+                    px[ch] = s_x[sj];
+                    py[ch] = s_y[sj];
+                    pz[ch] = s_z[sj];
+                    po[ch] = sf * TILE_REC;
                 }
                 const int nch = min( BCH, ( total - v0 + 63 ) >> 6 );
-                const int vs0 = vbase + ( a0 - c0 ) - v0;
-                unsigned short *row0 = chunk + ( a0 - tl ) * RS;
+                const int vs0 = vbase + ( i0 - c0 ) - v0;
+                unsigned short *row0 = chunk + ( i0 - al ) * RS;
                 if ( nch <= 4 )
-                    build_atoms<4, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                    build_atoms<4, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
                                              cap, trash, cnt );
                 else if ( nch <= 6 )
-                    build_atoms<6, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                    build_atoms<6, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
                                              cap, trash, cnt );
                 else if ( nch <= 8 )
-                    build_atoms<8, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                    build_atoms<8, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
                                              cap, trash, cnt );
                 else if ( nch == 9 )
-                    build_atoms<9, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS, rc2,
+                    build_atoms<9, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
                                              cap, trash, cnt );
                 else
-                    build_atoms<BCH, G, HALF>( px, py, pz, po, x, a0, na, vs0, row0, RS,
+                    build_atoms<BCH, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS,
                                                rc2, cap, trash, cnt );
             }
             if ( lane < na )
             {
-                counts[a0 + lane] = cnt;
+                counts[i0 + lane] = cnt;
                 mrow = max( mrow, cnt );
             }
         }
@@ -413,29 +528,31 @@ __global__ __launch_bounds__( BTPB ) void k_tile_build(
         // slab layout: per slab wave (gw = wave_lanes lanes per atom, 64 / gw
         // atoms), R4 groups of 64 eight-byte words; word (group g4, lane L)
         // holds entries (4 g4 + j) gw + L / (64 / gw), j = 0..3, of the
-        // wave's atom L % (64 / gw).  Words of lanes past the tile's atoms
-        // are never read and not written; entries at or past cap are the
-        // sentinel.
-        const int nwords = nw * R4 * 64, nat = H.info[TI_NAT];
-        const unsigned short sent = (unsigned short)( sent_slot * TILE_REC );
-        uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)woff[t] * wsz );
-        for ( int w = threadIdx.x; w < nwords; w += blockDim.x )
+        // wave's atom L % (64 / gw).  Only the words of this atom column's
+        // atoms (tile atoms ta0 .. ta0 + nat_c - 1) are written; entries at
+        // or past cap are the sentinel.
+        const int nat = H.info[TI_NAT], ta0 = H.aoff[c];
+        const int w0 = ta0 / APW, w1 = ( ta0 + nat_c - 1 ) / APW;
+        const int nwords = ( w1 - w0 + 1 ) * R4 * 64;
+        const unsigned short sent = (unsigned short)( fsent * TILE_REC );
+        uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)( woff[t] + w0 ) * wsz );
+        for ( int wd = threadIdx.x; wd < nwords; wd += blockDim.x )
         {
-            const int sw = w / ( R4 * 64 ), rem = w - sw * ( R4 * 64 );
+            const int sw = w0 + wd / ( R4 * 64 ), rem = wd % ( R4 * 64 );
             const int g4 = rem >> 6, L = rem & 63;
             const int gw = wave_lanes( G, nat, sw ), apw = 64 / gw;
             const int ta = sw * APW + L % apw;
-            if ( L % apw >= min( APW, nat - sw * APW ) )
+            if ( L % apw >= APW || ta < ta0 || ta >= ta0 + nat_c )
                 continue;
-            const unsigned short *row = chunk + ta * RS;
+            const unsigned short *row = chunk + ( ta - ta0 ) * RS;
             unsigned e4[4];
 #pragma unroll
             for ( int j = 0; j < 4; ++j )
             {
-                const int e = ( 4 * g4 + j ) * gw + L / apw;
-                e4[j] = e < cap ? row[e] : sent;
+                const int en = ( 4 * g4 + j ) * gw + L / apw;
+                e4[j] = en < cap ? row[en] : sent;
             }
-            dst[w] = make_uint2( e4[0] | ( e4[1] << 16 ), e4[2] | ( e4[3] << 16 ) );
+            dst[wd] = make_uint2( e4[0] | ( e4[1] << 16 ), e4[2] | ( e4[3] << 16 ) );
         }
     }
 #pragma unroll
@@ -457,13 +574,13 @@ __global__ __launch_bounds__( TPB ) void k_tile_to_csr(
     __shared__ TileHead H;
     constexpr int APW = 64 / G;
     const int t = blockIdx.x;
-    load_head( info, t, H );
+    load_head( info, t, T.W, H );
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
-    const int nat = H.info[TI_NAT], tl = H.info[TI_TL];
+    const int nat = H.info[TI_NAT];
     for ( int ta = threadIdx.x; ta < nat; ta += blockDim.x )
     {
-        const int i = tl + ta;
+        const int i = tile_atom( H, T.W, ta );
         const int w = ta / APW, aw = ta % APW;
         const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
         const int n = counts[i];
@@ -602,23 +719,31 @@ cmd_status tiles_ghost_refresh( cmd_system *S )
     return CMD_OK;
 }
 
-static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[4] )
+// Tile info for width W and height E.  h[0] largest staged count of a
+// tile, h[1] local atoms the tiles cover, h[2] most slab waves of a tile,
+// h[3] largest staged count of one atom column's 3 x 3 columns, h[4] most
+// atoms of one atom column, h[5] slab waves of all tiles (the scan's total);
+// read back in one wait.
+static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int W, int E, int h[6] )
 {
     TileGrid &T = N->tg;
     T.g = S->bin_grid;
     T.nc0 = T.g.n[0] - 2;
     T.nc1 = T.g.n[1] - 2;
+    T.W = W;
+    T.nt0 = ( T.nc0 + W - 1 ) / W;
+    T.nt1 = ( T.nc1 + W - 1 ) / W;
     T.E = E;
     T.tpc = ( T.g.n[2] - 2 + E - 1 ) / E;
-    T.ntiles = T.nc0 * T.nc1 * T.tpc;
+    T.ntiles = T.nt0 * T.nt1 * T.tpc;
     CMD_REQUIRE( T.nc0 > 0 && T.nc1 > 0 && T.tpc > 0, CMD_ERR_ARG,
                  "binning grid has no interior cells" );
     CMD_TRY( N->tile_info.reserve( (size_t)T.ntiles * TI_WORDS ) );
-    CMD_TRY( S->sc.ires.reserve( 4 ) );
+    CMD_TRY( S->sc.ires.reserve( 8 ) );
     DevBuf<int> &waves = N->grid_sort.cursor; // scratch: waves per tile
     CMD_TRY( waves.reserve( T.ntiles + 1 ) );
     CMD_TRY( N->grid_sort.start.reserve( T.ntiles + 1 ) ); // scan of waves
-    CMD_TRY( fill_i32( S->sc.ires.p, 0, 3, S->stream ) );
+    CMD_TRY( fill_i32( S->sc.ires.p, 0, 5, S->stream ) );
     const int apw = 64 / N->lanes;
     k_tile_info<<<( T.ntiles + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
         T, S->bins.start.p, S->gsort.start.p, apw, N->tile_info.p, waves.p,
@@ -626,13 +751,14 @@ static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int E, int h[4] )
     CMD_LAUNCH_CHECK();
     CMD_TRY( scan_exclusive( waves.p, N->grid_sort.start.p, T.ntiles, S->sc,
                              S->stream, nullptr ) );
-    // h[0..2] from the info kernel, h[3] = slab waves of all tiles (the
-    // scan's total), in one wait
-    CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, 3 * sizeof( int ),
+    CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, 5 * sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
-    CMD_HIP( hipMemcpyAsync( h + 3, N->grid_sort.start.p + T.ntiles, sizeof( int ),
+    CMD_HIP( hipMemcpyAsync( h + 5, N->grid_sort.start.p + T.ntiles, sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
+    // waves per block of the force kernels: as tuned, else one per slab
+    // wave of the largest tile, at most 8 (a tile with more loops)
+    T.bw = N->tile_bw > 0 ? N->tile_bw : std::min( std::max( h[2], 1 ), 8 );
     return CMD_OK;
 }
 
@@ -643,20 +769,30 @@ static size_t tile_slab_entries( const cmd_neigh *N, int nw )
     return std::max<size_t>( (size_t)nw * R4 * 256, 1 );
 }
 
-// LDS limits of one list-build block: the hard per-block limit, and the
-// size that leaves room for four blocks (32 waves) per CU, which the row
-// chunk is sized to when the staged positions allow it.
-static void build_lds_limits( size_t *hard, size_t *target )
+// LDS of one block (hard limit, less the static tables) and of one CU.
+static void lds_limits( size_t *block, size_t *cu )
 {
-    int dev = 0, lim = 0;
+    int dev = 0, lim = 0, lcu = 0;
     if ( hipGetDevice( &dev ) != hipSuccess ||
          hipDeviceGetAttribute( &lim, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                 dev ) != hipSuccess ||
          lim <= 0 )
         lim = 65536;
-    const size_t statics = 4096; // tile header and cell tables
-    *hard = (size_t)lim > statics ? (size_t)lim - statics : 0;
-    *target = (size_t)lim >= 2 * 65536 ? (size_t)lim / 4 - statics : *hard / 2;
+    if ( hipDeviceGetAttribute( &lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                dev ) != hipSuccess ||
+         lcu < lim )
+        lcu = lim;
+    const size_t statics = 8192; // tile header, cell and range tables
+    *block = (size_t)lim > statics ? (size_t)lim - statics : 0;
+    *cu = (size_t)lcu;
+}
+
+// LDS of the force kernels on tiles with `slots` staged atoms: a position
+// record and, with several types, a type per slot (lj.hip); the half list
+// adds a reaction record and an energy weight per slot.
+static size_t force_lds( int slots, bool half )
+{
+    return (size_t)( slots + 1 ) * ( TILE_REC + sizeof( int ) ) * ( half ? 2 : 1 ) + 16;
 }
 
 template <int G>
@@ -664,15 +800,16 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, size_t lds,
                           int *max_row )
 {
     const double rc2 = N->neigh_cut * N->neigh_cut;
-    // one wave per cell of the tile
+    // one block per atom column of a tile, one wave per cell of the column
     const int threads = 64 * std::min( std::max( N->tg.E, 1 ), BTPB / 64 );
+    const int nb = N->tg.ntiles * N->tg.W * N->tg.W;
     if ( N->half )
-        k_tile_build<G, true><<<N->tg.ntiles, threads, lds, S->stream>>>(
+        k_tile_build<G, true><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
             N->tslab.p, N->counts.p, max_row );
     else
-        k_tile_build<G, false><<<N->tg.ntiles, threads, lds, S->stream>>>(
+        k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
             N->tslab.p, N->counts.p, max_row );
@@ -686,47 +823,52 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
     const int G = N->lanes;
     if ( N->cap < 1 )
         N->cap = std::max( 1, N->max_neigh_guess );
-    size_t hard = 0, target = 0;
-    build_lds_limits( &hard, &target );
-    (void)target;
-    // tile height: the requested one, or the tallest whose neighbourhood and
-    // rows fit in LDS
-    int E = N->tile_e > 0 ? N->tile_e : 4;
-    int h[4] = { 0, 0, 0, 0 };
+    size_t hard = 0, lds_cu = 0;
+    lds_limits( &hard, &lds_cu );
+    // tile width and height: the requested ones (2 x 2 columns, 3 cells by
+    // default), lowered while the neighbourhood and rows do not fit in LDS,
+    // or while the force kernel would fit fewer blocks per CU than on the
+    // layout the autotuner chose (occ_blocks): a denser neighbourhood than
+    // the tuned one (the perfect lattice at step 0) would otherwise lose a
+    // block of occupancy
+    const int W = N->tile_w > 0 ? N->tile_w : 2;
+    int E = std::min( N->tile_e > 0 ? N->tile_e : ( W == 2 ? 3 : 4 ), EMAX - 2 );
+    int h[6] = { 0, 0, 0, 0, 0, 0 };
     for ( int attempt = 0; attempt < 4; ++attempt )
     {
         size_t lds = 0;
         for ( ;; )
         {
-            CMD_TRY( tile_info_for( N, S, std::min( E, EMAX - 2 ), h ) );
+            CMD_TRY( tile_info_for( N, S, W, E, h ) );
             // local atoms in the grid's padding cells (not wrapped into the
             // box) belong to no tile: use the global layout
             if ( h[1] != S->n_local )
                 return CMD_OK;
             const int R4 = ( ( N->cap + G - 1 ) / G + 3 ) / 4;
-            const size_t wbytes =
-                (size_t)( 64 / G ) * tile_row_stride( R4, G ) * sizeof( unsigned short );
-            // coordinate planes of the staged atoms and the sentinel slot,
-            // the trash area of rejected entries, then the rows of every
-            // slab wave of the tile
-            lds = tile_planes_bytes( h[0] + 1 ) + 64 * sizeof( unsigned short ) +
-                  (size_t)std::max( h[2], 1 ) * wbytes;
-            // the force kernels' records on the same tiles (lj.hip): a
-            // position record and, with several types, a type per slot;
-            // the half list adds a reaction record and an energy weight
-            const size_t slot_bytes = TILE_REC + sizeof( int );
-            const size_t force_lds =
-                (size_t)( h[0] + 1 ) * slot_bytes * ( N->half ? 2 : 1 ) + 8;
-            if ( h[0] <= MAX_SLOTS && lds <= hard && force_lds <= hard )
+            // the build block: coordinate planes of its 3 x 3 columns and
+            // the sentinel slot, the trash area of rejected entries, then
+            // the rows of its atom column
+            lds = tile_planes_bytes( h[3] + 1 ) + 64 * sizeof( unsigned short ) +
+                  ( (size_t)std::max( h[4], 1 ) * tile_row_stride( R4, G ) + 8 ) *
+                      sizeof( unsigned short );
+            const size_t flds = force_lds( h[0], N->half != 0 );
+            const bool fits = h[0] <= MAX_SLOTS && lds <= hard && flds <= hard;
+            // blocks per CU of the force kernel (its static tables included)
+            const int blocks = (int)std::min( lds_cu / ( flds + 1024 ),
+                                              (size_t)( 32 / N->tg.bw ) );
+            if ( fits && ( N->occ_blocks == 0 || blocks >= N->occ_blocks || E == 1 ) )
+            {
+                N->tile_blocks = blocks;
                 break;
+            }
             if ( E == 1 )
                 return CMD_OK; // too dense for LDS tiles: global path
-            E = std::max( 1, E * 2 / 3 );
+            E = fits ? E - 1 : std::max( 1, E * 2 / 3 );
         }
         N->max_tile_slots = h[0];
-        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[3] ) ) );
+        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[5] ) ) );
         CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, S->stream ) );
-        const int slot_cap = h[0] + 1;
+        const int slot_cap = h[3] + 1;
         switch ( G )
         {
         case 2:

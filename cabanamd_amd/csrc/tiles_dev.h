@@ -1,38 +1,53 @@
 // tiles_dev.h -- device helpers shared by the tiled list build (tiles.hip)
-// and the tiled force kernel (lj.hip).  See tiles.hip for the layout.
+// and the tiled force kernels (lj.hip).  See tiles.hip for the layout.
 #pragma once
 
 #include "internal.h"
 
 namespace cmd
 {
-static constexpr int TI_WORDS = 44;
+// A tile is W x W interior columns of the binning grid (W = 1 or 2) over E
+// cells [e0, e1).  Its atoms are the local atoms of those cells: one
+// contiguous run per atom column (the cell-sorted order runs along a
+// column), taken in atom-column order c = ca * W + cb.  Every neighbour
+// candidate of those atoms lies in the (W + 2) x (W + 2) staged columns
+// around them, cells [e0 - 1, e1]: per staged column one contiguous run of
+// locals and one of cell-sorted ghosts (segments 2q and 2q + 1 of staged
+// column q = da * (W + 2) + db).  Staged columns past the padded grid (the
+// right edge of a tile whose second atom column does not exist) are empty.
+constexpr int TILE_WMAX = 2;
+constexpr int TILE_QMAX = ( TILE_WMAX + 2 ) * ( TILE_WMAX + 2 ); // staged columns
+constexpr int TILE_CMAX = TILE_WMAX * TILE_WMAX;                 // atom columns
+static constexpr int TI_WORDS = 4 * TILE_QMAX + 2 * TILE_CMAX + 4;
 enum
 {
-    TI_LB = 0,     // [9] first local index of each stencil column run
-    TI_LLEN = 9,   // [9] local run lengths
-    TI_GB = 18,    // [9] first sorted-ghost slot of each column run
-    TI_GLEN = 27,  // [9] ghost run lengths
-    TI_TL = 36,    // first local atom of the tile
-    TI_NAT = 37,   // atoms in the tile
-    TI_TOTAL = 38, // staged positions
-    TI_EA = 39,    // first staged cell (e0 - 1, clamped)
-    TI_NST = 40    // staged cells per column
+    TI_LB = 0,                         // [Q] first local index of each staged column run
+    TI_LLEN = TILE_QMAX,               // [Q] local run lengths
+    TI_GB = 2 * TILE_QMAX,             // [Q] first sorted-ghost slot of each run
+    TI_GLEN = 3 * TILE_QMAX,           // [Q] ghost run lengths
+    TI_AL = 4 * TILE_QMAX,             // [C] first local atom of each atom column
+    TI_AN = 4 * TILE_QMAX + TILE_CMAX, // [C] atoms of each atom column
+    TI_NAT = 4 * TILE_QMAX + 2 * TILE_CMAX, // atoms in the tile
+    TI_TOTAL,                               // staged positions
+    TI_EA,                                  // first staged cell (e0 - 1, clamped)
+    TI_NST                                  // staged cells per column
 };
 
 constexpr int TILE_STRIP = 4;
 
-__device__ __forceinline__ void tile_coords( const TileGrid &T, int t, int &a,
-                                             int &b, int &e0, int &e1 )
+__host__ __device__ constexpr int tile_q( int W ) { return ( W + 2 ) * ( W + 2 ); }
+
+__device__ __forceinline__ void tile_coords( const TileGrid &T, int t, int &a0,
+                                             int &b0, int &e0, int &e1 )
 {
-    // columns in strips of TILE_STRIP rows of a, b outer and a inner within
-    // a strip: the 3 x 3 columns around a column run close together in
-    // time, so a staged column is mostly re-read from L2
+    // tile columns in strips of TILE_STRIP rows of a, b outer and a inner
+    // within a strip: the columns around a tile are staged by its neighbours
+    // close together in time, so a staged column is mostly re-read from L2
     const int col = t / T.tpc, z = t % T.tpc;
-    const int strip = col / ( TILE_STRIP * T.nc1 ), r = col % ( TILE_STRIP * T.nc1 );
-    const int h = min( TILE_STRIP, T.nc0 - strip * TILE_STRIP );
-    a = 1 + strip * TILE_STRIP + r % h;
-    b = 1 + r / h;
+    const int strip = col / ( TILE_STRIP * T.nt1 ), r = col % ( TILE_STRIP * T.nt1 );
+    const int h = min( TILE_STRIP, T.nt0 - strip * TILE_STRIP );
+    a0 = 1 + T.W * ( strip * TILE_STRIP + r % h );
+    b0 = 1 + T.W * ( r / h );
     e0 = 1 + z * T.E;
     e1 = min( e0 + T.E, T.g.n[2] - 1 );
 }
@@ -51,15 +66,18 @@ __device__ __forceinline__ int xcd_tile( int b, int nb )
     return ( xcd < r ) ? xcd * ( q + 1 ) + k : r * ( q + 1 ) + ( xcd - r ) * q + k;
 }
 
-// Shared tile header: info words and the 19 segment offsets
-// (segment 2q = locals of column q, 2q+1 = its ghosts).
+// Shared tile header: info words, the segment offsets in LDS slots
+// (segment 2q = locals of staged column q, 2q + 1 = its ghosts) and the
+// tile-atom offsets of the atom columns.
 struct TileHead
 {
     int info[TI_WORDS];
-    int segoff[19];
+    int segoff[2 * TILE_QMAX + 1];
+    int aoff[TILE_CMAX + 1];
+    int nseg; // 2 (W + 2)^2
 };
 
-__device__ inline void load_head( const int *__restrict__ info, int t, TileHead &H )
+__device__ inline void load_head( const int *__restrict__ info, int t, int W, TileHead &H )
 {
     const int *I = info + (size_t)t * TI_WORDS;
     for ( int k = threadIdx.x; k < TI_WORDS; k += blockDim.x )
@@ -67,17 +85,41 @@ __device__ inline void load_head( const int *__restrict__ info, int t, TileHead 
     __syncthreads();
     if ( threadIdx.x == 0 )
     {
+        const int nq = tile_q( W );
         int off = 0;
-        for ( int q = 0; q < 9; ++q )
+        for ( int q = 0; q < nq; ++q )
         {
             H.segoff[2 * q] = off;
             off += H.info[TI_LLEN + q];
             H.segoff[2 * q + 1] = off;
             off += H.info[TI_GLEN + q];
         }
-        H.segoff[18] = off;
+        H.segoff[2 * nq] = off;
+        H.nseg = 2 * nq;
+        int a = 0;
+        for ( int c = 0; c < W * W; ++c )
+        {
+            H.aoff[c] = a;
+            a += H.info[TI_AN + c];
+        }
+        H.aoff[W * W] = a;
     }
     __syncthreads();
+}
+
+// System index of tile atom ta (ta < the tile's atom count).
+__device__ __forceinline__ int tile_atom( const TileHead &H, int W, int ta )
+{
+    int c = 0;
+    while ( c + 1 < W * W && ta >= H.aoff[c + 1] )
+        ++c;
+    return H.info[TI_AL + c] + ( ta - H.aoff[c] );
+}
+
+// Staged column of atom column c.
+__device__ __forceinline__ int atom_col_q( int W, int c )
+{
+    return ( c / W + 1 ) * ( W + 2 ) + ( c % W + 1 );
 }
 
 // Position of a tile's sentinel slot (slot TOTAL, just past the staged
@@ -105,16 +147,16 @@ __device__ __forceinline__ int wave_lanes( int G, int nat, int w )
     return g;
 }
 
-// Copy the 18 runs of positions (locals from x, ghosts from xg) into LDS as
-// 24-byte records, followed by the sentinel record; with WITH_TYPE the types
-// follow as ints, `cap` records on.
+// Copy the staged runs of positions (locals from x, ghosts from xg) into LDS
+// as 24-byte records, followed by the sentinel record; with WITH_TYPE the
+// types follow as ints, `cap` records on.
 template <bool WITH_TYPE>
 __device__ inline void stage_records( const TileHead &H, const double4 *__restrict__ x,
                                       const double4 *__restrict__ xg,
                                       double *__restrict__ s_rec, int cap )
 {
     int *s_t = reinterpret_cast<int *>( s_rec + 3 * cap );
-    const int total = H.segoff[18];
+    const int total = H.segoff[H.nseg];
     constexpr int B = 2; // slots per thread whose loads are in flight together
     int seg = 0;
     for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
@@ -150,57 +192,6 @@ __device__ inline void stage_records( const TileHead &H, const double4 *__restri
         s_rec[3 * total] = TILE_FAR;
         s_rec[3 * total + 1] = TILE_FAR;
         s_rec[3 * total + 2] = TILE_FAR;
-        if ( WITH_TYPE )
-            s_t[total] = 0;
-    }
-    __syncthreads();
-}
-
-// The same staging as separate coordinate planes (x, y, z doubles, and the
-// type as an int when more than one type is present), `cap` slots each: a
-// gather then reads three 8-byte words, which spread over the LDS banks
-// better than one 32-byte record.
-template <bool WITH_TYPE>
-__device__ inline void stage_planes( const TileHead &H, const double4 *__restrict__ x,
-                                     const double4 *__restrict__ xg,
-                                     double *__restrict__ s_x, int cap )
-{
-    double *s_y = s_x + cap, *s_z = s_x + 2 * cap;
-    int *s_t = reinterpret_cast<int *>( s_x + 3 * cap );
-    const int total = H.segoff[18];
-    constexpr int B = 4;
-    int seg = 0;
-    for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
-    {
-        double4 p[B];
-#pragma unroll
-        for ( int u = 0; u < B; ++u )
-        {
-            const int k = min( k0 + u * (int)blockDim.x, total - 1 );
-            while ( k >= H.segoff[seg + 1] )
-                ++seg;
-            const int q = seg >> 1, off = k - H.segoff[seg];
-            const double4 *__restrict__ base = ( seg & 1 ) ? xg : x;
-            const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
-            p[u] = base[idx];
-        }
-#pragma unroll
-        for ( int u = 0; u < B; ++u )
-        {
-            const int k = min( k0 + u * (int)blockDim.x, total );
-            s_x[k] = p[u].x;
-            s_y[k] = p[u].y;
-            s_z[k] = p[u].z;
-            if ( WITH_TYPE )
-                s_t[k] = (int)p[u].w;
-        }
-    }
-    __syncthreads();
-    if ( threadIdx.x == 0 )
-    {
-        s_x[total] = TILE_FAR;
-        s_y[total] = TILE_FAR;
-        s_z[total] = TILE_FAR;
         if ( WITH_TYPE )
             s_t[total] = 0;
     }

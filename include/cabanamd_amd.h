@@ -159,8 +159,12 @@ cmd_status cmd_neigh_destroy( cmd_neigh *n );
  * force kernels follow the layout of the list they are given. */
 cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes );
 /* Tiled layout control (no reference counterpart; for tuning and tests):
- * tile height in binning cells (0 = choose). */
+ * tile height in binning cells (0 = choose) and tile width, 1 or 2 binning
+ * columns per side (0 = choose: 2); the shape of the last tiled build
+ * (0, 0 when the list is not tiled). */
 cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height );
+cmd_status cmd_neigh_set_tile_width( cmd_neigh *n, int width );
+cmd_status cmd_neigh_tile_shape( const cmd_neigh *n, int *width, int *height );
 cmd_status cmd_neigh_build( cmd_neigh *n, cmd_system *s );
 cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
                            int *max_row, int *max_neigh_guess );
@@ -363,6 +367,7 @@ cmd_status cmd_md_set_lanes( cmd_md *md, int lanes );
 /* Tile height (cells) of the tiled list when lanes are fixed (no reference
  * counterpart; for tuning): 0 keeps the default. */
 cmd_status cmd_md_set_tile( cmd_md *md, int height );
+cmd_status cmd_md_set_tile_width( cmd_md *md, int width );
 /* Ghost refresh fused into the step (no reference counterpart; on by
  * default): on tiled full lists the fused step also writes the next step's
  * sorted ghost positions, in place of a separate update_halo pass.  Off
@@ -406,11 +411,13 @@ cmd_status cmd_md_timers( const cmd_md *md, double out[6] );
 cmd_status cmd_md_profile( cmd_md *md, int enable );
 cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
                                 double *mean_ms );
-/* The list layout init chose (lanes per atom, tile height in cells, 0 for
- * the global slab) and the host seconds init spent choosing it; no
+/* The list layout init chose (lanes per atom, tile width in columns and
+ * height in cells, 0 for the global slab) and the host seconds init spent
+ * choosing it; no
  * reference counterpart (the reference's layout is fixed at compile time,
  * neighbor_verlet.h:30-32). */
-cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile, double *seconds );
+cmd_status cmd_md_tune_info( const cmd_md *md, int *lanes, int *tile_width, int *tile,
+                             double *seconds );
 
 /* ---- batched ensemble -------------------------------------------------------
  * No reference counterpart: CabanaMD steps one system per process.  A batch

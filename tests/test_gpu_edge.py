@@ -28,7 +28,7 @@ def _grid_lists(x, L, half):
     return x, xg, off, idx, send, counts
 
 
-def _engine(M, x, L, half, lanes, guess=50):
+def _engine(M, x, L, half, lanes, guess=50, width=0):
     s = M.System(1)
     s.create_domain([0, 0, 0], L, RC)
     s.upload(x, np.zeros_like(x), None, np.zeros(len(x), np.int32),
@@ -36,14 +36,15 @@ def _engine(M, x, L, half, lanes, guess=50):
     M.Binning(s).create_binning(RC, RC, RC, 1, True, False, True)
     comm = M.Comm(s, RC)
     comm.exchange_halo()
-    nl = M.NeighborVerlet(RC, half, guess, lanes=lanes)
+    nl = M.NeighborVerlet(RC, half, guess, lanes=lanes, tile_width=width)
     nl.create(s)
     return s, comm, nl
 
 
+@pytest.mark.parametrize("width", [1, 2])
 @pytest.mark.parametrize("half", [False, True])
 @pytest.mark.parametrize("lanes", [2, 8])
-def test_dense_box_lists_exact(gpu, half, lanes):
+def test_dense_box_lists_exact(gpu, half, lanes, width):
     """Density 2.4 (about 53 atoms per cell, some above 64; about 1,400
     stencil candidates per cell, so the tiled build takes two register
     passes), rows of ~220: lists equal to the oracle's, entry for entry."""
@@ -53,7 +54,7 @@ def test_dense_box_lists_exact(gpu, half, lanes):
     n = int(2.4 * np.prod(L))
     x = rng.uniform(0.0, 1.0, (n, 3)) * np.array(L)
     xw, xg, off_o, idx_o, _, _ = _grid_lists(x, L, half)
-    s, comm, nl = _engine(M, x, L, half, lanes, guess=20)
+    s, comm, nl = _engine(M, x, L, half, lanes, guess=20, width=width)
     lay = nl.layout()
     print("layout", lay, "rows up to", nl.info()["max_row"])
     assert lay["tiled"]  # the register-pass build, not the global fallback

@@ -200,9 +200,10 @@ def test_neighbor_lattice_exact(gpu, half):
     assert nl.max_neigh_guess >= int(np.diff(off).max())
 
 
+@pytest.mark.parametrize("width", [1, 2])
 @pytest.mark.parametrize("half", [False, True])
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8])
-def test_neighbor_on_binning_grid_exact(gpu, half, lanes):
+def test_neighbor_on_binning_grid_exact(gpu, half, lanes, width):
     """After create_binning the list is searched on the binning grid; rows
     must equal the oracle's build on that grid, in any slab layout."""
     M = gpu
@@ -222,19 +223,22 @@ def test_neighbor_on_binning_grid_exact(gpu, half, lanes):
     M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
     comm = M.Comm(s, 2.8)
     comm.exchange_halo()
-    nl = M.NeighborVerlet(2.8, half, 50, lanes=lanes)
+    nl = M.NeighborVerlet(2.8, half, 50, lanes=lanes, tile_width=width)
     nl.create(s)
     off, idx = nl.get()
+    lay = nl.layout()
+    assert lay["tiled"] == (lanes > 1)
+    assert lay["tile_width"] == (width if lanes > 1 else 0)
     assert np.array_equal(s.download()[0], xg)
     assert np.array_equal(off, off_o)
     assert np.array_equal(idx, idx_o)
 
 
-@pytest.mark.parametrize("tile", [0, 8])
+@pytest.mark.parametrize("shape", ["1x0", "1x8", "2x0", "2x2"])
 @pytest.mark.parametrize("wrapped", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("lanes", [2, 4, 8])
-def test_lj_tiled_force(gpu, prec, lanes, wrapped, tile):
+def test_lj_tiled_force(gpu, prec, lanes, wrapped, shape):
     """Full list built on the binning grid with 2+ lanes per atom uses the
     LDS-tiled layout when every local atom lies in an interior cell; atoms
     left unwrapped in the padding cells make the build use the global layout.
@@ -251,8 +255,14 @@ def test_lj_tiled_force(gpu, prec, lanes, wrapped, tile):
     M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
     comm = M.Comm(s, 2.8)
     comm.exchange_halo()
-    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes, tile=tile)
+    width, tile = (int(v) for v in shape.split("x"))
+    nl = M.NeighborVerlet(2.8, False, 50, lanes=lanes, tile=tile, tile_width=width)
     nl.create(s)
+    lay = nl.layout()
+    assert lay["tiled"] == wrapped
+    if wrapped:
+        assert lay["tile_width"] == width
+        assert lay["tile_height"] == (tile or (3 if width == 2 else 4))
     F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
     F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
     F.compute(s, nl)
@@ -578,8 +588,8 @@ def test_init_tuning_restores_initial_state(gpu):
     a.init()
     b = M.CbnMD(M.lattice_params(**kw), precision=M.PREC_FP64, lanes=1)
     b.init()
-    g, tile, secs = a.tune_info()
-    print(f"tuned layout: lanes {g}, tile {tile}, {secs:.3f} s", flush=True)
+    g, tile, secs, width = a.tune_info()
+    print(f"tuned layout: lanes {g}, tile {width}x{width}x{tile}, {secs:.3f} s", flush=True)
     assert g in (1, 2, 4, 8) and secs > 0.0
     xa, va, _, _, ia = a.arrays()
     xb, vb, _, _, ib = b.arrays()
