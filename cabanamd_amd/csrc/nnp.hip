@@ -81,6 +81,13 @@ struct NnpElemPlan
     float b2[HPAD];
     float w3[HPAD];
     float b3;
+    // The layer weights as three BF16 parts (w = hi + mid + lo to FP32
+    // precision), laid out as the MFMA B operands read them: [out][in] for
+    // the forward products, [in][out] for the backward ones.
+    alignas( 16 ) unsigned short W1s[3][HPAD][KPAD];
+    alignas( 16 ) unsigned short W1Ts[3][KPAD][HPAD];
+    alignas( 16 ) unsigned short W2s[3][HPAD][HPAD];
+    alignas( 16 ) unsigned short W2Ts[3][HPAD][HPAD];
 };
 
 struct NnpPlan
@@ -141,6 +148,35 @@ static int find_or_add( std::vector<std::pair<double, double>> &v, double a,
             return (int)k;
     v.push_back( { a, b } );
     return (int)v.size() - 1;
+}
+
+// BF16 bits of a float, rounded to nearest even (the format's NaN and
+// infinity are not produced from the finite weights and activations here).
+__host__ __device__ inline unsigned short bf16_bits( float f )
+{
+    unsigned u;
+    std::memcpy( &u, &f, sizeof( u ) );
+    u += 0x7fffu + ( ( u >> 16 ) & 1u );
+    return (unsigned short)( u >> 16 );
+}
+
+__host__ __device__ inline float bf16_value( unsigned short b )
+{
+    const unsigned u = (unsigned)b << 16;
+    float f;
+    std::memcpy( &f, &u, sizeof( f ) );
+    return f;
+}
+
+// x = hi + mid + lo: each part the BF16 rounding of what the parts before it
+// leave, so the three hold the 24 significant bits of an FP32 value.
+__host__ __device__ inline void bf16_split3( float x, unsigned short part[3] )
+{
+    part[0] = bf16_bits( x );
+    const float r1 = x - bf16_value( part[0] );
+    part[1] = bf16_bits( r1 );
+    const float r2 = r1 - bf16_value( part[1] );
+    part[2] = bf16_bits( r2 );
 }
 
 static cmd_status build_plan( const cmd_nnp_model &m, NnpPlan &P )
@@ -297,6 +333,29 @@ static cmd_status build_plan( const cmd_nnp_model &m, NnpPlan &P )
         for ( int k = 0; k < Q.h2; ++k )
             Q.w3[k] = (float)E.W[2][k];
         Q.b3 = (float)E.b[2][0];
+        for ( int o = 0; o < HPAD; ++o )
+        {
+            for ( int k = 0; k < KPAD; ++k )
+            {
+                unsigned short part[3];
+                bf16_split3( Q.W1[o][k], part );
+                for ( int q = 0; q < 3; ++q )
+                {
+                    Q.W1s[q][o][k] = part[q];
+                    Q.W1Ts[q][k][o] = part[q];
+                }
+            }
+            for ( int k = 0; k < HPAD; ++k )
+            {
+                unsigned short part[3];
+                bf16_split3( Q.W2[o][k], part );
+                for ( int q = 0; q < 3; ++q )
+                {
+                    Q.W2s[q][o][k] = part[q];
+                    Q.W2Ts[q][k][o] = part[q];
+                }
+            }
+        }
     }
     P.rc = (float)rc;
     P.rc2 = (float)( rc * rc );
@@ -701,24 +760,66 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     }
 }
 
-// ---- network: BF16 MFMA, 16 atoms of one element per wave
-typedef short bf16x4 __attribute__( ( ext_vector_type( 4 ) ) );
+// ---- network: split-BF16 MFMA, 16 atoms of one element per wave
+//
+// Every layer product runs on gfx950's 16x16x32 BF16 MFMA with FP32
+// accumulation.  Each FP32 operand is split into three BF16 parts (a = ah +
+// am + al, exact for FP32 values), and a product sums the six part products
+// of order up to 2^-16 relative (ah bh, ah bm, am bh, am bm, ah bl, al bh),
+// smallest first: the result matches an FP32 product to about 2^-24, while a
+// single BF16 product carries 2^-8.  The weights come pre-split from the
+// plan; activations are split as they are loaded.
+typedef __bf16 bf16x8 __attribute__( ( ext_vector_type( 8 ) ) );
+typedef short s16x8 __attribute__( ( ext_vector_type( 8 ) ) );
 typedef float f32x4 __attribute__( ( ext_vector_type( 4 ) ) );
-
-__device__ __forceinline__ short to_bf16( float f )
-{
-    unsigned u = __float_as_uint( f );
-    u += 0x7fffu + ( ( u >> 16 ) & 1u ); // round to nearest even
-    return (short)( u >> 16 );
-}
 
 __device__ __forceinline__ float act_f( float z, int tanh_ )
 {
     return tanh_ ? tanhf( z ) : z;
 }
 
-// MFMA 16x16x16 operand layout (per lane l): A[row = l % 16][k = 4 (l / 16)
-// + 0..3], B[k = 4 (l / 16) + 0..3][col = l % 16], D[row = 4 (l / 16) + 0..3]
+// The three BF16 parts of 8 consecutive FP32 values.
+__device__ __forceinline__ void split8( const float *v, bf16x8 &h, bf16x8 &m, bf16x8 &l )
+{
+    s16x8 sh, sm, sl;
+#pragma unroll
+    for ( int u = 0; u < 8; ++u )
+    {
+        unsigned short part[3];
+        bf16_split3( v[u], part );
+        sh[u] = (short)part[0];
+        sm[u] = (short)part[1];
+        sl[u] = (short)part[2];
+    }
+    h = __builtin_bit_cast( bf16x8, sh );
+    m = __builtin_bit_cast( bf16x8, sm );
+    l = __builtin_bit_cast( bf16x8, sl );
+}
+
+__device__ __forceinline__ bf16x8 load8( const unsigned short *p )
+{
+    return __builtin_bit_cast( bf16x8, *reinterpret_cast<const s16x8 *>( p ) );
+}
+
+// c += A B over one 32-deep K block, A from the three parts of the lane's
+// 8 activations, B from the three weight parts at w[0], w[stride], w[2 stride].
+__device__ __forceinline__ f32x4 mfma_split( const float *a8, const unsigned short *w,
+                                             size_t stride, f32x4 c )
+{
+    bf16x8 ah, am, al;
+    split8( a8, ah, am, al );
+    const bf16x8 bh = load8( w ), bm = load8( w + stride ), bl = load8( w + 2 * stride );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( al, bh, c, 0, 0, 0 );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( ah, bl, c, 0, 0, 0 );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( am, bm, c, 0, 0, 0 );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( am, bh, c, 0, 0, 0 );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( ah, bm, c, 0, 0, 0 );
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16( ah, bh, c, 0, 0, 0 );
+    return c;
+}
+
+// MFMA 16x16x32 operand layout (per lane l): A[row = l % 16][k = 8 (l / 16)
+// + 0..7], B[k = 8 (l / 16) + 0..7][col = l % 16], D[row = 4 (l / 16) + 0..3]
 // [col = l % 16].
 __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restrict__ list,
                                                    const int *__restrict__ count,
@@ -735,7 +836,7 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restr
         return;
     const NnpElemPlan &Q = P->el[elem];
     float( *A )[KPAD + 4] = sA[wv];
-    const int row = lane & 15, kq = ( lane >> 4 ) * 4;
+    const int row = lane & 15, kq = ( lane >> 4 ) * 4, k8 = ( lane >> 4 ) * 8;
     // scaled inputs, row-major in LDS
     for ( int k = lane; k < 16 * KPAD; k += 64 )
     {
@@ -749,24 +850,16 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restr
         A[r][c] = v;
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-    // layer 1: h1[16 x 32] = Gs[16 x 96] W1^T
+    // layer 1: h1[16 x 32] = Gs[16 x 96] W1^T, B[k][col] = W1[col][k]
     f32x4 h1[2];
 #pragma unroll
     for ( int nt = 0; nt < 2; ++nt )
     {
         f32x4 c = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-        for ( int kb = 0; kb < KPAD / 16; ++kb )
-        {
-            bf16x4 a, b;
-#pragma unroll
-            for ( int u = 0; u < 4; ++u )
-            {
-                a[u] = to_bf16( A[row][kb * 16 + kq + u] );
-                b[u] = to_bf16( Q.W1[nt * 16 + row][kb * 16 + kq + u] );
-            }
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k( a, b, c, 0, 0, 0 );
-        }
+        for ( int kb = 0; kb < KPAD / 32; ++kb )
+            c = mfma_split( &A[row][kb * 32 + k8], &Q.W1s[0][nt * 16 + row][kb * 32 + k8],
+                            (size_t)HPAD * KPAD, c );
         h1[nt] = c;
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
@@ -783,26 +876,12 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restr
             A[r][col] = h;
         }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-    // layer 2: h2[16 x 32] = h1 W2^T
+    // layer 2: h2[16 x 32] = h1 W2^T, B[k][col] = W2[col][k]
     f32x4 h2[2];
 #pragma unroll
     for ( int nt = 0; nt < 2; ++nt )
-    {
-        f32x4 c = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for ( int kb = 0; kb < 2; ++kb )
-        {
-            bf16x4 a, b;
-#pragma unroll
-            for ( int u = 0; u < 4; ++u )
-            {
-                a[u] = to_bf16( A[row][kb * 16 + kq + u] );
-                b[u] = to_bf16( Q.W2[nt * 16 + row][kb * 16 + kq + u] );
-            }
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k( a, b, c, 0, 0, 0 );
-        }
-        h2[nt] = c;
-    }
+        h2[nt] = mfma_split( &A[row][k8], &Q.W2s[0][nt * 16 + row][k8], (size_t)HPAD * HPAD,
+                             f32x4{ 0.f, 0.f, 0.f, 0.f } );
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
     // output layer and the first backward step (VALU, FP32):
     // E = w3 . h2 + b3; delta2 = w3 * act'(z2)
@@ -839,22 +918,8 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restr
     f32x4 d1[2];
 #pragma unroll
     for ( int nt = 0; nt < 2; ++nt )
-    {
-        f32x4 c = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for ( int kb = 0; kb < 2; ++kb )
-        {
-            bf16x4 a, b;
-#pragma unroll
-            for ( int u = 0; u < 4; ++u )
-            {
-                a[u] = to_bf16( A[row][kb * 16 + kq + u] );
-                b[u] = to_bf16( Q.W2[kb * 16 + kq + u][nt * 16 + row] );
-            }
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k( a, b, c, 0, 0, 0 );
-        }
-        d1[nt] = c;
-    }
+        d1[nt] = mfma_split( &A[row][k8], &Q.W2Ts[0][nt * 16 + row][k8], (size_t)HPAD * HPAD,
+                             f32x4{ 0.f, 0.f, 0.f, 0.f } );
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
 #pragma unroll
     for ( int nt = 0; nt < 2; ++nt )
@@ -870,19 +935,8 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp( int elem, const int *__restr
 #pragma unroll
     for ( int nt = 0; nt < KPAD / 16; ++nt )
     {
-        f32x4 c = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for ( int kb = 0; kb < 2; ++kb )
-        {
-            bf16x4 a, b;
-#pragma unroll
-            for ( int u = 0; u < 4; ++u )
-            {
-                a[u] = to_bf16( A[row][kb * 16 + kq + u] );
-                b[u] = to_bf16( Q.W1[kb * 16 + kq + u][nt * 16 + row] );
-            }
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k( a, b, c, 0, 0, 0 );
-        }
+        const f32x4 c = mfma_split( &A[row][k8], &Q.W1Ts[0][nt * 16 + row][k8],
+                                    (size_t)KPAD * HPAD, f32x4{ 0.f, 0.f, 0.f, 0.f } );
         const int col = nt * 16 + row;
 #pragma unroll
         for ( int u = 0; u < 4; ++u )
@@ -1749,8 +1803,8 @@ cmd_status cmd_force_nnp_create( const cmd_nnp_model *m, cmd_force **out )
     f->ntypes = m->nelem;
     f->nnp = new cmd_nnp;
     f->nnp->model = *m;
-    NnpPlan probe;
-    const cmd_status st = build_plan( f->nnp->model, probe );
+    std::vector<NnpPlan> probe( 1 ); // a plan holds the split weights: keep it off the stack
+    const cmd_status st = build_plan( f->nnp->model, probe[0] );
     if ( st != CMD_OK )
     {
         delete f->nnp;

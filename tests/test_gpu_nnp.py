@@ -1,12 +1,15 @@
 """NNP force on the GPU against the FP64 NNP oracle, through the C ABI.
 
-Symmetry functions and forces run in FP32 and the network layers in BF16
-MFMA (CMD_PREC_MIXED) or FP32 (CMD_PREC_FP64), so the bars are relative:
+Symmetry functions and forces run in FP32 and the network layers in
+split-BF16 MFMA (CMD_PREC_MIXED: each FP32 operand as three BF16 parts, six
+part products per layer product, FP32 accumulation) or FP32 on the VALU
+(CMD_PREC_FP64), so the bars are relative:
   G:        max |dG| <= 2e-5 * max |G| per function
-  E_i:      FP32 layers 1e-4 of the energy scale, BF16 layers 3e-2
-  forces:   ||dF||/||F|| <= 1e-3 (FP32 layers), 5e-2 (BF16 layers)
-BF16 has an 8-bit mantissa (~4e-3 relative per operand), which bounds what
-the MFMA layers can reach.
+  E_i:      1e-4 of the energy scale
+  forces:   ||dF||/||F|| <= 1e-3, and <= 1e-5 on the benchmark model (the
+            bar the batched and served forms are held to)
+A single BF16 product would carry its 8-bit mantissa (~4e-3 relative per
+operand); the split brings the layers to FP32 accuracy.
 """
 import numpy as np
 import pytest
@@ -74,9 +77,9 @@ def _check(r, nsf, prec):
     scale = np.maximum(np.abs(Go).max(axis=0), 1e-12)
     assert (np.abs(G - Go) / scale).max() < 2e-5
     escale = max(np.abs(r["Eo"]).max(), 1e-3)
-    etol = 1e-4 if prec == "fp32" else 3e-2
+    etol = 1e-4
     assert np.abs(r["E"] - r["Eo"]).max() / escale < etol
-    ftol = 1e-3 if prec == "fp32" else 5e-2
+    ftol = 1e-3
     err = np.linalg.norm(r["f"] - r["fo"]) / np.linalg.norm(r["fo"])
     assert err < ftol, err
     assert np.abs(r["fo"]).max() > 1e-6  # forces are not trivially zero
@@ -112,11 +115,13 @@ def test_nnp_benchmark_model_after_scaling_fit(gpu):
     # the fitted ranges cover every atom's functions
     assert all(np.all(m["gmax"][e] >= m["gmin"][e]) for e in range(2))
     r = _run(M, s, comm, nl, F, m)
-    _check(r, 70, "fp32")
+    err32 = _check(r, 70, "fp32")
     F.set_precision(M.PREC_MIXED)
     r = _run(M, s, comm, nl, F, m)
     err = _check(r, 70, "bf16")
-    print(f"benchmark model bf16: force rel L2 {err:.2e}")
+    print(f"benchmark model: force rel L2 {err32:.2e} (FP32 layers), "
+          f"{err:.2e} (split-BF16 MFMA layers)")
+    assert err32 < 1e-5 and err < 1e-5
 
 
 def test_nnp_reference_fixture(gpu):
@@ -190,4 +195,4 @@ def test_nnp_md_loop_conserves_energy(gpu, prec):
     drift = np.abs(e - e[0]).max() / np.abs(e).max()
     print(f"NNP MD {prec}: energy drift {drift:.2e} over 100 steps")
     assert th.shape[0] == 11
-    assert drift < (1e-3 if prec == "fp32" else 1e-2)
+    assert drift < 1e-3
