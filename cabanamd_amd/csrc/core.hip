@@ -831,9 +831,12 @@ __global__ void k_permute( int n, int cap, const int *__restrict__ perm,
     vo[k] = v[s];
     vo[cap + k] = v[cap + s];
     vo[2 * cap + k] = v[2 * cap + s];
-    fo[k] = f[s];
-    fo[cap + k] = f[cap + s];
-    fo[2 * cap + k] = f[2 * cap + s];
+    if ( f ) // forces that a caller still reads
+    {
+        fo[k] = f[s];
+        fo[cap + k] = f[cap + s];
+        fo[2 * cap + k] = f[2 * cap + s];
+    }
     ido[k] = id[s];
 }
 
@@ -1419,6 +1422,19 @@ cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
                                double dz_in, int halo_depth, int do_local,
                                int do_ghost, int sort, int *perm_out )
 {
+    return binning_create( s, dx_in, dy_in, dz_in, halo_depth, do_local, do_ghost, sort,
+                           perm_out, true );
+}
+
+} // extern "C"
+
+// The same, with the forces left out of the permutation when keep_f is
+// false: the time loop recomputes them after every list rebuild before
+// anything reads them, so their rows are left in the old order.
+cmd_status cmd::binning_create( cmd_system *s, double dx_in, double dy_in, double dz_in,
+                                int halo_depth, int do_local, int do_ghost, int sort,
+                                int *perm_out, bool keep_f )
+{
     CMD_REQUIRE( s, CMD_ERR_ARG, "system is NULL" );
     CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
     CMD_REQUIRE( dx_in > 0 && dy_in > 0 && dz_in > 0, CMD_ERR_ARG,
@@ -1458,11 +1474,12 @@ cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
         const int cap = s->cap;
         k_permute<<<nblocks_for( n ), TPB, 0, s->stream>>>(
             n, cap, s->bins.perm.p, s->x.p, s->x_alt.p, s->v.p, s->v_alt.p,
-            s->f.p, s->f_alt.p, s->id.p, s->id_alt.p );
+            keep_f ? s->f.p : nullptr, s->f_alt.p, s->id.p, s->id_alt.p );
         CMD_LAUNCH_CHECK();
         s->x.swap( s->x_alt );
         s->v.swap( s->v_alt );
-        s->f.swap( s->f_alt );
+        if ( keep_f )
+            s->f.swap( s->f_alt );
         s->id.swap( s->id_alt );
         s->n_ghost = 0; // ghosts are not part of the permuted range
         s->bin_grid = g;
@@ -1477,6 +1494,8 @@ cmd_status cmd_binning_create( cmd_system *s, double dx_in, double dy_in,
     }
     return CMD_OK;
 }
+
+extern "C" {
 
 cmd_status cmd_thermo_temperature( cmd_system *s, double *T )
 {

@@ -288,6 +288,18 @@ struct cmd_neigh
     cmd::DevBuf<unsigned short> tslab;
     int max_tile_slots = 0;
     const int *gperm_view = nullptr; // the system's ghost sort at build time
+    // Sizes of the last tiled build (with some headroom), with which the
+    // next build is launched before its own tile sizes are read back
+    // (tiles_build_spec); valid only for the layout they were taken on.
+    struct Spec
+    {
+        bool valid = false;
+        int G = 0, half = 0, W = 0, E = 0, bw_req = 0, cap = 0;
+        int slot_cap = 0;   // build planes: staged slots + sentinel
+        int rows = 0;       // build rows: atoms of one atom column
+        int slab_waves = 0; // slab waves of all tiles
+        size_t lds = 0;     // build block's dynamic LDS
+    } spec;
 };
 
 // LJ parameters passed by value to kernels.
@@ -388,7 +400,24 @@ size_t slab_entries( int n_local, int cap, int lanes );
 cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N );
 // tiles.hip
 cmd_status tiles_prepare_ghosts( cmd_system *S );
+// cmd_binning_create with the forces optionally left out of the
+// permutation (keep_f false: the caller recomputes them before reading).
+cmd_status binning_create( cmd_system *s, double dx_in, double dy_in, double dz_in,
+                           int halo_depth, int do_local, int do_ghost, int sort, int *perm_out,
+                           bool keep_f );
 cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built );
+cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built );
+// Result words of the tiled build in the system's int scratch (sc.ires):
+// the tile-info maxima, then the build's longest row, its capacity flag and
+// the bounding-box test.
+enum
+{
+    TILE_RES_INFO = 0,     // 5 words, k_tile_info
+    TILE_RES_MAXROW = 5,   // longest row of the build
+    TILE_RES_OVER = 6,     // 1: a block's needs exceeded the launch sizes
+    TILE_RES_INSIDE = 7,   // 1: every position lies in the binning grid
+    TILE_RES_WORDS = 8
+};
 cmd_status tiles_csr( cmd_neigh *N, int64_t *nnz );
 cmd_status tiles_ghost_refresh( cmd_system *S );
 } // namespace cmd

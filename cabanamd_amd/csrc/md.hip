@@ -136,12 +136,20 @@ static cmd_status rebuild( cmd_md *md )
 {
     cmd_system *S = md->S;
     int k = md->begin( PH_OTHER );
-    CMD_TRY( cmd_binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut,
-                                 1, 1, 0, 1, nullptr ) );
+    // the time loop computes the forces afresh after a rebuild before it
+    // reads them: they are not carried through the sort
+    CMD_TRY( binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut, 1, 1, 0, 1,
+                             nullptr, false ) );
     md->end( k );
     k = md->begin( PH_COMM );
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
     md->end( k );
+    // ghost force rows are never written by full-list kernels: keep them 0
+    // (issued here, ahead of the list build's wait)
+    if ( !md->N->half )
+        for ( int c = 0; c < 3; ++c )
+            CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap + S->n_local, 0.0,
+                               S->n_ghost, S->stream ) );
     k = md->begin( PH_NEIGH );
     if ( md->retune )
     {
@@ -152,11 +160,6 @@ static cmd_status rebuild( cmd_md *md )
     else
         CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->end( k );
-    // ghost force rows are never written by full-list kernels: keep them 0
-    if ( !md->N->half )
-        for ( int c = 0; c < 3; ++c )
-            CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap + S->n_local, 0.0,
-                               S->n_ghost, S->stream ) );
     return CMD_OK;
 }
 

@@ -76,8 +76,11 @@ __global__ void k_bbox( int n, const double4 *__restrict__ x,
 }
 
 // One 64-lane wave: lane t folds partials t, t+64, ... then a butterfly.
+// With `inside`, it also records whether the box lies in [glo, ghi) on
+// every axis (1) or not (0), the test the tiled build's grid needs.
 __global__ void k_bbox_final( int nb, const double *__restrict__ part,
-                              double *__restrict__ out )
+                              double *__restrict__ out, double3 glo, double3 ghi,
+                              int *__restrict__ inside )
 {
     const int t = threadIdx.x;
     double v[6];
@@ -97,8 +100,15 @@ __global__ void k_bbox_final( int nb, const double *__restrict__ part,
         }
     }
     if ( t == 0 )
+    {
         for ( int k = 0; k < 6; ++k )
             out[k] = v[k];
+        if ( inside )
+            *inside = ( v[0] >= glo.x && v[1] >= glo.y && v[2] >= glo.z && v[3] < ghi.x &&
+                        v[4] < ghi.y && v[5] < ghi.z )
+                          ? 1
+                          : 0;
+    }
 }
 
 __global__ void k_gather_sorted( int n, const int *__restrict__ perm,
@@ -290,13 +300,53 @@ cmd_status neigh_build( cmd_neigh *N, cmd_system *S )
         N->max_row = 0;
         return CMD_OK;
     }
-    // (1) bounding box
     const int nbb = std::min( 1024, std::max( 1, ( nt + TPB - 1 ) / TPB ) );
     CMD_TRY( S->sc.dpart.reserve( 6 * (size_t)nbb ) );
     CMD_TRY( S->sc.dres.reserve( 8 ) );
+    // (0) the tiled layout of the last build, rebuilt without host waits
+    // before the build's own (tiles_build_spec): the bounding-box test runs
+    // on the device and is read back with the build's results; anything
+    // that does not hold sends the build down the path below
+    if ( N->allow_tiles && N->lanes >= 2 && S->bin_grid_valid && N->spec.valid )
+    {
+        const CellGrid &bg = S->bin_grid;
+        bool cells_ok = true;
+        double3 glo, ghi;
+        double lo[3], hi[3];
+        for ( int k = 0; k < 3; ++k )
+        {
+            lo[k] = bg.min[k];
+            hi[k] = bg.min[k] + bg.n[k] / bg.rd[k];
+            if ( !( 1.0 / bg.rd[k] >= N->neigh_cut ) )
+                cells_ok = false;
+        }
+        if ( cells_ok )
+        {
+            glo = make_double3( lo[0], lo[1], lo[2] );
+            ghi = make_double3( hi[0], hi[1], hi[2] );
+            CMD_TRY( S->sc.ires.reserve( TILE_RES_WORDS ) );
+            k_bbox<<<nbb, TPB, 0, st>>>( nt, S->x.p, S->sc.dpart.p );
+            CMD_LAUNCH_CHECK();
+            k_bbox_final<<<1, 64, 0, st>>>( nbb, S->sc.dpart.p, S->sc.dres.p, glo, ghi,
+                                            S->sc.ires.p + TILE_RES_INSIDE );
+            CMD_LAUNCH_CHECK();
+            if ( !S->xg_valid )
+                CMD_TRY( tiles_prepare_ghosts( S ) );
+            bool built = false;
+            CMD_TRY( tiles_build_spec( N, S, &built ) );
+            if ( built )
+            {
+                N->used_bin_grid = true;
+                N->nnz = -1;
+                return CMD_OK;
+            }
+        }
+    }
+    // (1) bounding box
     k_bbox<<<nbb, TPB, 0, st>>>( nt, S->x.p, S->sc.dpart.p );
     CMD_LAUNCH_CHECK();
-    k_bbox_final<<<1, 64, 0, st>>>( nbb, S->sc.dpart.p, S->sc.dres.p );
+    k_bbox_final<<<1, 64, 0, st>>>( nbb, S->sc.dpart.p, S->sc.dres.p, double3{}, double3{},
+                                    nullptr );
     CMD_LAUNCH_CHECK();
     double bb[6];
     CMD_HIP( hipMemcpyAsync( bb, S->sc.dres.p, sizeof( bb ),

@@ -142,7 +142,7 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
 // wave loads the stencil's candidates into registers, BCH chunks of 64 at a
 // time (lane = candidate, positions from LDS planes), and tests every atom
 // of the cell against them in turn, the atom's position and row length being
-// wave-uniform (scalar loads and registers); ballot + mbcnt place each
+// wave-uniform (read out of the lanes that hold them); ballot + mbcnt place each
 // accepted candidate in the atom's row, so rows come out in walk order.  The
 // block stages the 3 x 3 columns around its atom column (a subset of the
 // tile's staged columns, the same cell range) as coordinate planes; each
@@ -150,10 +150,12 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
 // staging, which is what the row stores.  The stencil is a list of
 // contiguous LDS ranges (per column: the locals of its three cells, or
 // locals and ghosts cell by cell where ghosts are present), concatenated so
-// that chunks run across range ends.  Rows are assembled in LDS one after
-// another (pre-filled with the sentinel), so an entry's address is the row
-// start plus its index, and are transposed into the slab layout on the way
-// out.
+// that chunks run across range ends; the range table sits in registers (lane
+// r = range r), and each pass of chunks applies the ranges that start inside
+// it in turn, so a lane ends with the range holding its candidate.  Rows are
+// assembled in LDS one after another (pre-filled with the sentinel), so an
+// entry's address is the row start plus its index, and are transposed into
+// the slab layout on the way out.
 static constexpr int MAXR = 9 * 3 * 2; // stencil ranges of one cell
 static constexpr int BCH = 10;         // candidate chunks of 64 held in registers
 
@@ -171,7 +173,8 @@ __host__ __device__ constexpr int tile_row_stride( int R4, int G )
     return R4 * 4 * G + 2;
 }
 
-// The test masks of one atom against NC register chunks of candidates.
+// The test masks of one atom against NC register chunks of candidates:
+// r = (dx^2 + dy^2) + dz^2 <= rc^2 in FP64, as the reference evaluates it.
 template <int NC, bool HALF>
 __device__ __forceinline__ void test_chunks( const double ( &px )[BCH], const double ( &py )[BCH],
                                              const double ( &pz )[BCH], const double4 pa, int vs,
@@ -201,63 +204,120 @@ __device__ __forceinline__ void test_chunks( const double ( &px )[BCH], const do
     }
 }
 
-// The accepted candidates of one atom's masks appended to its row in chunk
-// order; a lane that does not accept writes its own slot of a trash area
-// instead, so the stores need no branch.  Returns the new row length.
-template <int NC>
-__device__ __forceinline__ unsigned place_row( const int ( &po )[BCH],
-                                               const unsigned long long ( &m )[NC],
-                                               unsigned short *row, unsigned k0, int cap,
-                                               unsigned short *trash )
+// The same for two atoms (candidates vs and vs + 1), their operations
+// interleaved so that each dependent FP64 operation has the other atom's
+// independent one between it and its operand's producer.
+template <int NC, bool HALF>
+__device__ __forceinline__ void test_pair( const double ( &px )[BCH], const double ( &py )[BCH],
+                                           const double ( &pz )[BCH], const double4 pa,
+                                           const double4 pb, int vs, double rc2,
+                                           unsigned long long ( &ma )[NC],
+                                           unsigned long long ( &mb )[NC] )
 {
-    const int lane = threadIdx.x & 63;
+    const int csa = vs >> 6, csb = ( vs + 1 ) >> 6;
+    const unsigned long long nsa = ~( 1ull << ( vs & 63 ) );
+    const unsigned long long nsb = ~( 1ull << ( ( vs + 1 ) & 63 ) );
 #pragma unroll
     for ( int c = 0; c < NC; ++c )
     {
-        const unsigned k = __builtin_amdgcn_mbcnt_hi(
-            (unsigned)( m[c] >> 32 ), __builtin_amdgcn_mbcnt_lo( (unsigned)m[c], k0 ) );
-        const bool put =
-            __builtin_amdgcn_inverse_ballot_w64( m[c] & __builtin_amdgcn_ballot_w64( k < (unsigned)cap ) );
-        *( put ? row + k : trash + lane ) = (unsigned short)po[c];
+        const double dxa = pa.x - px[c], dxb = pb.x - px[c];
+        const double dya = pa.y - py[c], dyb = pb.y - py[c];
+        const double dza = pa.z - pz[c], dzb = pb.z - pz[c];
+        const double xa = dxa * dxa, xb = dxb * dxb;
+        const double ya = dya * dya, yb = dyb * dyb;
+        const double za = dza * dza, zb = dzb * dzb;
+        const double sa = xa + ya, sb = xb + yb;
+        const double ra = sa + za, rb = sb + zb;
+        ma[c] = __builtin_amdgcn_ballot_w64( ra <= rc2 ) & ( c == csa ? nsa : ~0ull );
+        mb[c] = __builtin_amdgcn_ballot_w64( rb <= rc2 ) & ( c == csb ? nsb : ~0ull );
+        if ( HALF )
+        {
+            const bool upa = ( px[c] > pa.x ) |
+                             ( ( px[c] == pa.x ) &
+                               ( ( py[c] > pa.y ) | ( ( py[c] == pa.y ) & ( pz[c] > pa.z ) ) ) );
+            const bool upb = ( px[c] > pb.x ) |
+                             ( ( px[c] == pb.x ) &
+                               ( ( py[c] > pb.y ) | ( ( py[c] == pb.y ) & ( pz[c] > pb.z ) ) ) );
+            ma[c] &= __builtin_amdgcn_ballot_w64( upa );
+            mb[c] &= __builtin_amdgcn_ballot_w64( upb );
+        }
+    }
+}
+
+// The accepted candidates of one atom's masks appended to its row in chunk
+// order.  A row holds cap entries and a spare slot at index cap: lanes that
+// do not accept, and accepted entries that would fall past the row's
+// capacity (the row length is still counted, and the list is rebuilt with
+// larger rows), store into the spare slot, so the stores need no branch.
+// Returns the new row length.
+template <int NC>
+__device__ __forceinline__ unsigned place_row( const int ( &po )[BCH],
+                                               const unsigned long long ( &m )[NC],
+                                               unsigned short *row, unsigned k0, unsigned cap )
+{
+#pragma unroll
+    for ( int c = 0; c < NC; ++c )
+    {
+        unsigned e = cap;
+        if ( __builtin_amdgcn_inverse_ballot_w64( m[c] ) )
+            e = min( __builtin_amdgcn_mbcnt_hi( (unsigned)( m[c] >> 32 ),
+                                                __builtin_amdgcn_mbcnt_lo( (unsigned)m[c], k0 ) ),
+                     cap );
+        row[e] = (unsigned short)po[c];
         k0 += (unsigned)__popcll( m[c] );
     }
     return k0;
 }
 
-// The atoms a0 .. a0 + na - 1 of a cell (candidates vs0 + l of this pass,
-// LDS rows row0 + l * RS) against NC register chunks of candidates, two
-// atoms at a time: both atoms' tests first (independent chains), then their
-// rows.  An atom's position and row length are wave-uniform (scalar loads
-// and registers).  Lane l of cnt holds atom a0 + l's row length.
+// Lane l's value of v, as a wave-uniform (scalar) value.
+__device__ __forceinline__ double readlane_f64( double v, int l )
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong( v );
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane( (int)(unsigned)b, l );
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane( (int)(unsigned)( b >> 32 ), l );
+    return __longlong_as_double( (long long)( ( (unsigned long long)hi << 32 ) | lo ) );
+}
+
+__device__ __forceinline__ double4 lane_pos( const double4 &a, int l )
+{
+    return make_double4( readlane_f64( a.x, l ), readlane_f64( a.y, l ),
+                         readlane_f64( a.z, l ), 0.0 );
+}
+
+// The atoms 0 .. na - 1 of a group of a cell's atoms (lane l of `at` holds
+// atom l's position; candidates vs0 + l of this pass, LDS rows row0 + l * RS)
+// against NC register chunks of candidates, two atoms at a time.  An atom's
+// position and row length are read out of their lanes as wave-uniform
+// values, so the loop waits on no memory.  Lane l of cnt holds atom l's row
+// length.
 template <int NC, int G, bool HALF>
 __device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const double ( &py )[BCH],
                                              const double ( &pz )[BCH], const int ( &po )[BCH],
-                                             const double4 *__restrict__ x, int a0, int na,
+                                             const double4 &at, int na,
                                              int vs0, unsigned short *row0, int RS, double rc2,
-                                             int cap, unsigned short *trash, int &cnt )
+                                             unsigned cap, int &cnt )
 {
-    const int lane = threadIdx.x & 63;
     int l = 0;
     for ( ; l + 1 < na; l += 2 )
     {
-        const double4 pa = x[a0 + l], pb = x[a0 + l + 1];
+        const double4 pa = lane_pos( at, l ), pb = lane_pos( at, l + 1 );
         const unsigned ka0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
         const unsigned kb0 = (unsigned)__builtin_amdgcn_readlane( cnt, l + 1 );
         unsigned long long ma[NC], mb[NC];
-        test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
-        test_chunks<NC, HALF>( px, py, pz, pb, vs0 + l + 1, rc2, mb );
-        const unsigned ka = place_row<NC>( po, ma, row0 + l * RS, ka0, cap, trash );
-        const unsigned kb = place_row<NC>( po, mb, row0 + ( l + 1 ) * RS, kb0, cap, trash );
+        test_pair<NC, HALF>( px, py, pz, pa, pb, vs0 + l, rc2, ma, mb );
+        const unsigned ka = place_row<NC>( po, ma, row0 + l * RS, ka0, cap );
+        const unsigned kb = place_row<NC>( po, mb, row0 + ( l + 1 ) * RS, kb0, cap );
+        const int lane = threadIdx.x & 63;
         cnt = lane == l ? (int)ka : ( lane == l + 1 ? (int)kb : cnt );
     }
     if ( l < na )
     {
-        const double4 pa = x[a0 + l];
+        const double4 pa = lane_pos( at, l );
         unsigned long long ma[NC];
         test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
-        const unsigned ka = place_row<NC>(
-            po, ma, row0 + l * RS, (unsigned)__builtin_amdgcn_readlane( cnt, l ), cap, trash );
-        if ( lane == l )
+        const unsigned ka = place_row<NC>( po, ma, row0 + l * RS,
+                                           (unsigned)__builtin_amdgcn_readlane( cnt, l ), cap );
+        if ( ( threadIdx.x & 63 ) == l )
             cnt = (int)ka;
     }
 }
@@ -270,8 +330,8 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     TileGrid T, const int *__restrict__ ls, const int *__restrict__ gs,
     const int *__restrict__ info, const int *__restrict__ woff,
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2,
-    int cap, int slot_cap, unsigned short *__restrict__ slab,
-    int *__restrict__ counts, int *__restrict__ max_row )
+    int cap, int slot_cap, int rows_cap, int waves_cap, unsigned short *__restrict__ slab,
+    int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over )
 {
     extern __shared__ double s_x[];
     __shared__ TileHead H;
@@ -311,19 +371,34 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         s_bseg[18] = off;
     }
     __syncthreads();
-    for ( int k = threadIdx.x; k < 9 * ( nst + 1 ); k += blockDim.x )
+    // the launch's sizes (planes, rows, slab) may come from an earlier
+    // build: a block whose needs exceed them writes nothing and flags the
+    // build, which is then redone with this build's sizes
+    if ( s_bseg[18] + 1 > slot_cap || nat_c > rows_cap || woff[t + 1] > waves_cap )
     {
-        const int qb = k / ( nst + 1 ), ce = k % ( nst + 1 );
-        const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
+        if ( threadIdx.x == 0 )
+            atomicOr( over, 1 );
+        return;
+    }
+    // cell starts of the 3 x 3 columns in build slots (one entry per
+    // thread: 9 (nst + 1) <= 9 (E + 3) <= 64 min(E, 8) threads); the
+    // loads are issued here and their results stored after the staging, so
+    // that both sets of loads are in flight together
+    const int kq = threadIdx.x;
+    const bool has_cs = kq < 9 * ( nst + 1 );
+    int cs_l = 0, cs_g = 0;
+    if ( has_cs )
+    {
+        const int qb = kq / ( nst + 1 ), ce = kq % ( nst + 1 );
         const int cl = col_base( T, a + qb / 3 - 1, b + qb % 3 - 1 ) + ea + ce;
-        s_lcs[qb][ce] = s_bseg[2 * qb] + ( ls[cl] - H.info[TI_LB + qf] );
-        s_gcs[qb][ce] = s_bseg[2 * qb + 1] + ( gs[cl] - H.info[TI_GB + qf] );
+        cs_l = ls[cl];
+        cs_g = gs[cl];
     }
     // coordinate planes of the 3 x 3 columns' runs, then the sentinel
     double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
     const int btotal = s_bseg[18];
     {
-        constexpr int B = 4;
+        constexpr int B = 8;
         int seg = 0;
         for ( int k0 = threadIdx.x; k0 < btotal; k0 += B * blockDim.x )
         {
@@ -347,6 +422,13 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                 s_z[k] = p[u].z;
             }
         }
+        if ( has_cs )
+        {
+            const int qb = kq / ( nst + 1 ), ce = kq % ( nst + 1 );
+            const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
+            s_lcs[qb][ce] = s_bseg[2 * qb] + ( cs_l - H.info[TI_LB + qf] );
+            s_gcs[qb][ce] = s_bseg[2 * qb + 1] + ( cs_g - H.info[TI_GB + qf] );
+        }
         __syncthreads();
         if ( threadIdx.x == 0 )
         {
@@ -355,11 +437,10 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
             s_z[btotal] = TILE_FAR;
         }
     }
-    // after the planes (rounded to 16 bytes): a 64-entry trash area, then
-    // the rows of this atom column's atoms
-    unsigned short *trash = reinterpret_cast<unsigned short *>(
+    // after the planes (rounded to 16 bytes): the rows of this atom column's
+    // atoms
+    unsigned short *chunk = reinterpret_cast<unsigned short *>(
         reinterpret_cast<char *>( s_x ) + tile_planes_bytes( slot_cap ) );
-    unsigned short *chunk = trash + 64;
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     const int wsz = R4 * 256; // slab entries of one slab wave
     // the stride is an odd number of words, which spreads the copy-out's
@@ -387,132 +468,197 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         const int ce = e - ea;
         // stencil cells of this cell inside the staged range
         const int elo = max( ce - 1, 0 ), ehi = min( ce + 1, nst - 1 );
-        // column q's ranges (lane q < 9), then their offsets in the list
-        int nr_q = 0;
+        // the stencil's ranges, lane r = range r: its first candidate index
+        // (t_p) and the offsets that turn a candidate index into its build
+        // slot (t_dj) and into its force record's byte offset (t_df).  Where
+        // none of the 9 columns holds ghosts in the stencil's cells, column q
+        // is the single range q (its locals of the three cells); otherwise
+        // the ranges are laid out through LDS (per column the locals of its
+        // three cells, or locals and ghosts cell by cell).
         bool gh = false;
         if ( lane < 9 )
-        {
             gh = s_gcs[lane][elo] != s_gcs[lane][ehi + 1];
-            nr_q = gh ? 2 * ( ehi - elo + 1 ) : 1;
-        }
-        int off = nr_q;
-#pragma unroll
-        for ( int o = 1; o < 16; o <<= 1 )
+        int nr, total, vbase;
+        int t_p = 0, t_dj = 0, t_df = 0;
+        if ( __builtin_amdgcn_ballot_w64( gh ) == 0 )
         {
-            const int u = __shfl_up( off, o, 64 );
-            if ( lane >= o )
-                off += u;
-        }
-        const int nr = __builtin_amdgcn_readfirstlane( __shfl( off, 8, 64 ) );
-        off -= nr_q;
-        const int off0 = off;
-        if ( lane < 9 )
-        {
-            if ( !gh )
+            int len = 0, s0 = 0;
+            if ( lane < 9 )
             {
-                rs[off] = s_lcs[lane][elo];
-                rf[off] = s_lcs[lane][elo] + s_dl[lane];
-                rp[off + 1] = s_lcs[lane][ehi + 1] - s_lcs[lane][elo];
+                s0 = s_lcs[lane][elo];
+                len = s_lcs[lane][ehi + 1] - s0;
             }
-            else
+            int inc = len;
+#pragma unroll
+            for ( int o = 1; o < 16; o <<= 1 )
             {
-                for ( int c2 = elo; c2 <= ehi; ++c2 )
+                const int u = __shfl_up( inc, o, 64 );
+                if ( lane >= o )
+                    inc += u;
+            }
+            nr = 9;
+            total = __builtin_amdgcn_readlane( inc, 8 );
+            if ( lane < 9 )
+            {
+                t_p = inc - len;
+                t_dj = s0 - t_p;
+                t_df = ( s0 + s_dl[lane] - t_p ) * TILE_REC;
+            }
+            // candidate index of the cell's first atom: in the centre
+            // column's (range 4's) locals, cell ce
+            int vb = 0;
+            if ( lane == 4 )
+                vb = t_p + ( s_lcs[4][ce] - s0 );
+            vbase = __builtin_amdgcn_readlane( vb, 4 );
+        }
+        else
+        {
+            // column q's ranges (lane q < 9), then their offsets in the list
+            const int nr_q = lane < 9 ? ( gh ? 2 * ( ehi - elo + 1 ) : 1 ) : 0;
+            int off = nr_q;
+#pragma unroll
+            for ( int o = 1; o < 16; o <<= 1 )
+            {
+                const int u = __shfl_up( off, o, 64 );
+                if ( lane >= o )
+                    off += u;
+            }
+            nr = __builtin_amdgcn_readlane( off, 8 );
+            off -= nr_q;
+            const int off0 = off;
+            if ( lane < 9 )
+            {
+                if ( !gh )
                 {
-                    rs[off] = s_lcs[lane][c2];
-                    rf[off] = s_lcs[lane][c2] + s_dl[lane];
-                    rp[off + 1] = s_lcs[lane][c2 + 1] - s_lcs[lane][c2];
-                    rs[off + 1] = s_gcs[lane][c2];
-                    rf[off + 1] = s_gcs[lane][c2] + s_dg[lane];
-                    rp[off + 2] = s_gcs[lane][c2 + 1] - s_gcs[lane][c2];
-                    off += 2;
+                    rs[off] = s_lcs[lane][elo];
+                    rf[off] = s_lcs[lane][elo] + s_dl[lane];
+                    rp[off + 1] = s_lcs[lane][ehi + 1] - s_lcs[lane][elo];
+                }
+                else
+                {
+                    for ( int c2 = elo; c2 <= ehi; ++c2 )
+                    {
+                        rs[off] = s_lcs[lane][c2];
+                        rf[off] = s_lcs[lane][c2] + s_dl[lane];
+                        rp[off + 1] = s_lcs[lane][c2 + 1] - s_lcs[lane][c2];
+                        rs[off + 1] = s_gcs[lane][c2];
+                        rf[off + 1] = s_gcs[lane][c2] + s_dg[lane];
+                        rp[off + 2] = s_gcs[lane][c2 + 1] - s_gcs[lane][c2];
+                        off += 2;
+                    }
                 }
             }
-        }
-        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-        // prefix of the range lengths (nr <= MAXR < 64: one range per lane)
-        int len = lane < nr ? rp[lane + 1] : 0;
-        int inc = len;
+            __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+            // prefix of the range lengths (nr <= MAXR < 64: one range per lane)
+            const int len = lane < nr ? rp[lane + 1] : 0;
+            int inc = len;
 #pragma unroll
-        for ( int o = 1; o < 64; o <<= 1 )
-        {
-            const int u = __shfl_up( inc, o, 64 );
-            if ( lane >= o )
-                inc += u;
+            for ( int o = 1; o < 64; o <<= 1 )
+            {
+                const int u = __shfl_up( inc, o, 64 );
+                if ( lane >= o )
+                    inc += u;
+            }
+            total = __builtin_amdgcn_readfirstlane( __shfl( inc, nr - 1, 64 ) );
+            if ( lane < nr )
+            {
+                t_p = inc - len;
+                t_dj = rs[lane] - t_p;
+                t_df = ( rf[lane] - t_p ) * TILE_REC;
+            }
+            // candidate index of the cell's first atom, in the centre
+            // column's (lane 4's) ranges: the locals of cell ce, its range
+            // 2 (ce - elo) where the column holds ghosts, else at offset
+            // ce - elo cells into its single range
+            int vb = 0, vo = 0;
+            if ( lane == 4 )
+            {
+                vb = gh ? off0 + 2 * ( ce - elo ) : off0;
+                vo = gh ? 0 : s_lcs[4][ce] - s_lcs[4][elo];
+            }
+            const int rc = __builtin_amdgcn_readlane( vb, 4 );
+            vbase = __builtin_amdgcn_readlane( t_p, rc ) + __builtin_amdgcn_readlane( vo, 4 );
+            __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
         }
-        if ( lane < nr )
-            rp[lane + 1] = inc;
-        if ( lane == 0 )
-            rp[0] = 0;
-        const int total = __builtin_amdgcn_readfirstlane( __shfl( inc, nr - 1, 64 ) );
-        __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-        // candidate index of the cell's first atom: its slot lies in the
-        // centre column's (lane 4's) local range of cell ce
-        int vb = 0;
-        if ( lane == 4 )
-            vb = gh ? rp[off0 + 2 * ( ce - elo )]
-                    : rp[off0] + ( s_lcs[4][ce] - s_lcs[4][elo] );
-        const int vbase = __builtin_amdgcn_readlane( vb, 4 );
-        // the cell's atoms, up to 64 at a time (lane l holds atom a0 + l)
-        const int cc = col_base( T, a, b ) + e;
-        const int c0 = __builtin_amdgcn_readfirstlane( ls[cc] );
-        const int c1 = __builtin_amdgcn_readfirstlane( ls[cc + 1] );
+        // the cell's atoms, up to 64 at a time (lane l holds atom a0 + l),
+        // from the centre column's planes: build slot = atom index - dl4
+        const int dl4 = H.info[TI_LB + ( ca + 1 ) * S + cb + 1] - s_bseg[8];
+        const int c0 = s_lcs[4][ce] + dl4, c1 = s_lcs[4][ce + 1] + dl4;
         for ( int i0 = c0; i0 < c1; i0 += 64 )
         {
             const int na = min( 64, c1 - i0 );
+            const int sa = i0 + min( lane, na - 1 ) - dl4;
+            const double4 at = make_double4( s_x[sa], s_y[sa], s_z[sa], 0.0 ); // lane l: atom i0 + l
             int cnt = 0; // row length of atom i0 + lane
+            int cur = 0; // range holding the pass's first candidate
             for ( int v0 = 0; v0 < total; v0 += BCH * 64 )
             {
                 // BCH chunks of candidates in registers (lane = candidate);
-                // past the end: the sentinel position, which no test accepts
+                // past the end: the sentinel position, which no test accepts.
+                // A candidate belongs to the last range that starts at or
+                // before it (ranges in order; an empty range is passed over
+                // by the next one, which starts at the same index): the
+                // ranges starting inside the pass are applied in turn.
+                while ( cur + 1 < nr && __builtin_amdgcn_readlane( t_p, cur + 1 ) <= v0 )
+                    ++cur;
+                int dj[BCH], df[BCH];
+                {
+                    const int d = __builtin_amdgcn_readlane( t_dj, cur );
+                    const int f = __builtin_amdgcn_readlane( t_df, cur );
+#pragma unroll
+                    for ( int ch = 0; ch < BCH; ++ch )
+                    {
+                        dj[ch] = d;
+                        df[ch] = f;
+                    }
+                }
+                for ( int r = cur + 1; r < nr; ++r )
+                {
+                    const int pr = __builtin_amdgcn_readlane( t_p, r );
+                    if ( pr >= v0 + BCH * 64 )
+                        break;
+                    const int d = __builtin_amdgcn_readlane( t_dj, r );
+                    const int f = __builtin_amdgcn_readlane( t_df, r );
+#pragma unroll
+                    for ( int ch = 0; ch < BCH; ++ch )
+                    {
+                        const bool in = v0 + ch * 64 + lane >= pr;
+                        dj[ch] = in ? d : dj[ch];
+                        df[ch] = in ? f : df[ch];
+                    }
+                }
                 double px[BCH], py[BCH], pz[BCH];
                 int po[BCH]; // the force tile's record byte offset, the value a row stores
 #pragma unroll
                 for ( int ch = 0; ch < BCH; ++ch )
                 {
                     const int v = v0 + ch * 64 + lane;
-                    int sj = bsent, sf = fsent;
-                    if ( v0 + ch * 64 < total )
-                    {
-                        // range holding candidate v: binary search over the
-                        // prefix of range lengths
-                        int lo = 0, hi = nr;
-                        while ( hi - lo > 1 )
-                        {
-                            const int mid = ( lo + hi ) >> 1;
-                            if ( rp[mid] <= v )
-                                lo = mid;
-                            else
-                                hi = mid;
-                        }
-                        if ( v < total )
-                        {
-                            sj = rs[lo] + ( v - rp[lo] );
-                            sf = rf[lo] + ( v - rp[lo] );
-                        }
-                    }
+                    const bool ok = v < total;
+                    const int sj = ok ? v + dj[ch] : bsent;
+                    po[ch] = ok ? v * TILE_REC + df[ch] : fsent * TILE_REC;
                     px[ch] = s_x[sj];
                     py[ch] = s_y[sj];
                     pz[ch] = s_z[sj];
-                    po[ch] = sf * TILE_REC;
                 }
                 const int nch = min( BCH, ( total - v0 + 63 ) >> 6 );
                 const int vs0 = vbase + ( i0 - c0 ) - v0;
                 unsigned short *row0 = chunk + ( i0 - al ) * RS;
+                const unsigned ucap = (unsigned)cap;
                 if ( nch <= 4 )
-                    build_atoms<4, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
-                                             cap, trash, cnt );
+                    build_atoms<4, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
+                                             ucap, cnt );
                 else if ( nch <= 6 )
-                    build_atoms<6, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
-                                             cap, trash, cnt );
+                    build_atoms<6, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
+                                             ucap, cnt );
                 else if ( nch <= 8 )
-                    build_atoms<8, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
-                                             cap, trash, cnt );
+                    build_atoms<8, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
+                                             ucap, cnt );
                 else if ( nch == 9 )
-                    build_atoms<9, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS, rc2,
-                                             cap, trash, cnt );
+                    build_atoms<9, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
+                                             ucap, cnt );
                 else
-                    build_atoms<BCH, G, HALF>( px, py, pz, po, x, i0, na, vs0, row0, RS,
-                                               rc2, cap, trash, cnt );
+                    build_atoms<BCH, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS,
+                                               rc2, ucap, cnt );
             }
             if ( lane < na )
             {
@@ -533,26 +679,30 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         // or past cap are the sentinel.
         const int nat = H.info[TI_NAT], ta0 = H.aoff[c];
         const int w0 = ta0 / APW, w1 = ( ta0 + nat_c - 1 ) / APW;
-        const int nwords = ( w1 - w0 + 1 ) * R4 * 64;
         const unsigned short sent = (unsigned short)( fsent * TILE_REC );
-        uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)( woff[t] + w0 ) * wsz );
-        for ( int wd = threadIdx.x; wd < nwords; wd += blockDim.x )
+        for ( int sw = w0; sw <= w1; ++sw )
         {
-            const int sw = w0 + wd / ( R4 * 64 ), rem = wd % ( R4 * 64 );
-            const int g4 = rem >> 6, L = rem & 63;
-            const int gw = wave_lanes( G, nat, sw ), apw = 64 / gw;
-            const int ta = sw * APW + L % apw;
-            if ( L % apw >= APW || ta < ta0 || ta >= ta0 + nat_c )
+            // lanes per atom of slab wave sw (a power of two): lane L of a
+            // word group holds the atom L % apw, entry column L / apw
+            const int lg = __builtin_ctz( (unsigned)wave_lanes( G, nat, sw ) );
+            const int sh = 6 - lg; // apw = 1 << sh
+            const int aw = lane & ( ( 1 << sh ) - 1 ), sub = lane >> sh;
+            const int ta = sw * APW + aw;
+            if ( aw >= APW || ta < ta0 || ta >= ta0 + nat_c )
                 continue;
             const unsigned short *row = chunk + ( ta - ta0 ) * RS;
-            unsigned e4[4];
-#pragma unroll
-            for ( int j = 0; j < 4; ++j )
+            uint2 *dst = reinterpret_cast<uint2 *>( slab + (size_t)( woff[t] + sw ) * wsz );
+            for ( int g4 = wv; g4 < R4; g4 += nwv )
             {
-                const int en = ( 4 * g4 + j ) * gw + L / apw;
-                e4[j] = en < cap ? row[en] : sent;
+                unsigned e4[4];
+#pragma unroll
+                for ( int j = 0; j < 4; ++j )
+                {
+                    const int en = ( ( 4 * g4 + j ) << lg ) + sub;
+                    e4[j] = en < cap ? row[en] : sent;
+                }
+                dst[g4 * 64 + lane] = make_uint2( e4[0] | ( e4[1] << 16 ), e4[2] | ( e4[3] << 16 ) );
             }
-            dst[wd] = make_uint2( e4[0] | ( e4[1] << 16 ), e4[2] | ( e4[3] << 16 ) );
         }
     }
 #pragma unroll
@@ -719,12 +869,13 @@ cmd_status tiles_ghost_refresh( cmd_system *S )
     return CMD_OK;
 }
 
-// Tile info for width W and height E.  h[0] largest staged count of a
-// tile, h[1] local atoms the tiles cover, h[2] most slab waves of a tile,
-// h[3] largest staged count of one atom column's 3 x 3 columns, h[4] most
-// atoms of one atom column, h[5] slab waves of all tiles (the scan's total);
-// read back in one wait.
-static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int W, int E, int h[6] )
+// Tile info for width W and height E, launched without waiting: the tile
+// table, slab waves per tile and their scan (total at start[ntiles]), and
+// the maxima in the result words: [0] largest staged count of a tile, [1]
+// local atoms the tiles cover, [2] most slab waves of a tile, [3] largest
+// staged count of one atom column's 3 x 3 columns, [4] most atoms of one
+// atom column.
+static cmd_status tile_info_launch( cmd_neigh *N, cmd_system *S, int W, int E )
 {
     TileGrid &T = N->tg;
     T.g = S->bin_grid;
@@ -739,27 +890,38 @@ static cmd_status tile_info_for( cmd_neigh *N, cmd_system *S, int W, int E, int 
     CMD_REQUIRE( T.nc0 > 0 && T.nc1 > 0 && T.tpc > 0, CMD_ERR_ARG,
                  "binning grid has no interior cells" );
     CMD_TRY( N->tile_info.reserve( (size_t)T.ntiles * TI_WORDS ) );
-    CMD_TRY( S->sc.ires.reserve( 8 ) );
+    CMD_TRY( S->sc.ires.reserve( TILE_RES_WORDS ) );
     DevBuf<int> &waves = N->grid_sort.cursor; // scratch: waves per tile
     CMD_TRY( waves.reserve( T.ntiles + 1 ) );
     CMD_TRY( N->grid_sort.start.reserve( T.ntiles + 1 ) ); // scan of waves
-    CMD_TRY( fill_i32( S->sc.ires.p, 0, 5, S->stream ) );
+    CMD_TRY( fill_i32( S->sc.ires.p + TILE_RES_INFO, 0, 5, S->stream ) );
     const int apw = 64 / N->lanes;
     k_tile_info<<<( T.ntiles + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
         T, S->bins.start.p, S->gsort.start.p, apw, N->tile_info.p, waves.p,
-        S->sc.ires.p );
+        S->sc.ires.p + TILE_RES_INFO );
     CMD_LAUNCH_CHECK();
     CMD_TRY( scan_exclusive( waves.p, N->grid_sort.start.p, T.ntiles, S->sc,
                              S->stream, nullptr ) );
-    CMD_HIP( hipMemcpyAsync( h, S->sc.ires.p, 5 * sizeof( int ),
+    return CMD_OK;
+}
+
+// Reads the result words (res, TILE_RES_WORDS of them) and the slab waves
+// of all tiles (h5) in one wait.
+static cmd_status tile_results( cmd_neigh *N, cmd_system *S, int *res, int *h5 )
+{
+    CMD_HIP( hipMemcpyAsync( res, S->sc.ires.p, TILE_RES_WORDS * sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
-    CMD_HIP( hipMemcpyAsync( h + 5, N->grid_sort.start.p + T.ntiles, sizeof( int ),
+    CMD_HIP( hipMemcpyAsync( h5, N->grid_sort.start.p + N->tg.ntiles, sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
-    // waves per block of the force kernels: as tuned, else one per slab
-    // wave of the largest tile, at most 8 (a tile with more loops)
-    T.bw = N->tile_bw > 0 ? N->tile_bw : std::min( std::max( h[2], 1 ), 8 );
     return CMD_OK;
+}
+
+// Waves per block of the force kernels: as tuned, else one per slab wave of
+// the largest tile, at most 8 (a tile with more loops).
+static int force_waves( const cmd_neigh *N, int most_waves )
+{
+    return N->tile_bw > 0 ? N->tile_bw : std::min( std::max( most_waves, 1 ), 8 );
 }
 
 static size_t tile_slab_entries( const cmd_neigh *N, int nw )
@@ -796,31 +958,106 @@ static size_t force_lds( int slots, bool half )
 }
 
 template <int G>
-static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, size_t lds,
-                          int *max_row )
+static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_cap,
+                          int waves_cap, size_t lds )
 {
     const double rc2 = N->neigh_cut * N->neigh_cut;
+    int *res = S->sc.ires.p;
     // one block per atom column of a tile, one wave per cell of the column
     const int threads = 64 * std::min( std::max( N->tg.E, 1 ), BTPB / 64 );
     const int nb = N->tg.ntiles * N->tg.W * N->tg.W;
     if ( N->half )
         k_tile_build<G, true><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
-            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
-            N->tslab.p, N->counts.p, max_row );
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap, rows_cap,
+            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
     else
         k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
-            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap,
-            N->tslab.p, N->counts.p, max_row );
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap, rows_cap,
+            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
+}
+
+static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_cap,
+                                  int waves_cap, size_t lds )
+{
+    // the build's result words start at zero (the device test of the
+    // bounding box, when used, is left as it was written)
+    CMD_TRY( fill_i32( S->sc.ires.p + TILE_RES_MAXROW, 0, 2, S->stream ) );
+    switch ( N->lanes )
+    {
+    case 2:
+        launch_build<2>( N, S, slot_cap, rows_cap, waves_cap, lds );
+        break;
+    case 4:
+        launch_build<4>( N, S, slot_cap, rows_cap, waves_cap, lds );
+        break;
+    default:
+        launch_build<8>( N, S, slot_cap, rows_cap, waves_cap, lds );
+        break;
+    }
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
+// LDS of the build block: coordinate planes of its 3 x 3 columns and the
+// sentinel slot, then the rows of its atom column (each with a spare slot
+// past its cap entries).
+static size_t build_lds( const cmd_neigh *N, int slot_cap, int rows )
+{
+    const int G = N->lanes;
+    const int R4 = ( ( N->cap + G - 1 ) / G + 3 ) / 4;
+    return tile_planes_bytes( slot_cap ) +
+           ( (size_t)std::max( rows, 1 ) * tile_row_stride( R4, G ) + 8 ) * sizeof( unsigned short );
+}
+
+// Tile height asked for: the set one, else 3 cells for 2 x 2 columns and 4
+// for single columns.
+static int requested_height( const cmd_neigh *N, int W )
+{
+    return std::min( N->tile_e > 0 ? N->tile_e : ( W == 2 ? 3 : 4 ), EMAX - 2 );
+}
+
+// Whether a layout with tile maxima h and build LDS `lds` fits the device
+// (fits) and keeps the force kernel's blocks per CU (blocks) at the tuned
+// level; returns both tests together.
+static bool layout_ok( const cmd_neigh *N, const int h[6], size_t lds, size_t hard,
+                       size_t lds_cu, int E, bool *fits, int *blocks )
+{
+    const size_t flds = force_lds( h[0], N->half != 0 );
+    *fits = h[0] <= MAX_SLOTS && lds <= hard && flds <= hard;
+    // blocks per CU of the force kernel (its static tables included)
+    *blocks = (int)std::min( lds_cu / ( flds + 1024 ), (size_t)( 32 / N->tg.bw ) );
+    return *fits && ( N->occ_blocks == 0 || *blocks >= N->occ_blocks || E == 1 );
+}
+
+// Sizes for the next build's launch: this build's, with headroom for the
+// atoms that move between columns meanwhile.
+static void record_spec( cmd_neigh *N, int W, int E, const int h[6], size_t hard )
+{
+    cmd_neigh::Spec &P = N->spec;
+    P.valid = false;
+    if ( E != requested_height( N, W ) )
+        return; // a lowered height is chosen afresh at each build
+    P.G = N->lanes;
+    P.half = N->half;
+    P.W = W;
+    P.E = E;
+    P.bw_req = N->tile_bw;
+    P.cap = N->cap;
+    P.slot_cap = std::min( h[3] + 1 + std::max( 16, h[3] / 32 ), MAX_SLOTS + 1 );
+    P.rows = h[4] + std::max( 2, h[4] / 20 );
+    P.slab_waves = h[5] + std::max( 64, h[5] / 50 );
+    P.lds = build_lds( N, P.slot_cap, P.rows );
+    P.valid = P.lds <= hard;
 }
 
 cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
 {
     *built = false;
+    N->spec.valid = false;
     if ( !S->bin_grid_valid || !S->xg_valid || N->lanes < 2 )
         return CMD_OK;
-    const int G = N->lanes;
     if ( N->cap < 1 )
         N->cap = std::max( 1, N->max_neigh_guess );
     size_t hard = 0, lds_cu = 0;
@@ -832,31 +1069,27 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
     // the tuned one (the perfect lattice at step 0) would otherwise lose a
     // block of occupancy
     const int W = N->tile_w > 0 ? N->tile_w : 2;
-    int E = std::min( N->tile_e > 0 ? N->tile_e : ( W == 2 ? 3 : 4 ), EMAX - 2 );
+    int E = requested_height( N, W );
     int h[6] = { 0, 0, 0, 0, 0, 0 };
+    int res[TILE_RES_WORDS];
     for ( int attempt = 0; attempt < 4; ++attempt )
     {
         size_t lds = 0;
         for ( ;; )
         {
-            CMD_TRY( tile_info_for( N, S, W, E, h ) );
+            CMD_TRY( tile_info_launch( N, S, W, E ) );
+            CMD_TRY( tile_results( N, S, res, &h[5] ) );
+            for ( int k = 0; k < 5; ++k )
+                h[k] = res[TILE_RES_INFO + k];
+            N->tg.bw = force_waves( N, h[2] );
             // local atoms in the grid's padding cells (not wrapped into the
             // box) belong to no tile: use the global layout
             if ( h[1] != S->n_local )
                 return CMD_OK;
-            const int R4 = ( ( N->cap + G - 1 ) / G + 3 ) / 4;
-            // the build block: coordinate planes of its 3 x 3 columns and
-            // the sentinel slot, the trash area of rejected entries, then
-            // the rows of its atom column
-            lds = tile_planes_bytes( h[3] + 1 ) + 64 * sizeof( unsigned short ) +
-                  ( (size_t)std::max( h[4], 1 ) * tile_row_stride( R4, G ) + 8 ) *
-                      sizeof( unsigned short );
-            const size_t flds = force_lds( h[0], N->half != 0 );
-            const bool fits = h[0] <= MAX_SLOTS && lds <= hard && flds <= hard;
-            // blocks per CU of the force kernel (its static tables included)
-            const int blocks = (int)std::min( lds_cu / ( flds + 1024 ),
-                                              (size_t)( 32 / N->tg.bw ) );
-            if ( fits && ( N->occ_blocks == 0 || blocks >= N->occ_blocks || E == 1 ) )
+            lds = build_lds( N, h[3] + 1, h[4] );
+            bool fits = false;
+            int blocks = 0;
+            if ( layout_ok( N, h, lds, hard, lds_cu, E, &fits, &blocks ) )
             {
                 N->tile_blocks = blocks;
                 break;
@@ -866,26 +1099,11 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             E = fits ? E - 1 : std::max( 1, E * 2 / 3 );
         }
         N->max_tile_slots = h[0];
-        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[5] ) ) );
-        CMD_TRY( fill_i32( S->sc.ires.p, 0, 1, S->stream ) );
-        const int slot_cap = h[3] + 1;
-        switch ( G )
-        {
-        case 2:
-            launch_build<2>( N, S, slot_cap, lds, S->sc.ires.p );
-            break;
-        case 4:
-            launch_build<4>( N, S, slot_cap, lds, S->sc.ires.p );
-            break;
-        default:
-            launch_build<8>( N, S, slot_cap, lds, S->sc.ires.p );
-            break;
-        }
-        CMD_LAUNCH_CHECK();
-        int mr = 0;
-        CMD_HIP( hipMemcpyAsync( &mr, S->sc.ires.p, sizeof( int ),
-                                 hipMemcpyDeviceToHost, S->stream ) );
-        CMD_HIP( hipStreamSynchronize( S->stream ) );
+        // the slab sized for the next build's launch too (record_spec)
+        CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[5] + std::max( 64, h[5] / 50 ) ) ) );
+        CMD_TRY( launch_build_g( N, S, h[3] + 1, h[4], h[5], lds ) );
+        CMD_TRY( tile_results( N, S, res, &h[5] ) );
+        const int mr = res[TILE_RES_MAXROW];
         N->max_row = mr;
         if ( mr > N->max_neigh_guess )
             N->max_neigh_guess = (int)( mr * 1.1 );
@@ -894,6 +1112,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             *built = true;
             N->tiled = true;
             N->gperm_view = S->gsort.perm.p;
+            record_spec( N, W, E, h, hard );
             return CMD_OK;
         }
         // rows overflowed: larger rows, and the LDS fit rechecked
@@ -901,6 +1120,54 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
     }
     set_error( "neighbour rows kept growing during the tiled build" );
     return CMD_ERR_CAPACITY;
+}
+
+// The tiled build launched with the sizes of the last one (record_spec) and
+// read back once, after the build: tile maxima, longest row, the capacity
+// flag of the build blocks and the device's bounding-box test (written by
+// the caller into TILE_RES_INSIDE).  Built only when every test passes and
+// the layout is the one tiles_build would choose from these maxima;
+// otherwise *built stays false and the caller builds afresh.
+cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
+{
+    *built = false;
+    const cmd_neigh::Spec P = N->spec;
+    N->spec.valid = false;
+    if ( !P.valid || !S->bin_grid_valid || !S->xg_valid || P.G != N->lanes ||
+         P.half != N->half || P.cap != N->cap || P.bw_req != N->tile_bw )
+        return CMD_OK;
+    const int W = N->tile_w > 0 ? N->tile_w : 2;
+    if ( P.W != W || P.E != requested_height( N, W ) )
+        return CMD_OK;
+    CMD_TRY( tile_info_launch( N, S, P.W, P.E ) );
+    CMD_TRY( N->tslab.reserve( tile_slab_entries( N, P.slab_waves ) ) );
+    CMD_TRY( launch_build_g( N, S, P.slot_cap, P.rows, P.slab_waves, P.lds ) );
+    int res[TILE_RES_WORDS], h[6] = { 0, 0, 0, 0, 0, 0 };
+    CMD_TRY( tile_results( N, S, res, &h[5] ) );
+    for ( int k = 0; k < 5; ++k )
+        h[k] = res[TILE_RES_INFO + k];
+    if ( res[TILE_RES_INSIDE] != 1 || res[TILE_RES_OVER] != 0 || h[1] != S->n_local )
+        return CMD_OK;
+    size_t hard = 0, lds_cu = 0;
+    lds_limits( &hard, &lds_cu );
+    N->tg.bw = force_waves( N, h[2] );
+    bool fits = false;
+    int blocks = 0;
+    if ( !layout_ok( N, h, build_lds( N, h[3] + 1, h[4] ), hard, lds_cu, P.E, &fits, &blocks ) )
+        return CMD_OK;
+    const int mr = res[TILE_RES_MAXROW];
+    N->max_row = mr;
+    if ( mr > N->max_neigh_guess )
+        N->max_neigh_guess = (int)( mr * 1.1 );
+    if ( mr > N->cap )
+        return CMD_OK; // rows overflowed: the caller's build regrows them
+    N->tile_blocks = blocks;
+    N->max_tile_slots = h[0];
+    N->tiled = true;
+    N->gperm_view = S->gsort.perm.p;
+    *built = true;
+    record_spec( N, P.W, P.E, h, hard );
+    return CMD_OK;
 }
 
 cmd_status tiles_csr( cmd_neigh *N, int64_t *nnz_out )
