@@ -455,7 +455,7 @@ __device__ void block_partials4( double a, double b, double c, double d,
 __global__ void k_final_initial( int n, int cap, const double4 *__restrict__ x,
                                  double4 *__restrict__ x_next,
                                  double *__restrict__ v,
-                                 const double *__restrict__ f, MassTable mt,
+                                 double *__restrict__ f, MassTable mt,
                                  StepCtl ctl, double *__restrict__ partial )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -466,6 +466,12 @@ __global__ void k_final_initial( int n, int cap, const double4 *__restrict__ x,
         const double m = mt.m[(int)p.w];
         const double dtfm = ctl.dtf / m;
         const double fx = f[i], fy = f[cap + i], fz = f[2 * cap + i];
+        if ( ctl.zero_f )
+        {
+            f[i] = 0.0;
+            f[cap + i] = 0.0;
+            f[2 * cap + i] = 0.0;
+        }
         double vx = v[i] + dtfm * fx;
         double vy = v[cap + i] + dtfm * fy;
         double vz = v[2 * cap + i] + dtfm * fz;
@@ -495,6 +501,29 @@ __global__ void k_final_initial( int n, int cap, const double4 *__restrict__ x,
                     p.z = cz + ctl.L[2];
             }
             x_next[i] = p;
+            // the next step's ghost refresh (update_halo,
+            // comm_mpi_impl.h:369-408): the drifted position, shifted per
+            // image, at the image's sorted ghost slot
+            if ( ctl.xg_next )
+                for ( int m = ctl.img_start[i]; m < ctl.img_start[i + 1]; ++m )
+                {
+                    const unsigned e = (unsigned)ctl.img_pk[m];
+                    const unsigned code = e >> IMG_CODE_SHIFT;
+                    double4 g = p;
+                    if ( code & 1 )
+                        g.x = g.x - ctl.L[0];
+                    if ( code & 2 )
+                        g.x = g.x + ctl.L[0];
+                    if ( code & 4 )
+                        g.y = g.y - ctl.L[1];
+                    if ( code & 8 )
+                        g.y = g.y + ctl.L[1];
+                    if ( code & 16 )
+                        g.z = g.z - ctl.L[2];
+                    if ( code & 32 )
+                        g.z = g.z + ctl.L[2];
+                    ctl.xg_next[e & IMG_SLOT_MASK] = g;
+                }
         }
         v[i] = vx;
         v[cap + i] = vy;

@@ -350,6 +350,8 @@ struct StepCtl
     int wrap_next;        // wrap the drifted positions (next step rebuilds)
     int thermo;           // accumulate KE, PE and virial partials
     int write_f;          // store f
+    int zero_f;           // (integrator) clear f after reading it: the next
+                          // step's forces accumulate into it
     // when set (with do_next_initial and no wrap), the drifted position of
     // every local atom is also written, shifted per its images' codes, to
     // xg_next at its sorted ghost slots: the next step's ghost refresh

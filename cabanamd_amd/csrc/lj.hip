@@ -684,16 +684,17 @@ __host__ __device__ constexpr size_t half_acc_offset( int slot_cap, bool one_typ
 // atomics); the atom's own total joins its slot's record.  When every row
 // of the tile is done, each slot with a nonzero record adds it to f of its
 // atom with global atomics (an atom is staged by several tiles); ghost
-// slots go to their ghost rows, which update_force folds.  Energy weights
-// follow compute_energy_half: 1 for a local j, 1/2 for a ghost j.
+// slots go to the local atoms they image, so update_force's fold has
+// nothing left to add.  Energy weights follow compute_energy_half: 1 for a
+// local j, 1/2 for a ghost j.
 template <int G, int PREC, bool ONE_TYPE, bool ENERGY>
 __global__ __launch_bounds__( 512 ) void k_tile_half(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
-    int lcap, int scap, int slot_cap, int n_local, const double4 *__restrict__ x,
+    int lcap, int scap, int slot_cap, const double4 *__restrict__ x,
     const double4 *__restrict__ xg, const int *__restrict__ gperm,
-    double *__restrict__ f, const int *__restrict__ counts,
-    const unsigned short *__restrict__ slab, const LJParams *__restrict__ Pd,
-    PairCoef one, double *__restrict__ partial )
+    const int *__restrict__ ghost_root, double *__restrict__ f,
+    const int *__restrict__ counts, const unsigned short *__restrict__ slab,
+    const LJParams *__restrict__ Pd, PairCoef one, double *__restrict__ partial )
 {
     extern __shared__ double s_rec[];
     __shared__ TileHead H;
@@ -706,19 +707,11 @@ __global__ __launch_bounds__( 512 ) void k_tile_half(
     char *s_acc = s_byte + half_acc_offset( slot_cap, ONE_TYPE );
     int *s_gh = reinterpret_cast<int *>( s_acc + (size_t)slot_cap * TILE_REC );
     const int total = H.segoff[H.nseg];
-    for ( int k = threadIdx.x; k < 3 * slot_cap; k += blockDim.x )
-        reinterpret_cast<double *>( s_acc )[k] = 0.0;
     if ( ENERGY )
     {
-        int seg = 0;
         for ( int k = threadIdx.x; k < slot_cap; k += blockDim.x )
-        {
-            while ( seg < H.nseg && k >= H.segoff[seg + 1] )
-                ++seg;
-            s_gh[k] = ( seg & 1 ) && k < total;
-        }
+            s_gh[k] = k < total && ( find_seg( H.segoff, H.nseg, k ) & 1 );
     }
-    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap ); // ends with a barrier
     const int R4 = ( ( lcap + G - 1 ) / G + 3 ) / 4;
     const unsigned short *st = slab + (size_t)woff[t] * R4 * 256;
     const int nw = woff[t + 1] - woff[t];
@@ -727,34 +720,80 @@ __global__ __launch_bounds__( 512 ) void k_tile_half(
     const int nwb = blockDim.x >> 6;
     const unsigned long long sent4 =
         0x0001000100010001ull * (unsigned)( H.segoff[H.nseg] * TILE_REC );
+    // the lane's atom in slab wave w: gw lanes per atom (G, or more in a
+    // part-full wave), the atom, and the LDS slot it is staged at (its
+    // staged column's local run); lanes past the atoms take the tile's
+    // first atom
+    auto lane_map = [&]( int w, int &gw, int &apw, int &sub, int &i, int &islot, bool &active )
+    {
+        gw = wave_lanes( G, nat, w );
+        apw = 64 / gw;
+        const int a = lane % apw;
+        sub = lane / apw;
+        const int ta = w * APW + a;
+        active = a < APW && ta < nat;
+        const int tq = active ? ta : 0;
+        int ac = 0;
+        while ( ac + 1 < T.W * T.W && tq >= H.aoff[ac + 1] )
+            ++ac;
+        i = H.info[TI_AL + ac] + ( tq - H.aoff[ac] );
+        const int qc = atom_col_q( T.W, ac );
+        islot = H.segoff[2 * qc] + ( i - H.info[TI_LB + qc] );
+    };
+    auto row_of = [&]( int w ) -> const unsigned long long *
+    {
+        return reinterpret_cast<const unsigned long long *>(
+            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
+    };
+    // the atom's position, row length and first three list words, loaded
+    // before the staging so that their memory latency overlaps it (words
+    // past a row's end are loaded but not used)
+    struct WaveIn
+    {
+        double4 xi;
+        int n;
+        unsigned long long q0, q1, q2;
+    };
+    auto prep = [&]( int w, WaveIn &P )
+    {
+        int gw, apw, sub, i, islot;
+        bool active;
+        lane_map( w, gw, apw, sub, i, islot, active );
+        P.xi = x[i];
+        P.n = counts[i];
+        const unsigned long long *row8 = row_of( w );
+        P.q0 = __builtin_nontemporal_load( row8 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        P.q1 = __builtin_nontemporal_load( row8 + (size_t)min( 1, R4 - 1 ) * 64 );
+        __builtin_amdgcn_sched_barrier( 0 );
+        P.q2 = __builtin_nontemporal_load( row8 + (size_t)min( 2, R4 - 1 ) * 64 );
+        __builtin_amdgcn_sched_barrier( 0 );
+    };
+    WaveIn P;
+    if ( wid < nw )
+        prep( wid, P );
+    // the reaction records are cleared while the staging loads are in flight
+    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap, reinterpret_cast<double *>( s_acc ),
+                              3 * slot_cap ); // ends with a barrier
     double pe = 0.0, vir = 0.0;
     for ( int w = wid; w < nw; w += nwb )
     {
-        // gw lanes per atom in this slab wave (G, or more in a part-full wave)
-        const int gw = wave_lanes( G, nat, w ), apw = 64 / gw;
-        const int a = lane % apw, sub = lane / apw;
-        const int ta = w * APW + a;
-        const bool active = a < APW && ta < nat;
-        // the atom, its atom column and the LDS slot it is staged at (its
-        // staged column's local run)
-        int ac = 0;
-        while ( ac + 1 < T.W * T.W && ta >= H.aoff[ac + 1] )
-            ++ac;
-        const int i = H.info[TI_AL + ac] + ( ta - H.aoff[ac] );
-        const int qc = atom_col_q( T.W, ac );
-        const int islot = H.segoff[2 * qc] + ( i - H.info[TI_LB + qc] );
+        if ( w != wid )
+            prep( w, P );
+        int gw, apw, sub, i, islot;
+        bool active;
+        lane_map( w, gw, apw, sub, i, islot, active );
         double fx = 0.0, fy = 0.0, fz = 0.0;
         const double4 xi =
-            active ? x[i] : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
+            active ? P.xi : make_double4( -TILE_FAR, -TILE_FAR, -TILE_FAR, 0.0 );
         const int ti = (int)xi.w;
-        const int n = active ? counts[i] : 0;
+        const int n = active ? P.n : 0;
         int nr = n > sub ? ( n - sub + gw - 1 ) / gw : 0;
 #pragma unroll
         for ( int o = 32; o > 0; o >>= 1 )
             nr = max( nr, __shfl_xor( nr, o, 64 ) );
         nr = __builtin_amdgcn_readfirstlane( nr );
-        const unsigned long long *row8 = reinterpret_cast<const unsigned long long *>(
-            st + ( (size_t)w * R4 * 64 + lane ) * 4 );
+        const unsigned long long *row8 = row_of( w );
         auto body = [&]( unsigned off )
         {
             const double *p = reinterpret_cast<const double *>( s_byte + off );
@@ -799,13 +838,7 @@ __global__ __launch_bounds__( 512 ) void k_tile_half(
             body( (unsigned)( ( wd >> 32 ) & 0xffffull ) );
             body( (unsigned)( wd >> 48 ) );
         };
-        const unsigned long long q0i = word( 0 );
-        __builtin_amdgcn_sched_barrier( 0 );
-        const unsigned long long q1i = word( 1 );
-        __builtin_amdgcn_sched_barrier( 0 );
-        const unsigned long long q2i = word( 2 );
-        __builtin_amdgcn_sched_barrier( 0 );
-        unsigned long long q0 = q0i, q1 = q1i, q2 = q2i;
+        unsigned long long q0 = P.q0, q1 = P.q1, q2 = P.q2;
         for ( int k = 0; k < nq; k += 3 )
         {
             quad( q0 );
@@ -833,19 +866,20 @@ __global__ __launch_bounds__( 512 ) void k_tile_half(
     }
     __syncthreads();
     // every staged slot with a reaction: locals by run start, ghosts through
-    // the ghost sort permutation to their ghost rows
+    // the ghost sort permutation to the local atoms they image (a ghost's
+    // force is its source atom's, which is what update_force's fold of the
+    // ghost rows gives: the ghost rows are left untouched, and a fold adds
+    // nothing)
     {
-        int seg = 0;
         for ( int k = threadIdx.x; k < total; k += blockDim.x )
         {
             const double *r = reinterpret_cast<const double *>( s_acc + k * TILE_REC );
             const double ax = r[0], ay = r[1], az = r[2];
-            while ( k >= H.segoff[seg + 1] )
-                ++seg;
             if ( ax == 0.0 && ay == 0.0 && az == 0.0 )
                 continue;
+            const int seg = find_seg( H.segoff, H.nseg, k );
             const int q = seg >> 1, o = k - H.segoff[seg];
-            const int idx = ( seg & 1 ) ? n_local + gperm[H.info[TI_GB + q] + o]
+            const int idx = ( seg & 1 ) ? ghost_root[gperm[H.info[TI_GB + q] + o]]
                                         : H.info[TI_LB + q] + o;
             unsafeAtomicAdd( &f[idx], ax );
             unsafeAtomicAdd( &f[scap + idx], ay );
@@ -1256,9 +1290,9 @@ static void launch_tile_half_k( const cmd_neigh *N, cmd_system *S, const LJParam
     const size_t lds = half_acc_offset( slot_cap, ONE ) + (size_t)slot_cap * TILE_REC +
                        ( ENERGY ? (size_t)slot_cap * sizeof( int ) : 0 );
     k_tile_half<G, PREC, ONE, ENERGY><<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
-        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
-        S->n_local, S->x.p, S->xg.p, S->gsort.perm.p, S->f.p, N->counts.p,
-        N->tslab.p, Pd, c, partial );
+        N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap, S->x.p,
+        S->xg.p, S->gsort.perm.p, S->ghost_root.p, S->f.p, N->counts.p, N->tslab.p, Pd, c,
+        partial );
 }
 
 template <int G, int PREC>

@@ -307,7 +307,7 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         CMD_TRY( S->x_alt.reserve( S->cap ) );
         int nb_force = 0, nb_int = 0;
         ctl.xg_next = nullptr;
-        if ( md->fuse_ghosts && !half && N->tiled && S->xg_valid && S->img_valid &&
+        if ( md->fuse_ghosts && N->tiled && S->xg_valid && S->img_valid &&
              S->n_ghost > 0 && ctl.do_next_initial && !ctl.wrap_next )
         {
             CMD_TRY( S->xg_alt.reserve( S->n_ghost ) );
@@ -325,18 +325,33 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         }
         else
         {
+            // tiled half lists: the force kernel adds each ghost's reaction
+            // to the atom it images (nothing left to fold), and the
+            // integrator clears the forces it has read for the next step's
+            // sums, so f is cleared here only at the call's first step (it
+            // held the last call's forces for the first half-kick) and after
+            // a list rebuild (new ghost rows)
+            const bool tiled = N->tiled;
             int k = md->begin( PH_FORCE );
-            const int nt = S->n_local + S->n_ghost;
-            for ( int c = 0; c < 3; ++c )
-                CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap, 0.0, nt, st ) );
+            if ( !tiled || rb || s == s0 )
+            {
+                const int nt = S->n_local + S->n_ghost;
+                for ( int c = 0; c < 3; ++c )
+                    CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap, 0.0, nt, st ) );
+            }
             CMD_TRY( lj_half( F, S, N, ctl.thermo, md->partA.p, &nb_force ) );
             md->end( k );
-            k = md->begin( PH_COMM );
-            CMD_TRY( cmd_comm_update_force( S ) );
-            md->end( k );
+            if ( !tiled )
+            {
+                k = md->begin( PH_COMM );
+                CMD_TRY( cmd_comm_update_force( S ) );
+                md->end( k );
+            }
             k = md->begin( PH_INT );
+            ctl.zero_f = tiled && !last ? 1 : 0;
             CMD_TRY( nve_final_initial( S, ctl, S->x_alt.p, md->partB.p,
                                         &nb_int ) );
+            ctl.zero_f = 0;
             md->end( k );
         }
         if ( th )

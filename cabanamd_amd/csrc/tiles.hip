@@ -399,7 +399,6 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const int btotal = s_bseg[18];
     {
         constexpr int B = 8;
-        int seg = 0;
         for ( int k0 = threadIdx.x; k0 < btotal; k0 += B * blockDim.x )
         {
             double4 p[B];
@@ -407,8 +406,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
             for ( int u = 0; u < B; ++u )
             {
                 const int k = min( k0 + u * (int)blockDim.x, btotal - 1 );
-                while ( k >= s_bseg[seg + 1] )
-                    ++seg;
+                const int seg = find_seg( s_bseg, 18, k );
                 const int qb = seg >> 1, off = k - s_bseg[seg];
                 const int qf = ( ca + qb / 3 ) * S + cb + qb % 3;
                 p[u] = ( seg & 1 ) ? xg[H.info[TI_GB + qf] + off] : x[H.info[TI_LB + qf] + off];
@@ -1160,7 +1158,11 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     if ( mr > N->max_neigh_guess )
         N->max_neigh_guess = (int)( mr * 1.1 );
     if ( mr > N->cap )
-        return CMD_OK; // rows overflowed: the caller's build regrows them
+    {
+        // rows overflowed: the caller's build starts at the grown rows
+        N->cap = std::max( N->max_neigh_guess, mr );
+        return CMD_OK;
+    }
     N->tile_blocks = blocks;
     N->max_tile_slots = h[0];
     N->tiled = true;

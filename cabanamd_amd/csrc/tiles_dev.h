@@ -147,18 +147,45 @@ __device__ __forceinline__ int wave_lanes( int G, int nat, int w )
     return g;
 }
 
+// The segment s (0 <= s < nseg) with off[s] <= k < off[s + 1], for
+// off[0] <= k < off[nseg]: a binary search, so that the searches of a
+// thread's several slots are independent chains of a few LDS reads (a walk
+// from segment to segment would chain one read per segment passed).
+__device__ __forceinline__ int find_seg( const int *off, int nseg, int k )
+{
+    int lo = 0, hi = nseg;
+    while ( hi - lo > 1 )
+    {
+        const int mid = ( lo + hi ) >> 1;
+        if ( off[mid] <= k )
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Copy the staged runs of positions (locals from x, ghosts from xg) into LDS
 // as 24-byte records, followed by the sentinel record; with WITH_TYPE the
-// types follow as ints, `cap` records on.
+// types follow as ints, `cap` records on.  Each thread has up to B loads in
+// flight, so a tile's neighbourhood (about a thousand records) arrives in
+// one round of memory latency for blocks of 256 threads and more.  The
+// `nzero` doubles at `zero` (an LDS area of the caller's) are cleared while
+// the first round's loads are in flight.  Ends with a barrier.
 template <bool WITH_TYPE>
 __device__ inline void stage_records( const TileHead &H, const double4 *__restrict__ x,
                                       const double4 *__restrict__ xg,
-                                      double *__restrict__ s_rec, int cap )
+                                      double *__restrict__ s_rec, int cap,
+                                      double *__restrict__ zero = nullptr, int nzero = 0 )
 {
     int *s_t = reinterpret_cast<int *>( s_rec + 3 * cap );
     const int total = H.segoff[H.nseg];
-    constexpr int B = 2; // slots per thread whose loads are in flight together
-    int seg = 0;
+    constexpr int B = 8; // slots per thread whose loads are in flight together
+    auto clear = [&]()
+    {
+        for ( int k = threadIdx.x; k < nzero; k += blockDim.x )
+            zero[k] = 0.0;
+    };
     for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
     {
         double4 p[B];
@@ -167,13 +194,14 @@ __device__ inline void stage_records( const TileHead &H, const double4 *__restri
         {
             // past the end: reload the last slot (in bounds)
             const int k = min( k0 + u * (int)blockDim.x, total - 1 );
-            while ( k >= H.segoff[seg + 1] )
-                ++seg;
+            const int seg = find_seg( H.segoff, H.nseg, k );
             const int q = seg >> 1, off = k - H.segoff[seg];
             const double4 *__restrict__ base = ( seg & 1 ) ? xg : x;
             const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
             p[u] = base[idx];
         }
+        if ( k0 == (int)threadIdx.x )
+            clear();
         // stores past the end go to the sentinel record, rewritten below
 #pragma unroll
         for ( int u = 0; u < B; ++u )
@@ -186,6 +214,9 @@ __device__ inline void stage_records( const TileHead &H, const double4 *__restri
                 s_t[k] = (int)p[u].w;
         }
     }
+    // threads past the records clear their share too
+    if ( (int)threadIdx.x >= total )
+        clear();
     __syncthreads();
     if ( threadIdx.x == 0 )
     {
