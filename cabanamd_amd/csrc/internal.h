@@ -165,6 +165,16 @@ __device__ __forceinline__ int cell_coord( double c, double mn, double rd,
     return ic;
 }
 
+// Bytes of one staged position in the tiled force kernels' LDS records:
+// TILE_REC for FP64 records (x, y, z doubles), TILE_REC32 for the FP32
+// records of the mixed-precision full-list kernel (x, y, z floats about the
+// tile's centre and the type).  Tiled list entries are byte offsets of
+// records (slot * record size), so a gather is one address add and LDS reads
+// with immediate offsets; the largest tile (MAX_SLOTS staged atoms plus the
+// sentinel) must keep every offset inside 16 bits.
+constexpr int TILE_REC = 24;
+constexpr int TILE_REC32 = 16;
+
 // Tiles of the binning grid: a tile is the run of cells [e0, e1) of one
 // interior column (a, b); ntiles = nc0 * nc1 * tpc.
 struct TileGrid
@@ -287,6 +297,12 @@ struct cmd_neigh
     cmd::DevBuf<int64_t> tile_off; // slab offset of each tile (16-bit units)
     cmd::DevBuf<unsigned short> tslab;
     int max_tile_slots = 0;
+    // Bytes per staged record that the tiled list's entries count in (an
+    // entry is slot * rec): 24 for the FP64 records (x, y, z doubles), 16
+    // for the FP32 records of the mixed-precision full-list kernel (x, y, z
+    // floats about the tile's centre and the type)
+    int rec = cmd::TILE_REC;       // for the next build
+    int built_rec = cmd::TILE_REC; // of the tiled list as built
     const int *gperm_view = nullptr; // the system's ghost sort at build time
     // Sizes of the last tiled build (with some headroom), with which the
     // next build is launched before its own tile sizes are read back

@@ -467,7 +467,7 @@ __global__ __launch_bounds__( TPB ) void k_lj_full(
 // lanes sit at another far point.  A padded pair adds exact zeros.
 // LDS keeps the tiled kernels at three or four waves per SIMD, so the
 // compiler may use registers for more loads and pair chains in flight.
-template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY>
+template <int G, int PREC, bool ONE_TYPE, bool FUSED, bool ENERGY, bool F32REC>
 __global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 ) ) ) void k_tile_force(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
     int lcap, int scap, int slot_cap, const double4 *__restrict__ x,
@@ -495,8 +495,13 @@ __global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 )
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int nwb = blockDim.x >> 6;
     // four sentinel entries: the word a lane past the tile's atoms walks
+    constexpr int REC = F32REC ? TILE_REC32 : TILE_REC;
     const unsigned long long sent4 =
-        0x0001000100010001ull * (unsigned)( H.segoff[H.nseg] * TILE_REC );
+        0x0001000100010001ull * (unsigned)( H.segoff[H.nseg] * REC );
+    // FP32 records: coordinates about the centre of the staged region
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if constexpr ( F32REC )
+        tile_centre( T, t, ox, oy, oz );
     double pe = 0.0, vir = 0.0, mv2 = 0.0;
     // What a slab wave reads from global memory before its pair loop: the
     // atom, its row length and the first three list words.  The first
@@ -547,7 +552,10 @@ __global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 )
     WaveIn P;
     if ( wid < nw )
         prep( wid, P );
-    stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
+    if constexpr ( F32REC )
+        stage_records32( H, x, xg, reinterpret_cast<float4 *>( s_rec ), ox, oy, oz );
+    else
+        stage_records<!ONE_TYPE>( H, x, xg, s_rec, slot_cap );
     __builtin_amdgcn_s_setprio( 0 );
     for ( int w = wid; w < nw; w += nwb )
     {
@@ -590,6 +598,39 @@ __global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 )
                 img_e = ctl.img_start[i + 1];
             }
         }
+        // the atom about the tile's centre for FP32 records (inactive lanes
+        // at the far point, whose distance to every record overflows to
+        // +inf and is outside every cutoff)
+        const float xfx = active ? (float)( xi.x - ox ) : -(float)TILE_FAR;
+        const float xfy = active ? (float)( xi.y - oy ) : -(float)TILE_FAR;
+        const float xfz = active ? (float)( xi.z - oz ) : -(float)TILE_FAR;
+        const float csf = (float)one.cs;
+        // FP32 records: the pair in FP32 from the coordinate differences
+        // on, the force summed in FP64
+        auto body32 = [&]( const float4 &xj )
+        {
+            const float dx = xfx - xj.x;
+            const float dy = xfy - xj.y;
+            const float dz = xfz - xj.z;
+            const float rsq = __builtin_fmaf( dx, dx, __builtin_fmaf( dy, dy, dz * dz ) );
+            const PairCoef c = coef<ONE_TYPE>( Pd, one, ti, __float_as_int( xj.w ) );
+            const bool in = rsq < ( ONE_TYPE ? csf : (float)c.cs );
+            const float r2inv = __builtin_amdgcn_rcpf( in ? rsq : __builtin_inff() );
+            const float r6inv = r2inv * r2inv * r2inv;
+            const float fp = ( r6inv * __builtin_fmaf( c.lj1f, r6inv, -c.lj2f ) ) * r2inv;
+            if ( ENERGY )
+            {
+                const float ep =
+                    r6inv * __builtin_fmaf( 0.5f * c.lj1f, r6inv, -c.lj2f ) * ( 1.0f / 6.0f );
+                pe += in ? (double)ep - c.esh : 0.0;
+                // a far record's distance overflows to +inf in FP32
+                vir += in ? (double)rsq * (double)fp : 0.0;
+            }
+            const double fpd = (double)fp;
+            fx = __builtin_fma( (double)dx, fpd, fx );
+            fy = __builtin_fma( (double)dy, fpd, fy );
+            fz = __builtin_fma( (double)dz, fpd, fz );
+        };
         auto body = [&]( const double4 &xj )
         {
             const double dx = xi.x - xj.x;
@@ -625,17 +666,36 @@ __global__ __launch_bounds__( 512 ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 )
         const int klast = max( nq - 1, 0 );
         auto word = [&]( int k ) -> unsigned long long
         { return __builtin_nontemporal_load( row8 + (size_t)min( k, klast ) * 64 ); };
+        auto fetch32 = [&]( unsigned long long wd, int q ) -> float4
+        {
+            const unsigned off = (unsigned)( ( wd >> ( 16 * q ) ) & 0xffffull );
+            return *reinterpret_cast<const float4 *>( s_byte + off );
+        };
         auto quad = [&]( unsigned long long wd )
         {
             const unsigned long long cur = active ? wd : sent4;
-            const double4 p0 = fetch( cur, 0 );
-            const double4 p1 = fetch( cur, 1 );
-            const double4 p2 = fetch( cur, 2 );
-            const double4 p3 = fetch( cur, 3 );
-            body( p0 );
-            body( p1 );
-            body( p2 );
-            body( p3 );
+            if constexpr ( F32REC )
+            {
+                const float4 p0 = fetch32( cur, 0 );
+                const float4 p1 = fetch32( cur, 1 );
+                const float4 p2 = fetch32( cur, 2 );
+                const float4 p3 = fetch32( cur, 3 );
+                body32( p0 );
+                body32( p1 );
+                body32( p2 );
+                body32( p3 );
+            }
+            else
+            {
+                const double4 p0 = fetch( cur, 0 );
+                const double4 p1 = fetch( cur, 1 );
+                const double4 p2 = fetch( cur, 2 );
+                const double4 p3 = fetch( cur, 3 );
+                body( p0 );
+                body( p1 );
+                body( p2 );
+                body( p3 );
+            }
         };
         unsigned long long q0 = P.q0, q1 = P.q1, q2 = P.q2;
         for ( int k = 0; k < nq; k += 3 )
@@ -1118,10 +1178,26 @@ static void launch_tile_k( cmd_neigh *N, cmd_system *S, double4 *xn,
 {
     // staged records plus the sentinel (and an int type per slot)
     const int slot_cap = std::max( N->max_tile_slots, 1 ) + 1;
+    if constexpr ( PREC == CMD_PREC_MIXED )
+    {
+        if ( N->built_rec == TILE_REC32 )
+        {
+            // FP32 records (the type inside); at least the 4 x 8 doubles of
+            // the thermo block sums, which reuse them
+            const size_t lds =
+                std::max<size_t>( (size_t)slot_cap * TILE_REC32, 32 * sizeof( double ) );
+            k_tile_force<G, PREC, ONE, FUSED, ENERGY, true>
+                <<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
+                    N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
+                    S->x.p, S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt,
+                    ctl, partial );
+            return;
+        }
+    }
     // at least the 4 x 8 doubles of the thermo block sums, which reuse it
     const size_t lds = std::max<size_t>(
         (size_t)slot_cap * ( TILE_REC + ( ONE ? 0 : sizeof( int ) ) ), 32 * sizeof( double ) );
-    k_tile_force<G, PREC, ONE, FUSED, ENERGY><<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
+    k_tile_force<G, PREC, ONE, FUSED, ENERGY, false><<<N->tg.ntiles, 64 * N->tg.bw, lds, S->stream>>>(
         N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, S->cap, slot_cap,
         S->x.p, S->xg.p, xn, S->v.p, S->f.p, N->counts.p, N->tslab.p, Pd, c, mt,
         ctl, partial );
@@ -1176,6 +1252,9 @@ static cmd_status launch_tile( int prec, bool one, bool fused,
                                double *partial )
 {
     cmd_neigh *N = const_cast<cmd_neigh *>( Nc );
+    CMD_REQUIRE( N->built_rec == TILE_REC || prec == CMD_PREC_MIXED, CMD_ERR_STATE,
+                 "the tiled list addresses FP32 records (mixed precision); rebuild it for "
+                 "FP64 forces" );
     if ( prec == CMD_PREC_FP64 )
     {
         if ( one )

@@ -48,6 +48,7 @@ struct cmd_md
     bool fuse_ghosts = true; // update_halo inside the fused step (tiled lists)
     // set for one rebuild: the list is built by timing every layout
     bool retune = false;
+    bool relist = false; // rebuild the list at the next step (its records changed)
     double tune_s = 0.0; // host seconds spent choosing the layout at init
     std::vector<double> thermo; // rows of 5: step, T, PE/N, (PE+KE)/N, W
     cmd_nnp_model *nnp_model = nullptr; // injected NNP model (owned)
@@ -258,11 +259,15 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
         ctl.L[k] = S->dom.L[k];
 
     auto t0 = std::chrono::steady_clock::now();
+    // a list that addresses the other precision's records is rebuilt at
+    // this call's first step
+    const bool relist = md->relist;
+    md->relist = false;
     // initial_integrate of the first step of this call (in place)
     {
         int k = md->begin( PH_INT );
         CMD_TRY( cmd_nve_initial( S ) );
-        if ( s0 % p.every == 0 )
+        if ( s0 % p.every == 0 || relist )
             CMD_TRY( wrap_local( S ) );
         md->end( k );
     }
@@ -271,7 +276,7 @@ static cmd_status run_steps( cmd_md *md, int nsteps )
     bool ghosts_next = false;
     for ( int s = s0; s <= s1; ++s )
     {
-        const bool rb = ( s % p.every == 0 );
+        const bool rb = ( s % p.every == 0 ) || ( relist && s == s0 );
         if ( rb )
         {
             S->n_ghost = 0;
@@ -478,6 +483,19 @@ cmd_status cmd_md_set_precision( cmd_md *md, int prec )
     md->prec = prec;
     if ( md->F )
         CMD_TRY( cmd_force_set_precision( md->F, prec ) );
+    // full lists address FP32 records in mixed precision, FP64 ones
+    // otherwise: a built list of the other kind is rebuilt before its next use
+    if ( md->N && !md->N->half && md->F && md->F->kind != 2 )
+    {
+        const int rec = prec == CMD_PREC_MIXED ? TILE_REC32 : TILE_REC;
+        if ( md->N->rec != rec )
+        {
+            md->N->rec = rec;
+            md->N->spec.valid = false;
+            if ( md->N->tiled )
+                md->relist = true;
+        }
+    }
     return CMD_OK;
 }
 
@@ -661,6 +679,9 @@ cmd_status cmd_md_init( cmd_md *md )
         CMD_TRY( cmd_neigh_set_tile( md->N, md->tile ) );
     if ( !nnp && md->tile_w > 0 )
         CMD_TRY( cmd_neigh_set_tile_width( md->N, md->tile_w ) );
+    // the mixed-precision full-list kernel stages FP32 records
+    if ( !nnp && !p.half && md->prec == CMD_PREC_MIXED )
+        md->N->rec = TILE_REC32;
     if ( md->lanes == 0 && !p.half && !nnp )
         CMD_TRY( tune_layout( md, n, x, v, type, id ) );
     // cabanamd_impl.h:197-223: exchange, sort, halo, list, f = 0, force

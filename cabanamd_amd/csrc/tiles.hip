@@ -329,7 +329,7 @@ template <int G, bool HALF>
 __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 ) ) ) void k_tile_build(
     TileGrid T, const int *__restrict__ ls, const int *__restrict__ gs,
     const int *__restrict__ info, const int *__restrict__ woff,
-    const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2,
+    const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2, int rec,
     int cap, int slot_cap, int rows_cap, int waves_cap, unsigned short *__restrict__ slab,
     int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over )
 {
@@ -450,7 +450,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const int fsent = H.segoff[H.nseg]; // the force tile's sentinel slot
     // rows start as the force tile's sentinel record
     {
-        const unsigned sent = (unsigned)( fsent * TILE_REC );
+        const unsigned sent = (unsigned)( fsent * rec );
         const unsigned sent2 = sent | ( sent << 16 );
         const uint4 fill = make_uint4( sent2, sent2, sent2, sent2 );
         const int n16 = ( nat_c * RS + 7 ) / 8;
@@ -500,7 +500,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
             {
                 t_p = inc - len;
                 t_dj = s0 - t_p;
-                t_df = ( s0 + s_dl[lane] - t_p ) * TILE_REC;
+                t_df = ( s0 + s_dl[lane] - t_p ) * rec;
             }
             // candidate index of the cell's first atom: in the centre
             // column's (range 4's) locals, cell ce
@@ -562,7 +562,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
             {
                 t_p = inc - len;
                 t_dj = rs[lane] - t_p;
-                t_df = ( rf[lane] - t_p ) * TILE_REC;
+                t_df = ( rf[lane] - t_p ) * rec;
             }
             // candidate index of the cell's first atom, in the centre
             // column's (lane 4's) ranges: the locals of cell ce, its range
@@ -633,7 +633,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                     const int v = v0 + ch * 64 + lane;
                     const bool ok = v < total;
                     const int sj = ok ? v + dj[ch] : bsent;
-                    po[ch] = ok ? v * TILE_REC + df[ch] : fsent * TILE_REC;
+                    po[ch] = ok ? v * rec + df[ch] : fsent * rec;
                     px[ch] = s_x[sj];
                     py[ch] = s_y[sj];
                     pz[ch] = s_z[sj];
@@ -677,7 +677,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         // or past cap are the sentinel.
         const int nat = H.info[TI_NAT], ta0 = H.aoff[c];
         const int w0 = ta0 / APW, w1 = ( ta0 + nat_c - 1 ) / APW;
-        const unsigned short sent = (unsigned short)( fsent * TILE_REC );
+        const unsigned short sent = (unsigned short)( fsent * rec );
         for ( int sw = w0; sw <= w1; ++sw )
         {
             // lanes per atom of slab wave sw (a power of two): lane L of a
@@ -715,7 +715,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
 template <int G>
 __global__ __launch_bounds__( TPB ) void k_tile_to_csr(
     TileGrid T, const int *__restrict__ info, const int *__restrict__ woff,
-    int cap, int n_local, const unsigned short *__restrict__ slab,
+    int cap, int n_local, int rec, const unsigned short *__restrict__ slab,
     const int *__restrict__ counts, const int *__restrict__ gperm,
     const int64_t *__restrict__ offsets, int *__restrict__ idx )
 {
@@ -737,10 +737,8 @@ __global__ __launch_bounds__( TPB ) void k_tile_to_csr(
         {
             const int r = k / gw, lane = ( k % gw ) * apw + aw;
             const int sj =
-                st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )] / TILE_REC;
-            int seg = 0;
-            while ( sj >= H.segoff[seg + 1] )
-                ++seg;
+                st[( ( w * R4 + ( r >> 2 ) ) * 64 + lane ) * 4 + ( r & 3 )] / rec;
+            const int seg = find_seg( H.segoff, H.nseg, sj );
             const int q = seg >> 1, off = sj - H.segoff[seg];
             idx[o + k] = ( seg & 1 ) ? n_local + gperm[H.info[TI_GB + q] + off]
                                      : H.info[TI_LB + q] + off;
@@ -948,11 +946,13 @@ static void lds_limits( size_t *block, size_t *cu )
 }
 
 // LDS of the force kernels on tiles with `slots` staged atoms: a position
-// record and, with several types, a type per slot (lj.hip); the half list
-// adds a reaction record and an energy weight per slot.
-static size_t force_lds( int slots, bool half )
+// record and, with several types, a type per slot (lj.hip; FP32 records
+// hold the type); the half list adds a reaction record and an energy weight
+// per slot.
+static size_t force_lds( int slots, bool half, int rec )
 {
-    return (size_t)( slots + 1 ) * ( TILE_REC + sizeof( int ) ) * ( half ? 2 : 1 ) + 16;
+    const size_t per = rec == TILE_REC32 ? (size_t)TILE_REC32 : TILE_REC + sizeof( int );
+    return (size_t)( slots + 1 ) * per * ( half ? 2 : 1 ) + 16;
 }
 
 template <int G>
@@ -967,12 +967,12 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
     if ( N->half )
         k_tile_build<G, true><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
-            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap, rows_cap,
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
     else
         k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
-            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->cap, slot_cap, rows_cap,
+            N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
 }
 
@@ -1022,7 +1022,7 @@ static int requested_height( const cmd_neigh *N, int W )
 static bool layout_ok( const cmd_neigh *N, const int h[6], size_t lds, size_t hard,
                        size_t lds_cu, int E, bool *fits, int *blocks )
 {
-    const size_t flds = force_lds( h[0], N->half != 0 );
+    const size_t flds = force_lds( h[0], N->half != 0, N->rec );
     *fits = h[0] <= MAX_SLOTS && lds <= hard && flds <= hard;
     // blocks per CU of the force kernel (its static tables included)
     *blocks = (int)std::min( lds_cu / ( flds + 1024 ), (size_t)( 32 / N->tg.bw ) );
@@ -1054,6 +1054,8 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
 {
     *built = false;
     N->spec.valid = false;
+    if ( N->half )
+        N->rec = TILE_REC; // the half-list kernel stages FP64 records
     if ( !S->bin_grid_valid || !S->xg_valid || N->lanes < 2 )
         return CMD_OK;
     if ( N->cap < 1 )
@@ -1109,6 +1111,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
         {
             *built = true;
             N->tiled = true;
+            N->built_rec = N->rec;
             N->gperm_view = S->gsort.perm.p;
             record_spec( N, W, E, h, hard );
             return CMD_OK;
@@ -1166,6 +1169,7 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     N->tile_blocks = blocks;
     N->max_tile_slots = h[0];
     N->tiled = true;
+    N->built_rec = N->rec;
     N->gperm_view = S->gsort.perm.p;
     *built = true;
     record_spec( N, P.W, P.E, h, hard );
@@ -1180,17 +1184,17 @@ cmd_status tiles_csr( cmd_neigh *N, int64_t *nnz_out )
     {
     case 2:
         k_tile_to_csr<2><<<nt, TPB, 0, N->stream>>>(
-            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows,
+            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows, N->built_rec,
             N->tslab.p, N->counts.p, N->gperm_view, N->offsets.p, N->idx.p );
         break;
     case 4:
         k_tile_to_csr<4><<<nt, TPB, 0, N->stream>>>(
-            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows,
+            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows, N->built_rec,
             N->tslab.p, N->counts.p, N->gperm_view, N->offsets.p, N->idx.p );
         break;
     default:
         k_tile_to_csr<8><<<nt, TPB, 0, N->stream>>>(
-            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows,
+            N->tg, N->tile_info.p, N->grid_sort.start.p, N->cap, N->n_rows, N->built_rec,
             N->tslab.p, N->counts.p, N->gperm_view, N->offsets.p, N->idx.p );
         break;
     }

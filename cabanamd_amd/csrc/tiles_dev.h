@@ -127,12 +127,8 @@ __device__ __forceinline__ int atom_col_q( int W, int c )
 // beyond every cutoff, so a padded pair contributes exact zeros.
 static constexpr double TILE_FAR = 1.0e30;
 
-// Bytes of one staged position in the force kernel's LDS records (x, y, z
-// doubles).  Tiled list entries are byte offsets of records (slot * 24), so
-// a gather is one address add and two LDS reads with immediate offsets; the
-// largest tile (MAX_SLOTS staged atoms plus the sentinel) must keep every
-// offset inside 16 bits.
-static constexpr int TILE_REC = 24;
+// Record sizes (TILE_REC, TILE_REC32) are defined with the neighbour list
+// (internal.h): tiled list entries are byte offsets of records.
 
 // Lanes per atom of slab wave w of a tile with nat atoms, for tiled lists
 // built with G lanes per atom (64 / G atoms per full wave): G, doubled up to
@@ -227,6 +223,60 @@ __device__ inline void stage_records( const TileHead &H, const double4 *__restri
             s_t[total] = 0;
     }
     __syncthreads();
+}
+
+// The same neighbourhood staged as 16-byte FP32 records for the
+// mixed-precision full-list kernel: x, y, z about the tile's centre (ox, oy,
+// oz; the staged region spans a few cells, so the coordinates keep about
+// 1e-7 of a cell in absolute precision) and the type's bits in w, followed
+// by the sentinel record.  Ends with a barrier.
+__device__ inline void stage_records32( const TileHead &H, const double4 *__restrict__ x,
+                                        const double4 *__restrict__ xg,
+                                        float4 *__restrict__ s_rec, double ox, double oy,
+                                        double oz )
+{
+    const int total = H.segoff[H.nseg];
+    constexpr int B = 8; // slots per thread whose loads are in flight together
+    for ( int k0 = threadIdx.x; k0 < total; k0 += B * blockDim.x )
+    {
+        double4 p[B];
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+        {
+            // past the end: reload the last slot (in bounds)
+            const int k = min( k0 + u * (int)blockDim.x, total - 1 );
+            const int seg = find_seg( H.segoff, H.nseg, k );
+            const int q = seg >> 1, off = k - H.segoff[seg];
+            const double4 *__restrict__ base = ( seg & 1 ) ? xg : x;
+            const int idx = ( ( seg & 1 ) ? H.info[TI_GB + q] : H.info[TI_LB + q] ) + off;
+            p[u] = base[idx];
+        }
+        // stores past the end go to the sentinel record, rewritten below
+#pragma unroll
+        for ( int u = 0; u < B; ++u )
+        {
+            const int k = min( k0 + u * (int)blockDim.x, total );
+            s_rec[k] = make_float4( (float)( p[u].x - ox ), (float)( p[u].y - oy ),
+                                    (float)( p[u].z - oz ), __int_as_float( (int)p[u].w ) );
+        }
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+        s_rec[total] = make_float4( (float)TILE_FAR, (float)TILE_FAR, (float)TILE_FAR,
+                                    __int_as_float( 0 ) );
+    __syncthreads();
+}
+
+// Centre of tile t's staged region, the origin of its FP32 records.
+__device__ __forceinline__ void tile_centre( const TileGrid &T, int t, double &ox, double &oy,
+                                             double &oz )
+{
+    int a0, b0, e0, e1;
+    tile_coords( T, t, a0, b0, e0, e1 );
+    // columns a0 - 1 .. a0 + W, cells e0 - 1 .. e1
+    ox = T.g.min[0] + ( a0 - 1 + 0.5 * ( T.W + 2 ) ) / T.g.rd[0];
+    oy = T.g.min[1] + ( b0 - 1 + 0.5 * ( T.W + 2 ) ) / T.g.rd[1];
+    oz = T.g.min[2] + 0.5 * ( e0 + e1 ) / T.g.rd[2];
 }
 
 } // namespace cmd
