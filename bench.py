@@ -15,6 +15,7 @@ timed steps.  cpu_baseline times the FP64 OpenMP oracle on a bounded sample
 of the same workload on the host's cores.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -207,6 +208,23 @@ def _nnp_case(M, n, steps, warmup, prec, accuracy=False):
     md.run(steps)
     M._lib.call("cmd_device_synchronize")
     dt = time.perf_counter() - t0
+    # the force phase's GPU time, from a few more steps with the engine's
+    # per-phase events on (kept out of the timed steps)
+    md.profile(True)
+    md.run(min(steps, 10))
+    _, force_ms = md.kernel_stats()
+    md.profile(False)
+    # the work of one force call under the engine's per-term flop model
+    # (FP32: add, multiply 1, fused multiply-add 2, transcendental 1)
+    wk = (C.c_double * 6)()
+    M._lib.call("cmd_force_nnp_work", md.force.handle, wk)
+    gf = [wk[2] / 1e9, wk[3] / 1e9, wk[4] / 1e9]
+    work = dict(neighbours_per_atom=wk[0] / max(wk[5], 1),
+                triplets_per_atom=wk[1] / max(wk[5], 1),
+                gflop_symmetry_functions=gf[0], gflop_force_chain=gf[1],
+                gflop_layers=gf[2],
+                force_phase_ms=force_ms,
+                model_tflops=sum(gf) / force_ms if force_ms > 0 else None)
     th = md.thermo()
     pe = th[:, 2] * n / 27.211384021355236
     ke = (th[:, 3] - th[:, 2]) * n
@@ -214,7 +232,8 @@ def _nnp_case(M, n, steps, warmup, prec, accuracy=False):
     out = dict(n=n, steps=steps, seconds=dt, ms_per_step=1e3 * dt / steps,
                atom_steps_per_s=n * steps / dt, init_s=t_init,
                energy_drift=float(np.abs(e - e[0]).max() / np.abs(e).max()),
-               layers="bf16-mfma" if prec == M.PREC_MIXED else "fp32")
+               layers="bf16-mfma" if prec == M.PREC_MIXED else "fp32",
+               work=work)
     if accuracy:
         out["force_error_vs_fp64_oracle"] = _nnp_force_error(M, md, model)
     del md
@@ -539,9 +558,17 @@ def main():
             r = _nnp_case(M, sz, args.nnp_steps, 5, prec,
                           accuracy=(not args.no_cpu and sz == 3000))
             nnp[f"{sz}-{r['layers']}"] = r
+            w = r["work"]
             print(f"# nnp {sz} {r['layers']}: {r['atom_steps_per_s']:.4e} "
                   f"atom-steps/s, {r['ms_per_step']:.3f} ms/step, drift "
-                  f"{r['energy_drift']:.2e}", file=sys.stderr, flush=True)
+                  f"{r['energy_drift']:.2e}; per force call "
+                  f"{w['gflop_symmetry_functions']:.2f} + "
+                  f"{w['gflop_force_chain']:.2f} + {w['gflop_layers']:.3f} "
+                  f"GFLOP in {w['force_phase_ms']:.3f} ms = "
+                  f"{w['model_tflops'] or 0:.1f} TFLOP/s (model), "
+                  f"{w['neighbours_per_atom']:.1f} neighbours and "
+                  f"{w['triplets_per_atom']:.0f} triplets per atom",
+                  file=sys.stderr, flush=True)
     batch = None
     if args.batch > 0:
         batch = _batch_case(M, args.batch, args.batch_steps, M.PREC_MIXED,

@@ -141,6 +141,7 @@ _PROTOS = {
     "cmd_force_nnp_create": ([VP, C.POINTER(VP)], None),
     "cmd_force_nnp_fit_scaling": ([VP, VP, VP], None),
     "cmd_force_nnp_model": ([VP, C.POINTER(VP)], None),
+    "cmd_force_nnp_work": ([VP, C.POINTER(C.c_double)], None),
     "cmd_force_nnp_atomic": ([VP, VP, C.c_int, FP, FP, FP], None),
     "cmd_nnp_lattice_create": ([C.c_int, C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, IP, DP,

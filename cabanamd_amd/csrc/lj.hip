@@ -1548,8 +1548,9 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     // per slab wave of the largest tile)
     static const Cand all[] = { { 1, 0, 0, 0 }, { 2, 2, 2, 4 }, { 2, 2, 2, 2 }, { 2, 2, 3, 4 },
                                 { 2, 2, 3, 2 }, { 2, 2, 3, 8 }, { 2, 2, 4, 4 }, { 2, 2, 4, 2 },
+                                { 2, 2, 4, 8 }, { 2, 2, 5, 4 }, { 2, 2, 6, 4 }, { 2, 2, 6, 8 },
                                 { 2, 1, 4, 4 }, { 2, 1, 6, 4 }, { 2, 1, 6, 2 }, { 4, 2, 3, 4 },
-                                { 4, 1, 6, 4 }, { 8, 2, 3, 4 } };
+                                { 4, 2, 4, 4 }, { 4, 1, 6, 4 }, { 8, 2, 3, 4 } };
     // CMD_TUNE=G,W,E,bw limits the search to one layout (for measurements)
     std::vector<Cand> cv( std::begin( all ), std::end( all ) );
     if ( const char *e = std::getenv( "CMD_TUNE" ) )

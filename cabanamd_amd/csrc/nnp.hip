@@ -29,6 +29,7 @@
 // product of the two.  Both the reference fixture and the benchmark model
 // have this form; other sets are rejected at setup.
 #include "internal.h"
+#include "roofline.h"
 
 #include <algorithm>
 #include <cmath>
@@ -134,7 +135,9 @@ struct cmd_nnp
     cmd::DevBuf<float> G, dEdG, Ei;
     cmd::DevBuf<int> elist, ecount, flag;
     cmd::DevBuf<double> part;
+    cmd::DevBuf<int4> terms; // per atom: neighbours, triplets, element (last compute)
     int cap = 0;
+    int n_last = 0;
 };
 
 namespace cmd
@@ -589,7 +592,7 @@ __device__ __forceinline__ void pair_of( int p, int n, int &a, int &q )
 // Used where the body keeps many accumulators in registers.
 template <class Body>
 __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
-                                             unsigned *buf, Body body )
+                                             unsigned *buf, Body body, int *count = nullptr )
 {
     const int lane = threadIdx.x & 63;
     // candidate pairs, 64 per test: with n <= 64 rows a and n - 2 - a side
@@ -648,6 +651,8 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
             ++it;
         }
         __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
+        if ( count )
+            *count += cnt;
         const int per = ( cnt + 63 ) >> 6;
         for ( int j = 0; j < per; ++j )
         {
@@ -669,7 +674,7 @@ template <int NAG, int NAP>
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     int n_local, const double4 *__restrict__ x, const int64_t *__restrict__ off,
     const int *__restrict__ idx, const NnpPlan *__restrict__ P,
-    float *__restrict__ G, int *__restrict__ flag )
+    float *__restrict__ G, int *__restrict__ flag, int4 *__restrict__ terms )
 {
     __shared__ NbLds lds[NNP_TPB / 64];
     __shared__ unsigned tbuf[NNP_TPB / 64][TCAP];
@@ -728,7 +733,9 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
 #pragma unroll
         for ( int s = 0; s < NA_MAX; ++s )
             acc[s] = 0.0f;
-        for_triplets( L, n, rc2, tbuf[wv], [&]( int a, int q ) {
+        int ntrip = 0;
+        for_triplets(
+            L, n, rc2, tbuf[wv], [&]( int a, int q ) {
             const float4 A = L.d[a], B = L.d[q], Ea = L.e[a], Eb = L.e[q];
             const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
             const float r3 = __builtin_amdgcn_sqrtf( cx * cx + cy * cy + cz * cz );
@@ -752,7 +759,13 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
                         acc[g * AP_MAX + pp] += Eg * Pw[pp];
                 }
             }
-        } );
+            },
+            &ntrip );
+        // the work model's term counts (roofline.h) of this atom: its
+        // neighbours and triplets, and its element for the per-element
+        // grids (one store per atom, summed when asked for)
+        if ( terms && lane == 0 )
+            terms[i] = make_int4( n, ntrip, e, 0 );
         wave_reduce_scatter<NA_MAX>( acc );
         const int g = lane / AP_MAX, pp = lane % AP_MAX;
         if ( g < Q.nag && pp < Q.nap )
@@ -1292,6 +1305,8 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
     if ( n == 0 )
         return CMD_OK;
     CMD_TRY( fill_i32( reinterpret_cast<int *>( M->G.p ), 0, (size_t)n * NSF_LD, st ) );
+    CMD_TRY( M->terms.reserve( std::max( n, 1 ) ) );
+    M->n_last = n;
     const int nbw = ( n + NNP_TPB / 64 - 1 ) / ( NNP_TPB / 64 );
     // the angular grid of every element, for the kernels compiled for it
     bool g58 = M->plan.nelem > 0;
@@ -1299,10 +1314,10 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
         g58 = g58 && M->plan.el[e].nag == 5 && M->plan.el[e].nap == 8;
     if ( g58 )
         k_nnp_sf<5, 8><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
-                                                 M->dplan.p, M->G.p, M->flag.p );
+                                                 M->dplan.p, M->G.p, M->flag.p, M->terms.p );
     else
         k_nnp_sf<0, 0><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
-                                                 M->dplan.p, M->G.p, M->flag.p );
+                                                 M->dplan.p, M->G.p, M->flag.p, M->terms.p );
     CMD_LAUNCH_CHECK();
     if ( M->prec == CMD_PREC_MIXED )
     {
@@ -1919,6 +1934,52 @@ cmd_status cmd_force_nnp_model( const cmd_force *f, const cmd_nnp_model **m )
 {
     CMD_REQUIRE( f && f->kind == 2 && m, CMD_ERR_ARG, "bad argument" );
     *m = &f->nnp->model;
+    return CMD_OK;
+}
+
+// Work of the last compute under the flop model of roofline.h: out[0]
+// neighbours inside rc, [1] triplets, [2] FP32 flops of the symmetry
+// functions, [3] of the force chain, [4] of the network layers, [5] atoms.
+cmd_status cmd_force_nnp_work( cmd_force *f, double out[6] )
+{
+    namespace F = cmd::nnp_flops;
+    CMD_REQUIRE( f && f->kind == 2 && out, CMD_ERR_ARG, "bad argument" );
+    cmd_nnp *M = f->nnp;
+    double nb = 0.0, rad = 0.0, tri = 0.0, angp = 0.0, angg = 0.0, ang = 0.0;
+    if ( M->n_last > 0 && M->terms.p )
+    {
+        std::vector<int4> t( M->n_last );
+        CMD_HIP( hipMemcpy( t.data(), M->terms.p, sizeof( int4 ) * t.size(),
+                            hipMemcpyDeviceToHost ) );
+        for ( const int4 &a : t )
+        {
+            const cmd::NnpElemPlan &Q = M->plan.el[std::min( std::max( a.z, 0 ), M->plan.nelem - 1 )];
+            nb += a.x;
+            rad += (double)a.x * Q.nrg;
+            tri += a.y;
+            angp += (double)a.y * Q.nap;
+            angg += (double)a.y * Q.nag;
+            ang += (double)a.y * Q.nag * Q.nap;
+        }
+    }
+    out[0] = nb;
+    out[1] = tri;
+    out[2] = nb * F::SF_NB + rad * F::SF_RAD + tri * F::SF_TRIP + angp * F::SF_ANGP +
+             angg * F::SF_ANGG + ang * F::SF_ANG;
+    out[3] = nb * F::F_NB + rad * F::F_RAD + tri * F::F_TRIP + angp * F::F_ANGP +
+             angg * F::F_ANGG + ang * F::F_ANG;
+    // layers: the element with the largest network stands for all (the
+    // models here give every element the same shape)
+    double fwd = 0.0;
+    for ( int e = 0; e < M->plan.nelem; ++e )
+    {
+        const cmd::NnpElemPlan &Q = M->plan.el[e];
+        // nsf -> h1 -> h2 -> 1
+        const double per = 2.0 * ( (double)Q.nsf * Q.h1 + (double)Q.h1 * Q.h2 + Q.h2 );
+        fwd = std::max( fwd, per );
+    }
+    out[4] = F::MLP_PASSES * fwd * M->n_last;
+    out[5] = M->n_last;
     return CMD_OK;
 }
 

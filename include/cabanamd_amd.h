@@ -251,6 +251,14 @@ cmd_status cmd_force_nnp_create( const cmd_nnp_model *m, cmd_force **out );
 cmd_status cmd_force_nnp_fit_scaling( cmd_force *f, cmd_system *s,
                                       cmd_neigh *n );
 cmd_status cmd_force_nnp_model( const cmd_force *f, const cmd_nnp_model **m );
+/* Work of the last compute (no reference counterpart: the measurement of
+ * the NNP force against the FP32 roofline).  out[0] neighbours inside rc,
+ * [1] neighbour pairs with their third side inside rc, [2] FP32 flops of
+ * the symmetry functions, [3] of the force chain, [4] of the network
+ * layers (forward and backward), [5] atoms; flops under the per-term model
+ * of the engine's work model (add, multiply 1, fused multiply-add 2,
+ * transcendental 1). */
+cmd_status cmd_force_nnp_work( cmd_force *f, double out[6] );
 /* Per-atom G, dE/dG (ld floats per atom) and atomic energies of the last
  * compute (System_NNP's G, dEdG and E, system_nnp_1aosoa.h:19-57). */
 cmd_status cmd_force_nnp_atomic( cmd_force *f, cmd_system *s, int ld,
