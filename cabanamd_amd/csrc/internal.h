@@ -415,7 +415,9 @@ cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N );
 cmd_status neigh_build( cmd_neigh *N, cmd_system *S );
 cmd_status neigh_csr( cmd_neigh *N );
 size_t slab_entries( int n_local, int cap, int lanes );
-cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N );
+// Layout search for the fused full-list step; a layout scores its step
+// time plus its list build time over the `every` steps the list serves.
+cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N, int every );
 // tiles.hip
 cmd_status tiles_prepare_ghosts( cmd_system *S );
 // cmd_binning_create with the forces optionally left out of the

@@ -1538,7 +1538,7 @@ cmd_status lj_energy_virial( const cmd_force *F, cmd_system *S,
 // column 4 to 8 cells tall -- time the force kernel on it and keep the
 // fastest; the list is left built in that layout, and later builds keep at
 // least its force-kernel blocks per CU.
-cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
+cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N, int every )
 {
     struct Cand
     {
@@ -1575,9 +1575,12 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     CMD_TRY( S->v_alt.reserve( 3 * (size_t)S->cap ) );
     const size_t vbytes = sizeof( double ) * 3 * (size_t)S->cap;
     CMD_HIP( hipMemcpyAsync( S->v_alt.p, S->v.p, vbytes, hipMemcpyDeviceToDevice, S->stream ) );
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, b0, b1;
     CMD_HIP( hipEventCreate( &e0 ) );
     CMD_HIP( hipEventCreate( &e1 ) );
+    CMD_HIP( hipEventCreate( &b0 ) );
+    CMD_HIP( hipEventCreate( &b1 ) );
+    every = std::max( every, 1 );
     float best = 1e30f;
     Cand bestc = { 1, 0, 0, 0 };
     std::vector<float> tms( ncand, 1e30f );
@@ -1590,9 +1593,11 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
         N->tile_w = cd.W;
         N->tile_e = cd.E;
         N->tile_bw = cd.bw;
+        (void)hipEventRecord( b0, S->stream );
         st = neigh_build( N, S );
         if ( st != CMD_OK )
             break;
+        (void)hipEventRecord( b1, S->stream );
         if ( cd.E > 0 && !N->tiled )
             continue; // the tiled layout does not apply here
         constexpr int reps = 6;
@@ -1606,12 +1611,17 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
         (void)hipEventRecord( e1, S->stream );
         (void)hipEventSynchronize( e1 );
         (void)hipMemcpyAsync( S->v.p, S->v_alt.p, vbytes, hipMemcpyDeviceToDevice, S->stream );
-        float ms = 0.0f;
+        float ms = 0.0f, bms = 0.0f;
         (void)hipEventElapsedTime( &ms, e0, e1 );
+        (void)hipEventElapsedTime( &bms, b0, b1 );
         if ( std::getenv( "CMD_VERBOSE" ) )
-            std::fprintf( stderr, "# autotune lanes=%d tile=%dx%dx%d waves/block=%d: %.4f ms/step\n",
+            std::fprintf( stderr,
+                          "# autotune lanes=%d tile=%dx%dx%d waves/block=%d: %.4f ms/step, "
+                          "list build %.3f ms\n",
                           cd.G, N->tiled ? N->tg.W : 0, N->tiled ? N->tg.W : 0,
-                          N->tiled ? N->tg.E : 0, N->tiled ? N->tg.bw : 0, ms / reps );
+                          N->tiled ? N->tg.E : 0, N->tiled ? N->tg.bw : 0, ms / reps, bms );
+        // the step time plus the build's share of the steps it serves
+        ms += bms * reps / every;
         tms[ci] = ms;
         if ( ms < best )
         {
@@ -1640,6 +1650,8 @@ cmd_status autotune_lanes( cmd_force *F, cmd_system *S, cmd_neigh *N )
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
+    (void)hipEventDestroy( b0 );
+    (void)hipEventDestroy( b1 );
     CMD_HIP( hipMemcpyAsync( S->v.p, S->v_alt.p, vbytes, hipMemcpyDeviceToDevice, S->stream ) );
     CMD_TRY( st );
     F->lanes = bestc.G;

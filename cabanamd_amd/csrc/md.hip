@@ -155,7 +155,7 @@ static cmd_status rebuild( cmd_md *md )
     if ( md->retune )
     {
         md->retune = false;
-        CMD_TRY( autotune_lanes( md->F, S, md->N ) ); // leaves the list built
+        CMD_TRY( autotune_lanes( md->F, S, md->N, md->p.every ) ); // leaves the list built
         md->F->lanes = md->N->lanes;
     }
     else
