@@ -181,7 +181,7 @@ struct TileGrid
 {
     CellGrid g;
     int nc0, nc1; // interior columns: a in [1, n0-2], b in [1, n1-2]
-    int W;        // tile width: W x W interior columns per tile (1 or 2)
+    int W;        // tile width: W x W interior columns per tile (1, 2 or 3)
     int nt0, nt1; // tile columns: ceil(nc0 / W) x ceil(nc1 / W)
     int E, tpc;   // tile height in cells, tiles per column
     int ntiles;

@@ -522,8 +522,8 @@ cmd_status cmd_md_set_tile_width( cmd_md *md, int width )
 {
     CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
     CMD_REQUIRE( !md->inited, CMD_ERR_STATE, "the tile width is set before init" );
-    CMD_REQUIRE( width >= 0 && width <= 2, CMD_ERR_ARG,
-                 "tile width must be 0 (choose), 1 or 2 columns" );
+    CMD_REQUIRE( width >= 0 && width <= 3, CMD_ERR_ARG,
+                 "tile width must be 0 (choose), 1, 2 or 3 columns" );
     md->tile_w = width;
     return CMD_OK;
 }

@@ -567,8 +567,8 @@ cmd_status cmd_neigh_info( const cmd_neigh *n, int *n_rows, int64_t *nnz,
 cmd_status cmd_neigh_set_tile_width( cmd_neigh *n, int width )
 {
     CMD_REQUIRE( n, CMD_ERR_ARG, "neighbor list is NULL" );
-    CMD_REQUIRE( width >= 0 && width <= 2, CMD_ERR_ARG,
-                 "tile width must be 0 (choose), 1 or 2 columns" );
+    CMD_REQUIRE( width >= 0 && width <= 3, CMD_ERR_ARG,
+                 "tile width must be 0 (choose), 1, 2 or 3 columns" );
     if ( width != n->tile_w )
     {
         n->tile_w = width;

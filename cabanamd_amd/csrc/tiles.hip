@@ -4,7 +4,7 @@
 // forces as ForceLJ::compute_force_full (force_lj_cabana_neigh_impl.h:
 // 151-203); only the work decomposition and the list encoding differ.
 //
-// A tile is W x W interior columns (W = 1 or 2) of the binning grid (cells
+// A tile is W x W interior columns (W = 1, 2 or 3) of the binning grid (cells
 // >= the neighbour radius) over E cells [e0, e1).  Its atoms are the local
 // atoms of those cells, one contiguous run per atom column (locals are cell
 // sorted by create_binning), and every neighbour candidate lies in the
@@ -43,6 +43,17 @@ static constexpr int EMAX = 16;
 static constexpr int MAX_SLOTS = 2720; // largest staged neighbourhood (65 KB of records)
 static_assert( ( MAX_SLOTS + 1 ) * TILE_REC + 16 < 65536,
                "record offsets must fit the 16-bit list entries" );
+// The same bound for 16-byte FP32 records: the sentinel's byte offset
+// (slot MAX_SLOTS32) still fits the 16-bit entries.
+static constexpr int MAX_SLOTS32 = ( 65536 - 16 ) / TILE_REC32 - 1;
+static_assert( MAX_SLOTS32 * TILE_REC32 < 65536, "record offsets must fit the 16-bit list entries" );
+
+// Largest staged neighbourhood of a force tile for list entries addressing
+// records of `rec` bytes.
+static int max_force_slots( int rec )
+{
+    return rec == TILE_REC32 ? MAX_SLOTS32 : MAX_SLOTS;
+}
 
 // Per-tile runs, atom columns and staged count; slab waves per tile.
 // slots[0] gets the largest staged count of a tile, slots[1] the number of
@@ -1001,19 +1012,23 @@ static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int
 // LDS of the build block: coordinate planes of its 3 x 3 columns and the
 // sentinel slot, then the rows of its atom column (each with a spare slot
 // past its cap entries).
-static size_t build_lds( const cmd_neigh *N, int slot_cap, int rows )
+static size_t build_lds_cap( int G, int cap, int slot_cap, int rows )
 {
-    const int G = N->lanes;
-    const int R4 = ( ( N->cap + G - 1 ) / G + 3 ) / 4;
+    const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     return tile_planes_bytes( slot_cap ) +
            ( (size_t)std::max( rows, 1 ) * tile_row_stride( R4, G ) + 8 ) * sizeof( unsigned short );
 }
 
-// Tile height asked for: the set one, else 3 cells for 2 x 2 columns and 4
-// for single columns.
+static size_t build_lds( const cmd_neigh *N, int slot_cap, int rows )
+{
+    return build_lds_cap( N->lanes, N->cap, slot_cap, rows );
+}
+
+// Tile height asked for: the set one, else 4 cells for single columns and 3
+// for wider tiles.
 static int requested_height( const cmd_neigh *N, int W )
 {
-    return std::min( N->tile_e > 0 ? N->tile_e : ( W == 2 ? 3 : 4 ), EMAX - 2 );
+    return std::min( N->tile_e > 0 ? N->tile_e : ( W == 1 ? 4 : 3 ), EMAX - 2 );
 }
 
 // Whether a layout with tile maxima h and build LDS `lds` fits the device
@@ -1023,15 +1038,19 @@ static bool layout_ok( const cmd_neigh *N, const int h[6], size_t lds, size_t ha
                        size_t lds_cu, int E, bool *fits, int *blocks )
 {
     const size_t flds = force_lds( h[0], N->half != 0, N->rec );
-    *fits = h[0] <= MAX_SLOTS && lds <= hard && flds <= hard;
+    *fits = h[0] <= max_force_slots( N->rec ) && lds <= hard && flds <= hard;
     // blocks per CU of the force kernel (its static tables included)
     *blocks = (int)std::min( lds_cu / ( flds + 1024 ), (size_t)( 32 / N->tg.bw ) );
     return *fits && ( N->occ_blocks == 0 || *blocks >= N->occ_blocks || E == 1 );
 }
 
 // Sizes for the next build's launch: this build's, with headroom for the
-// atoms that move between columns meanwhile.
-static void record_spec( cmd_neigh *N, int W, int E, const int h[6], size_t hard )
+// atoms that move between columns meanwhile.  The row capacity grows ahead
+// of the rows: a start from a lattice melts, and its longest row keeps
+// growing for many rebuilds; a row past the capacity would send the next
+// build through the exact path (a second build).  mr is this build's
+// longest row.
+static void record_spec( cmd_neigh *N, int W, int E, const int h[6], int mr, size_t hard )
 {
     cmd_neigh::Spec &P = N->spec;
     P.valid = false;
@@ -1043,11 +1062,25 @@ static void record_spec( cmd_neigh *N, int W, int E, const int h[6], size_t hard
     P.E = E;
     P.bw_req = N->tile_bw;
     P.cap = N->cap;
+    if ( mr + std::max( 4, mr / 16 ) > P.cap )
+        P.cap = mr + std::max( 8, mr / 8 );
     P.slot_cap = std::min( h[3] + 1 + std::max( 16, h[3] / 32 ), MAX_SLOTS + 1 );
     P.rows = h[4] + std::max( 2, h[4] / 20 );
     P.slab_waves = h[5] + std::max( 64, h[5] / 50 );
-    P.lds = build_lds( N, P.slot_cap, P.rows );
+    P.lds = build_lds_cap( N->lanes, P.cap, P.slot_cap, P.rows );
     P.valid = P.lds <= hard;
+}
+
+// With CMD_VERBOSE set: the layout a tiled build chose and the sizes that
+// set the force kernel's LDS (largest staged neighbourhood) and blocks per CU.
+static void report_layout( const cmd_neigh *N, const char *how, int W, int E, const int h[6] )
+{
+    if ( !std::getenv( "CMD_VERBOSE" ) )
+        return;
+    std::fprintf( stderr,
+                  "# %s tiled build %dx%dx%d: tile slots max %d, 3x3 column slots max %d, "
+                  "column atoms max %d, slab waves %d, force blocks/CU %d, waves/block %d\n",
+                  how, W, W, E, h[0], h[3], h[4], h[5], N->tile_blocks, N->tg.bw );
 }
 
 cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
@@ -1113,7 +1146,8 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             N->tiled = true;
             N->built_rec = N->rec;
             N->gperm_view = S->gsort.perm.p;
-            record_spec( N, W, E, h, hard );
+            record_spec( N, W, E, h, mr, hard );
+            report_layout( N, "exact", W, E, h );
             return CMD_OK;
         }
         // rows overflowed: larger rows, and the LDS fit rechecked
@@ -1135,11 +1169,12 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     const cmd_neigh::Spec P = N->spec;
     N->spec.valid = false;
     if ( !P.valid || !S->bin_grid_valid || !S->xg_valid || P.G != N->lanes ||
-         P.half != N->half || P.cap != N->cap || P.bw_req != N->tile_bw )
+         P.half != N->half || P.cap < N->cap || P.bw_req != N->tile_bw )
         return CMD_OK;
     const int W = N->tile_w > 0 ? N->tile_w : 2;
     if ( P.W != W || P.E != requested_height( N, W ) )
         return CMD_OK;
+    N->cap = P.cap; // grown ahead of the rows (record_spec)
     CMD_TRY( tile_info_launch( N, S, P.W, P.E ) );
     CMD_TRY( N->tslab.reserve( tile_slab_entries( N, P.slab_waves ) ) );
     CMD_TRY( launch_build_g( N, S, P.slot_cap, P.rows, P.slab_waves, P.lds ) );
@@ -1172,7 +1207,8 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     N->built_rec = N->rec;
     N->gperm_view = S->gsort.perm.p;
     *built = true;
-    record_spec( N, P.W, P.E, h, hard );
+    record_spec( N, P.W, P.E, h, mr, hard );
+    report_layout( N, "speculative", P.W, P.E, h );
     return CMD_OK;
 }
 

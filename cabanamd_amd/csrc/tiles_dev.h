@@ -6,7 +6,7 @@
 
 namespace cmd
 {
-// A tile is W x W interior columns of the binning grid (W = 1 or 2) over E
+// A tile is W x W interior columns of the binning grid (W = 1, 2 or 3) over E
 // cells [e0, e1).  Its atoms are the local atoms of those cells: one
 // contiguous run per atom column (the cell-sorted order runs along a
 // column), taken in atom-column order c = ca * W + cb.  Every neighbour
@@ -15,7 +15,7 @@ namespace cmd
 // locals and one of cell-sorted ghosts (segments 2q and 2q + 1 of staged
 // column q = da * (W + 2) + db).  Staged columns past the padded grid (the
 // right edge of a tile whose second atom column does not exist) are empty.
-constexpr int TILE_WMAX = 2;
+constexpr int TILE_WMAX = 3;
 constexpr int TILE_QMAX = ( TILE_WMAX + 2 ) * ( TILE_WMAX + 2 ); // staged columns
 constexpr int TILE_CMAX = TILE_WMAX * TILE_WMAX;                 // atom columns
 static constexpr int TI_WORDS = 4 * TILE_QMAX + 2 * TILE_CMAX + 4;

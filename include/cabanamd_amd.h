@@ -159,7 +159,7 @@ cmd_status cmd_neigh_destroy( cmd_neigh *n );
  * force kernels follow the layout of the list they are given. */
 cmd_status cmd_neigh_set_lanes( cmd_neigh *n, int lanes );
 /* Tiled layout control (no reference counterpart; for tuning and tests):
- * tile height in binning cells (0 = choose) and tile width, 1 or 2 binning
+ * tile height in binning cells (0 = choose) and tile width, 1, 2 or 3 binning
  * columns per side (0 = choose: 2); the shape of the last tiled build
  * (0, 0 when the list is not tiled). */
 cmd_status cmd_neigh_set_tile( cmd_neigh *n, int height );

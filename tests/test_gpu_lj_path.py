@@ -200,7 +200,7 @@ def test_neighbor_lattice_exact(gpu, half):
     assert nl.max_neigh_guess >= int(np.diff(off).max())
 
 
-@pytest.mark.parametrize("width", [1, 2])
+@pytest.mark.parametrize("width", [1, 2, 3])
 @pytest.mark.parametrize("half", [False, True])
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8])
 def test_neighbor_on_binning_grid_exact(gpu, half, lanes, width):
@@ -227,14 +227,18 @@ def test_neighbor_on_binning_grid_exact(gpu, half, lanes, width):
     nl.create(s)
     off, idx = nl.get()
     lay = nl.layout()
-    assert lay["tiled"] == (lanes > 1)
-    assert lay["tile_width"] == (width if lanes > 1 else 0)
+    # a half list on 3 x 3 tiles stages FP64 records and reactions for 25
+    # columns, which may exceed a block's LDS at every height (then the
+    # global layout): only its rows are checked
+    if not (half and width == 3):
+        assert lay["tiled"] == (lanes > 1)
+        assert lay["tile_width"] == (width if lanes > 1 else 0)
     assert np.array_equal(s.download()[0], xg)
     assert np.array_equal(off, off_o)
     assert np.array_equal(idx, idx_o)
 
 
-@pytest.mark.parametrize("shape", ["1x0", "1x8", "2x0", "2x2"])
+@pytest.mark.parametrize("shape", ["1x0", "1x8", "2x0", "2x2", "3x0", "3x2"])
 @pytest.mark.parametrize("wrapped", [True, False])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("lanes", [2, 4, 8])
@@ -262,7 +266,13 @@ def test_lj_tiled_force(gpu, prec, lanes, wrapped, shape):
     assert lay["tiled"] == wrapped
     if wrapped:
         assert lay["tile_width"] == width
-        assert lay["tile_height"] == (tile or (3 if width == 2 else 4))
+        want = tile or (4 if width == 1 else 3)
+        if width == 3:
+            # 25 staged columns of FP64 records: the height is lowered until
+            # the tile fits a block's LDS
+            assert 1 <= lay["tile_height"] <= want
+        else:
+            assert lay["tile_height"] == want
     F = M.ForceLJ(s, precision=M.PREC_FP64 if prec == "fp64" else M.PREC_MIXED)
     F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
     F.compute(s, nl)
