@@ -425,6 +425,16 @@ class ForceNNP:
              G.ctypes.data_as(fp), D.ctypes.data_as(fp), E.ctypes.data_as(fp))
         return G, D, E
 
+    def work(self):
+        """Work model of the last compute: neighbours and triplets summed
+        over the atoms, FP32 flops of the symmetry functions, the force chain
+        and the layers, and the atom count."""
+        w = (C.c_double * 6)()
+        call("cmd_force_nnp_work", self._h, w)
+        return dict(neighbours=int(w[0]), triplets=int(w[1]),
+                    flop_symmetry_functions=w[2], flop_force_chain=w[3],
+                    flop_layers=w[4], atoms=int(w[5]))
+
     def name(self):
         return "Force:NNPHIP"
 

@@ -669,8 +669,11 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
 
 // NAG, NAP > 0: the angular (eta, rs) and (lambda, zeta) grid sizes of
 // every element, fixed at compile time (loops unrolled straight); 0: read
-// from the plan.
-template <int NAG, int NAP>
+// from the plan.  RS0: every element's angular rs are 0, so the Gaussian
+// argument r_ij^2 + r_ik^2 + r_jk^2 is shared by all eta (a kernel of its
+// own: with both paths in one kernel the registers of the longer one would
+// cost occupancy).
+template <int NAG, int NAP, bool RS0>
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
     int n_local, const double4 *__restrict__ x, const int64_t *__restrict__ off,
     const int *__restrict__ idx, const NnpPlan *__restrict__ P,
@@ -743,20 +746,40 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_sf(
             const float cs = ( A.x * B.x + A.y * B.y + A.z * B.z ) * ( Ea.x * Eb.x );
             float Pw[AP_MAX];
             ang_powers<false>( K, cs, Pw, nullptr );
-#pragma unroll
-            for ( int pp = 0; pp < AP_MAX; ++pp )
-                Pw[pp] *= F;
-#pragma unroll
-            for ( int g = 0; g < AG_MAX; ++g )
+            if constexpr ( RS0 )
             {
-                if ( g < K.nag )
-                {
-                    const float rs = K.rs[g];
-                    const float t1 = A.w - rs, t2 = B.w - rs, t3 = r3 - rs;
-                    const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
+                // one Gaussian argument, and the cutoff product joins the
+                // Gaussian (nag multiplies, not nap)
+                const float s2 = A.w * A.w + B.w * B.w + r3 * r3;
 #pragma unroll
-                    for ( int pp = 0; pp < AP_MAX; ++pp )
-                        acc[g * AP_MAX + pp] += Eg * Pw[pp];
+                for ( int g = 0; g < AG_MAX; ++g )
+                {
+                    if ( g < K.nag )
+                    {
+                        const float EgF = __expf( -K.eta[g] * s2 ) * F;
+#pragma unroll
+                        for ( int pp = 0; pp < AP_MAX; ++pp )
+                            acc[g * AP_MAX + pp] += EgF * Pw[pp];
+                    }
+                }
+            }
+            else
+            {
+#pragma unroll
+                for ( int pp = 0; pp < AP_MAX; ++pp )
+                    Pw[pp] *= F;
+#pragma unroll
+                for ( int g = 0; g < AG_MAX; ++g )
+                {
+                    if ( g < K.nag )
+                    {
+                        const float rs = K.rs[g];
+                        const float t1 = A.w - rs, t2 = B.w - rs, t3 = r3 - rs;
+                        const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
+#pragma unroll
+                        for ( int pp = 0; pp < AP_MAX; ++pp )
+                            acc[g * AP_MAX + pp] += Eg * Pw[pp];
+                    }
                 }
             }
             },
@@ -1013,7 +1036,7 @@ __global__ __launch_bounds__( 256 ) void k_nnp_mlp_f32( int n_local,
 }
 
 // ---- forces
-template <int NAG, int NAP>
+template <int NAG, int NAP, bool RS0>
 __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     int n_local, int fcap, const double4 *__restrict__ x,
     const int64_t *__restrict__ off, const int *__restrict__ idx,
@@ -1085,8 +1108,8 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
         fiz += vz;
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
-    // angular: coefficients in grid order, read from LDS per pair (keeps
-    // 64 registers free for occupancy)
+    // angular: coefficients in grid order in LDS (the generic grid reads
+    // them per pair, which keeps 64 registers free for occupancy)
     {
         float *cg = reinterpret_cast<float *>( sCg[wv] );
         for ( int k = lane; k < NA_MAX; k += 64 )
@@ -1097,6 +1120,22 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     }
     __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
     const float4 *cg4 = sCg[wv];
+    // with the grid fixed at compile time the coefficients are read once per
+    // atom as wave-uniform values (scalar registers, the excess spilled to
+    // lanes of vector registers) instead of from LDS per triplet: 4 % off
+    // the kernel at 192,000 atoms
+    float cgr[( NAG > 0 ? NAG : 1 ) * ( NAP > 0 ? NAP : 1 )];
+    if constexpr ( NAG > 0 && NAP > 0 )
+    {
+        const float *cgf = reinterpret_cast<const float *>( sCg[wv] );
+#pragma unroll
+        for ( int g = 0; g < NAG; ++g )
+#pragma unroll
+            for ( int pp = 0; pp < NAP; ++pp )
+                cgr[g * NAP + pp] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readfirstlane(
+                               __builtin_bit_cast( int, cgf[g * AP_MAX + pp] ) ) );
+    }
     // angular: each lane takes a contiguous block of the compacted (a-major)
     // pair list, so the reaction on a stays in registers until a changes;
     // the one on q goes to LDS.
@@ -1130,15 +1169,22 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
         float Pw[AP_MAX], Dw[AP_MAX];
         ang_powers<true>( K, cs, Pw, Dw );
         float Gs = 0.f, As = 0.f, B1 = 0.f, B2 = 0.f, B3 = 0.f;
-#pragma unroll
-        for ( int g = 0; g < AG_MAX; ++g )
+        // sum_p c_gp P_p and sum_p c_gp D_p of one (eta, rs)
+        auto cdots = [&]( int g, float &sp, float &sd )
         {
-            if ( g < K.nag )
+            sp = 0.f;
+            sd = 0.f;
+            if constexpr ( NAG > 0 && NAP > 0 )
             {
-                const float rs = K.rs[g];
-                const float t1 = r1 - rs, t2 = r2 - rs, t3 = r3 - rs;
-                const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
-                float sp = 0.f, sd = 0.f;
+#pragma unroll
+                for ( int pp = 0; pp < NAP; ++pp )
+                {
+                    sp = fmaf( cgr[g * NAP + pp], Pw[pp], sp );
+                    sd = fmaf( cgr[g * NAP + pp], Dw[pp], sd );
+                }
+            }
+            else
+            {
 #pragma unroll
                 for ( int q4 = 0; q4 < AP_MAX / 4; ++q4 )
                 {
@@ -1151,13 +1197,52 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
                         sd = fmaf( cv[u], Dw[q4 * 4 + u], sd );
                     }
                 }
-                const float w = sp * Eg; // sum_p c P E
-                Gs += w;
-                As = fmaf( sd, Eg, As );
-                const float m2 = -2.f * K.eta[g] * w;
-                B1 = fmaf( m2, t1, B1 );
-                B2 = fmaf( m2, t2, B2 );
-                B3 = fmaf( m2, t3, B3 );
+            }
+        };
+        if constexpr ( RS0 )
+        {
+            // one Gaussian argument, and the distance gradients share one
+            // factor, sum_g -2 eta_g w_g
+            const float s2 = r1 * r1 + r2 * r2 + r3 * r3;
+            float M2 = 0.f;
+#pragma unroll
+            for ( int g = 0; g < AG_MAX; ++g )
+            {
+                if ( g < K.nag )
+                {
+                    const float Eg = __expf( -K.eta[g] * s2 );
+                    float sp, sd;
+                    cdots( g, sp, sd );
+                    const float w = sp * Eg; // sum_p c P E
+                    Gs += w;
+                    As = fmaf( sd, Eg, As );
+                    M2 = fmaf( -2.f * K.eta[g], w, M2 );
+                }
+            }
+            B1 = M2 * r1;
+            B2 = M2 * r2;
+            B3 = M2 * r3;
+        }
+        else
+        {
+#pragma unroll
+            for ( int g = 0; g < AG_MAX; ++g )
+            {
+                if ( g < K.nag )
+                {
+                    const float rs = K.rs[g];
+                    const float t1 = r1 - rs, t2 = r2 - rs, t3 = r3 - rs;
+                    const float Eg = __expf( -K.eta[g] * ( t1 * t1 + t2 * t2 + t3 * t3 ) );
+                    float sp, sd;
+                    cdots( g, sp, sd );
+                    const float w = sp * Eg; // sum_p c P E
+                    Gs += w;
+                    As = fmaf( sd, Eg, As );
+                    const float m2 = -2.f * K.eta[g] * w;
+                    B1 = fmaf( m2, t1, B1 );
+                    B2 = fmaf( m2, t2, B2 );
+                    B3 = fmaf( m2, t3, B3 );
+                }
             }
         }
         // gradients of sum_s c_s T_s with respect to A = x_a - x_i and
@@ -1308,16 +1393,31 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
     CMD_TRY( M->terms.reserve( std::max( n, 1 ) ) );
     M->n_last = n;
     const int nbw = ( n + NNP_TPB / 64 - 1 ) / ( NNP_TPB / 64 );
-    // the angular grid of every element, for the kernels compiled for it
-    bool g58 = M->plan.nelem > 0;
+    // the angular grid of every element, for the kernels compiled for it:
+    // its size (5 x 8) and whether all its rs are 0
+    bool g58 = M->plan.nelem > 0, rs0 = true;
     for ( int e = 0; e < M->plan.nelem; ++e )
-        g58 = g58 && M->plan.el[e].nag == 5 && M->plan.el[e].nap == 8;
+    {
+        const NnpElemPlan &Q = M->plan.el[e];
+        g58 = g58 && Q.nag == 5 && Q.nap == 8;
+        for ( int g = 0; g < Q.nag; ++g )
+            rs0 = rs0 && Q.a_rs[g] == 0.f;
+    }
+#define CMD_SF( A, P, R )                                                                 \
+    k_nnp_sf<A, P, R><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p, M->dplan.p, \
+                                                M->G.p, M->flag.p, M->terms.p )
     if ( g58 )
-        k_nnp_sf<5, 8><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
-                                                 M->dplan.p, M->G.p, M->flag.p, M->terms.p );
+    {
+        if ( rs0 )
+            CMD_SF( 5, 8, true );
+        else
+            CMD_SF( 5, 8, false );
+    }
+    else if ( rs0 )
+        CMD_SF( 0, 0, true );
     else
-        k_nnp_sf<0, 0><<<nbw, NNP_TPB, 0, st>>>( n, S->x.p, N->offsets.p, N->idx.p,
-                                                 M->dplan.p, M->G.p, M->flag.p, M->terms.p );
+        CMD_SF( 0, 0, false );
+#undef CMD_SF
     CMD_LAUNCH_CHECK();
     if ( M->prec == CMD_PREC_MIXED )
     {
@@ -1340,14 +1440,21 @@ cmd_status nnp_compute( cmd_nnp *M, cmd_system *S, cmd_neigh *N )
                                                          M->Ei.p );
         CMD_LAUNCH_CHECK();
     }
+#define CMD_FORCE( A, P, R )                                                               \
+    k_nnp_force<A, P, R><<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p, N->idx.p, \
+                                                   M->dplan.p, M->dEdG.p, S->f.p, M->flag.p )
     if ( g58 )
-        k_nnp_force<5, 8><<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p,
-                                                    N->idx.p, M->dplan.p, M->dEdG.p,
-                                                    S->f.p, M->flag.p );
+    {
+        if ( rs0 )
+            CMD_FORCE( 5, 8, true );
+        else
+            CMD_FORCE( 5, 8, false );
+    }
+    else if ( rs0 )
+        CMD_FORCE( 0, 0, true );
     else
-        k_nnp_force<0, 0><<<nbw, NNP_TPB, 0, st>>>( n, S->cap, S->x.p, N->offsets.p,
-                                                    N->idx.p, M->dplan.p, M->dEdG.p,
-                                                    S->f.p, M->flag.p );
+        CMD_FORCE( 0, 0, false );
+#undef CMD_FORCE
     CMD_LAUNCH_CHECK();
     int fl = 0;
     CMD_HIP( hipMemcpyAsync( &fl, M->flag.p, sizeof( int ),
@@ -1945,29 +2052,37 @@ cmd_status cmd_force_nnp_work( cmd_force *f, double out[6] )
     namespace F = cmd::nnp_flops;
     CMD_REQUIRE( f && f->kind == 2 && out, CMD_ERR_ARG, "bad argument" );
     cmd_nnp *M = f->nnp;
-    double nb = 0.0, rad = 0.0, tri = 0.0, angp = 0.0, angg = 0.0, ang = 0.0;
+    double nb = 0.0, rad = 0.0, tri = 0.0, ang = 0.0, sf = 0.0, fc = 0.0;
     if ( M->n_last > 0 && M->terms.p )
     {
         std::vector<int4> t( M->n_last );
         CMD_HIP( hipMemcpy( t.data(), M->terms.p, sizeof( int4 ) * t.size(),
                             hipMemcpyDeviceToHost ) );
+        // the kernels' path: the shorter one when every element's angular
+        // rs are 0 (the launch in nnp_compute)
+        bool rs0 = true;
+        for ( int e = 0; e < M->plan.nelem; ++e )
+            for ( int g = 0; g < M->plan.el[e].nag; ++g )
+                rs0 = rs0 && M->plan.el[e].a_rs[g] == 0.f;
         for ( const int4 &a : t )
         {
-            const cmd::NnpElemPlan &Q = M->plan.el[std::min( std::max( a.z, 0 ), M->plan.nelem - 1 )];
+            const int e = std::min( std::max( a.z, 0 ), M->plan.nelem - 1 );
+            const cmd::NnpElemPlan &Q = M->plan.el[e];
+            const double tr = a.y, angp = tr * Q.nap, angg = tr * Q.nag;
             nb += a.x;
             rad += (double)a.x * Q.nrg;
-            tri += a.y;
-            angp += (double)a.y * Q.nap;
-            angg += (double)a.y * Q.nag;
-            ang += (double)a.y * Q.nag * Q.nap;
+            tri += tr;
+            ang += tr * Q.nag * Q.nap;
+            sf += rs0 ? tr * F::SF_TRIP_RS0 + angp * F::SF_ANGP_RS0 + angg * F::SF_ANGG_RS0
+                      : tr * F::SF_TRIP + angp * F::SF_ANGP + angg * F::SF_ANGG;
+            fc += rs0 ? tr * F::F_TRIP_RS0 + angp * F::F_ANGP + angg * F::F_ANGG_RS0
+                      : tr * F::F_TRIP + angp * F::F_ANGP + angg * F::F_ANGG;
         }
     }
     out[0] = nb;
     out[1] = tri;
-    out[2] = nb * F::SF_NB + rad * F::SF_RAD + tri * F::SF_TRIP + angp * F::SF_ANGP +
-             angg * F::SF_ANGG + ang * F::SF_ANG;
-    out[3] = nb * F::F_NB + rad * F::F_RAD + tri * F::F_TRIP + angp * F::F_ANGP +
-             angg * F::F_ANGG + ang * F::F_ANG;
+    out[2] = nb * F::SF_NB + rad * F::SF_RAD + sf + ang * F::SF_ANG;
+    out[3] = nb * F::F_NB + rad * F::F_RAD + fc + ang * F::F_ANG;
     // layers: the element with the largest network stands for all (the
     // models here give every element the same shape)
     double fwd = 0.0;

@@ -64,6 +64,20 @@ constexpr double F_ANGG = 19.0;
 // (triplet, function): the coefficient-weighted P and dP sums (2 fma)
 constexpr double F_ANG = 4.0;
 
+// When every element's angular rs are 0 both kernels take a shorter path:
+// the Gaussian argument r_ij^2 + r_ik^2 + r_jk^2 is formed once per triplet
+// (5) and shared by every eta.
+// k_nnp_sf: (triplet, (eta, rs)): * -eta (1), e^x (2), * F (1); the
+// (lambda, zeta) factor loses its * F (3)
+constexpr double SF_TRIP_RS0 = SF_TRIP + 5.0;
+constexpr double SF_ANGP_RS0 = 3.0;
+constexpr double SF_ANGG_RS0 = 4.0;
+// k_nnp_force: (triplet, (eta, rs)): e^x with * -eta (3), w (1), G sum (1),
+// dP sum (2), the shared gradient factor (2); per triplet that factor times
+// the three distances (3)
+constexpr double F_TRIP_RS0 = F_TRIP + 5.0 + 3.0;
+constexpr double F_ANGG_RS0 = 9.0;
+
 // network: forward 2 (nsf h1 + h1 h2 + h2) per atom, the backward pass
 // (dE/dG through the same layers) twice that
 constexpr double MLP_PASSES = 3.0;

@@ -20,13 +20,17 @@ from nnp_models import FIXTURE, random_model, to_oracle
 pytestmark = pytest.mark.gpu
 
 
-def _grid_model(seed=3, rc=5.0):
+def _grid_model(seed=3, rc=5.0, ang_rs=0.0):
     """random_model without its element-specific angular function, so the
-    functions form the grid the kernels evaluate."""
+    functions form the grid the kernels evaluate; ang_rs moves every angular
+    function's rs (0: the kernels' shared-argument path, else the general
+    one)."""
     m = random_model(seed=seed, rc=rc)
     for e in range(2):
         keep = [k for k, s in enumerate(m["sf"][e]) if not (s[0] == 3 and s[1] >= 0)]
-        m["sf"][e] = [m["sf"][e][k] for k in keep]
+        m["sf"][e] = [m["sf"][e][k] if m["sf"][e][k][0] == 2 else
+                      m["sf"][e][k][:4] + (ang_rs,) + m["sf"][e][k][5:]
+                      for k in keep]
         m["gmin"][e] = m["gmin"][e][keep]
         m["gmax"][e] = m["gmax"][e][keep]
         m["W"][e][0] = m["W"][e][0][:, keep]
@@ -86,10 +90,11 @@ def _check(r, nsf, prec):
     return err
 
 
+@pytest.mark.parametrize("ang_rs", [0.0, 0.4])
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_nnp_random_model_periodic(gpu, prec):
+def test_nnp_random_model_periodic(gpu, prec, ang_rs):
     M = gpu
-    m = _grid_model()
+    m = _grid_model(ang_rs=ang_rs)
     x, t, _ = M.nnp_lattice((6, 6, 6), spacing=2.0, jitter=0.1, seed=11)
     L = [12.0] * 3
     s, comm, nl = _system(M, x, t, L, 5.3)
@@ -97,9 +102,47 @@ def test_nnp_random_model_periodic(gpu, prec):
                    M.PREC_FP64 if prec == "fp32" else M.PREC_MIXED)
     r = _run(M, s, comm, nl, F, m)
     err = _check(r, len(m["sf"][0]), prec)
-    print(f"random model {prec}: force rel L2 {err:.2e}")
+    print(f"random model {prec}, angular rs {ang_rs}: force rel L2 {err:.2e}")
     # net force of a periodic system vanishes
     assert np.abs(r["f"].sum(axis=0)).max() < 1e-3 * np.abs(r["f"]).max()
+
+
+def test_nnp_work_counts(gpu):
+    """The work model's term counts (cmd_force_nnp_work): neighbours of each
+    local atom inside the model's cutoff and neighbour pairs (j, k) with r_jk
+    inside it, summed over the atoms, equal a brute-force FP64 count on the
+    same positions (pairs within 1e-4 of the cutoff, which the kernels'
+    FP32 distances may class either way, are the only slack)."""
+    M = gpu
+    rc = 5.0
+    m = _grid_model(rc=rc)
+    x, t, _ = M.nnp_lattice((6, 6, 6), spacing=2.0, jitter=0.1, seed=11)
+    L = [12.0] * 3
+    s, comm, nl = _system(M, x, t, L, 5.3)
+    F = M.ForceNNP(_engine_model(M, m), M.PREC_FP64)
+    F.compute(s, nl)
+    w = F.work()
+    xg = s.download()[0]
+    n = s.N_local
+    assert w["atoms"] == n
+    nb = tri = slack_nb = slack_tri = 0
+    for i in range(n):
+        d = np.linalg.norm(xg - xg[i], axis=1)
+        d[i] = np.inf
+        js = np.nonzero(d < rc)[0]
+        nb += len(js)
+        slack_nb += int(np.sum(np.abs(d - rc) < 1e-4))
+        p = xg[js]
+        djk = np.linalg.norm(p[:, None, :] - p[None, :, :], axis=2)
+        iu = np.triu_indices(len(js), 1)
+        tri += int(np.sum(djk[iu] < rc))
+        slack_tri += int(np.sum(np.abs(djk[iu] - rc) < 1e-4))
+    assert nb > 0 and tri > 0
+    assert abs(w["neighbours"] - nb) <= slack_nb, (w["neighbours"], nb, slack_nb)
+    assert abs(w["triplets"] - tri) <= slack_tri + 60 * slack_nb, (w["triplets"], tri)
+    # every term count enters the flops with a positive weight
+    assert w["flop_symmetry_functions"] > 20.0 * nb
+    assert w["flop_force_chain"] > w["flop_symmetry_functions"]
 
 
 def test_nnp_benchmark_model_after_scaling_fit(gpu):
