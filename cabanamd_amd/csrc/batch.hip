@@ -1210,8 +1210,11 @@ cmd_status cmd_batch_run( cmd_batch *b, int nsteps )
         return t < s1 && t % p.every != 0 && ( t + 1 ) % p.every != 0 &&
                !( p.thermo_rate > 0 && t % p.thermo_rate == 0 );
     };
-    // parameters uploaded per launch for several types would be a host copy
-    const bool graphs = b->graphs && b->F->ntypes == 1;
+    // several types: the coefficient tables go to the device now, so the
+    // captured steps launch with no host copy (a batch's coefficients are
+    // set at its creation and do not change)
+    CMD_TRY( lj_params_upload( b->F, st ) );
+    const bool graphs = b->graphs;
     for ( int s = s0; s <= s1; ++s )
     {
         b->step = s;

@@ -350,6 +350,10 @@ struct cmd_force
 namespace cmd
 {
 LJParams lj_params( const cmd_force *f );
+// The device copy of a multi-type force's coefficient tables, uploaded now
+// if they changed since the last upload (a copy and a wait on st); the
+// force kernels then use it without a host copy, so they may be captured.
+cmd_status lj_params_upload( const cmd_force *f, hipStream_t st );
 
 // A ghost image entry packs its sorted slot (below 2^26) and its 6-bit
 // shift code.

@@ -1107,6 +1107,12 @@ static cmd_status upload_params( const cmd_force *F, hipStream_t st,
     return CMD_OK;
 }
 
+cmd_status lj_params_upload( const cmd_force *F, hipStream_t st )
+{
+    LJParams *d = nullptr;
+    return F->ntypes > 1 ? upload_params( F, st, &d ) : CMD_OK;
+}
+
 static PairCoef one_coef( const LJParams &P )
 {
     PairCoef c;

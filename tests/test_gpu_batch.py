@@ -184,6 +184,43 @@ def test_batch_graph_replays_equal_eager(gpu, prec):
     assert e.graph_steps() == 0
 
 
+def test_batch_graphs_two_types(gpu):
+    """Two atom types (masses 1 and 2, their own cross coefficients): the
+    coefficient tables go to the device once, so the plain steps run as
+    graph replays here too, and every system's x, v, f and thermo rows are
+    bit-identical to the batch launched kernel by kernel."""
+    M = gpu
+
+    def params():
+        p = _params(M, thermo=10)
+        p.ntypes = 2
+        p.mass[1] = 2.0
+        p.ncoeff = 3
+        for k, (a, b, eps, sig) in enumerate(((1, 1, 1.0, 1.0), (1, 2, 0.8, 1.1),
+                                              (2, 2, 1.2, 0.95))):
+            p.coeff_i[k], p.coeff_j[k] = a, b
+            p.coeff_eps[k], p.coeff_sigma[k], p.coeff_cut[k] = eps, sig, 2.5
+        return p
+
+    g = M.Batch(params(), M.PREC_FP64, lanes=2, graphs=True)
+    e = M.Batch(params(), M.PREC_FP64, lanes=2, graphs=False)
+    hs = []
+    for cells, rho, temp, seed in SYSTEMS[:2]:
+        x, v, t, i, lo, hi = M.fcc_system(cells, rho, temp, seed)
+        t = t.copy()
+        t[::3] = 1
+        hs.append((g.add(x, t, lo, hi, v=v, id=i), e.add(x, t, lo, hi, v=v, id=i)))
+    for k in (27, 30):
+        g.run(k)
+        e.run(k)
+        for a, b in hs:
+            for u, w in zip(g.download(a), e.download(b)):
+                assert np.array_equal(u, w)
+            assert np.array_equal(g.thermo(a), e.thermo(b))
+    assert g.graph_steps() > 20
+    assert e.graph_steps() == 0
+
+
 def test_batch_precision_switch_with_graphs(gpu):
     """Switching precision between runs takes effect at once, also for the
     steps that replay captured graphs: a batch with graphs and one launched
