@@ -103,9 +103,25 @@ def _host_threads():
     return int(os.environ.get("OMP_NUM_THREADS", threads))
 
 
-def _oracle_run(ncells, steps, thermo=100, half=False):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _oracle_run(ncells, steps, thermo=100, half=False, timers=False):
     """The FP64 OpenMP oracle's MD loop on the LJ workload with Gaussian
-    velocities: (atom-steps/s of the step loop, thermo rows)."""
+    velocities: (atom-steps/s of the step loop, seconds, thermo rows), and
+    with timers its phase times (the reference's T_Force, T_Neigh, T_Comm,
+    T_Int, T_Other split, cabanamd_impl.h:404-425)."""
+    # threads bound one per core for the OpenMP runtime the oracle loads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     md = O.MD(O.lj_params(ncells=ncells, thermo=thermo, nsteps=steps,
@@ -114,14 +130,17 @@ def _oracle_run(ncells, steps, thermo=100, half=False):
     t0 = time.perf_counter()
     md.run(steps)
     dt = time.perf_counter() - t0
-    return md.n_local * steps / dt, dt, md.thermo()
+    out = (md.n_local * steps / dt, dt, md.thermo())
+    if timers:
+        out += ({k: float(v) for k, v in md.timers().items()},)
+    return out
 
 
 def _cpu_baseline(ncells, steps):
-    rate, dt, _ = _oracle_run(ncells, steps)
+    rate, dt, _, split = _oracle_run(ncells, steps, timers=True)
     n = 4 * ncells[0] * ncells[1] * ncells[2]
     return dict(value=rate, unit="atom-steps/s", cores=_host_threads(),
-                kind="port",
+                kind="port", cpu_model=_cpu_model(), split_s=split,
                 sample=(f"FP64 OpenMP oracle (restatement of the reference "
                         f"algorithm), same workload, {n} atoms, {steps} "
                         f"steps incl. {steps // 20} list rebuilds, "
@@ -156,9 +175,10 @@ def _c1_case(M, cpu=True, drift_steps=1000):
                          e_final=float(e[-1]))
         del md
     if cpu:
-        rate, dt, _ = _oracle_run(nc, 100)
+        rate, dt, _, split = _oracle_run(nc, 100, timers=True)
         out["cpu_oracle"] = dict(atom_steps_per_s=rate, seconds=dt,
-                                 cores=_host_threads(), kind="port")
+                                 cores=_host_threads(), kind="port",
+                                 cpu_model=_cpu_model(), split_s=split)
         _, _, th = _oracle_run(nc, drift_steps)
         e = th[:, 3]
         out["cpu_oracle"].update(drift_steps=drift_steps,
