@@ -590,7 +590,10 @@ __device__ __forceinline__ void pair_of( int p, int n, int &a, int &q )
 // into buf (TCAP entries per wave, a | q << 16), then handed to body(a, q)
 // with every lane busy (lane l takes entries l*per .. l*per + per - 1).
 // Used where the body keeps many accumulators in registers.
-template <class Body>
+// With PAIRS the body takes two pairs at once, (a0, q0) and (a1, q1), the
+// second valid when ok1 (else a repeat of the first): the two gradient
+// chains are independent, so their latencies overlap.
+template <bool PAIRS = false, class Body>
 __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
                                              unsigned *buf, Body body, int *count = nullptr )
 {
@@ -654,13 +657,31 @@ __device__ __forceinline__ void for_triplets( const NbLds &L, int n, float rc2,
         if ( count )
             *count += cnt;
         const int per = ( cnt + 63 ) >> 6;
-        for ( int j = 0; j < per; ++j )
+        if constexpr ( PAIRS )
         {
-            const int e = lane * per + j;
-            if ( e < cnt )
+            for ( int j = 0; j < per; j += 2 )
             {
-                const unsigned c = buf[e];
-                body( (int)( c & 0xffffu ), (int)( c >> 16 ) );
+                const int e = lane * per + j;
+                if ( e < cnt )
+                {
+                    const unsigned c0 = buf[e];
+                    const bool ok1 = j + 1 < per && e + 1 < cnt;
+                    const unsigned c1 = ok1 ? buf[e + 1] : c0;
+                    body( (int)( c0 & 0xffffu ), (int)( c0 >> 16 ), (int)( c1 & 0xffffu ),
+                          (int)( c1 >> 16 ), ok1 );
+                }
+            }
+        }
+        else
+        {
+            for ( int j = 0; j < per; ++j )
+            {
+                const int e = lane * per + j;
+                if ( e < cnt )
+                {
+                    const unsigned c = buf[e];
+                    body( (int)( c & 0xffffu ), (int)( c >> 16 ) );
+                }
             }
         }
         __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );
@@ -1141,21 +1162,11 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
     // the one on q goes to LDS.
     int cur = -1;
     float ax = 0.f, ay = 0.f, az = 0.f; // reaction on row atom cur
-    float4 A = make_float4( 0.f, 0.f, 0.f, 1.f ), Ea = make_float4( 1.f, 0.f, 0.f, 0.f );
-    for_triplets( L, n, rc2, tbuf[wv], [&]( int a, int q ) {
-        if ( a != cur )
-        {
-            if ( cur >= 0 )
-            {
-                atomicAdd( &sF[wv][cur][0], ax );
-                atomicAdd( &sF[wv][cur][1], ay );
-                atomicAdd( &sF[wv][cur][2], az );
-            }
-            cur = a;
-            ax = ay = az = 0.f;
-            A = L.d[a];
-            Ea = L.e[a];
-        }
+    // gradients of sum_s c_s T_s of the triplet (i, a, q) with respect to
+    // A = x_a - x_i (ga) and B = x_q - x_i (gb): no side effects, so two of
+    // them run interleaved
+    auto grad = [&]( int a, int q, float ( &ga )[3], float ( &gb )[3] ) {
+        const float4 A = L.d[a], Ea = L.e[a];
         const float4 B = L.d[q], Eb = L.e[q];
         const float cx = B.x - A.x, cy = B.y - A.y, cz = B.z - A.z;
         const float r1 = A.w, r2 = B.w;
@@ -1245,10 +1256,8 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
                 }
             }
         }
-        // gradients of sum_s c_s T_s with respect to A = x_a - x_i and
-        // B = x_q - x_i (T = pref P E F, r3 = |B - A|)
+        // (T = pref P E F, r3 = |B - A|)
         const float dFa1 = d1 * f2 * f3, dF3 = f1 * f2 * d3, dFb2 = f1 * d2 * f3;
-        float ga[3], gb[3];
         const float Av[3] = { A.x, A.y, A.z }, Bv[3] = { B.x, B.y, B.z },
                     Cv[3] = { cx, cy, cz };
 #pragma unroll
@@ -1260,6 +1269,21 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
             ga[k] = F * ( As * dca + B1 * u1 - B3 * u3 ) + Gs * ( dFa1 * u1 - dF3 * u3 );
             gb[k] = F * ( As * dcb + B2 * u2 + B3 * u3 ) + Gs * ( dFb2 * u2 + dF3 * u3 );
         }
+    };
+    // the reactions of one triplet: on a in registers while the row atom
+    // stays the same (the stream is a-major), on q through LDS
+    auto apply = [&]( int a, int q, const float ( &ga )[3], const float ( &gb )[3] ) {
+        if ( a != cur )
+        {
+            if ( cur >= 0 )
+            {
+                atomicAdd( &sF[wv][cur][0], ax );
+                atomicAdd( &sF[wv][cur][1], ay );
+                atomicAdd( &sF[wv][cur][2], az );
+            }
+            cur = a;
+            ax = ay = az = 0.f;
+        }
         ax -= ga[0];
         ay -= ga[1];
         az -= ga[2];
@@ -1269,7 +1293,24 @@ __global__ __launch_bounds__( NNP_TPB ) void k_nnp_force(
         fix += ga[0] + gb[0];
         fiy += ga[1] + gb[1];
         fiz += ga[2] + gb[2];
-    } );
+    };
+    if constexpr ( NAG > 0 && NAP > 0 )
+        for_triplets<true>( L, n, rc2, tbuf[wv],
+                            [&]( int a0, int q0, int a1, int q1, bool ok1 ) {
+                                float ga0[3], gb0[3], ga1[3], gb1[3];
+                                grad( a0, q0, ga0, gb0 );
+                                grad( a1, q1, ga1, gb1 );
+                                apply( a0, q0, ga0, gb0 );
+                                if ( ok1 )
+                                    apply( a1, q1, ga1, gb1 );
+                            } );
+    else
+        // the generic grid's registers leave no room for a second chain
+        for_triplets( L, n, rc2, tbuf[wv], [&]( int a, int q ) {
+            float ga[3], gb[3];
+            grad( a, q, ga, gb );
+            apply( a, q, ga, gb );
+        } );
     if ( cur >= 0 )
     {
         atomicAdd( &sF[wv][cur][0], ax );
