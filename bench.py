@@ -34,6 +34,8 @@ HBM_PEAK_GBS = 8000.0
 LJ_FULL_BYTES_PER_ATOM = 364
 FUSED_NVE_BYTES_PER_ATOM = 72
 ALG_BYTES_PER_ATOM = LJ_FULL_BYTES_PER_ATOM + FUSED_NVE_BYTES_PER_ATOM
+# the LJ legs time one fused force launch in this many steps with HIP events
+PROFILE_EVERY = 4
 
 
 def _force_error(md):
@@ -75,7 +77,10 @@ def _variant(M, ncells, steps, warmup, half, prec, lanes=0, accuracy=False):
     if warmup:
         md.run(warmup)
     M._lib.call("cmd_device_synchronize")
-    md.profile(True)
+    # HIP events around one fused force launch in PROFILE_EVERY: each event
+    # is a marker that idles the device a few microseconds, so timing every
+    # launch would slow every step it times
+    md.profile(PROFILE_EVERY)
     t0 = time.perf_counter()
     md.run(steps)
     M._lib.call("cmd_device_synchronize")
@@ -638,7 +643,8 @@ def main():
             "traffic": traffic,
             "kernel": "k_tile_force (LDS-tiled full-list LJ fused with "
                       "the NVE half-kicks, drift and wrap), mean of "
-                      f"{h['kernel_launches']} launches = "
+                      f"{h['kernel_launches']} launches (one in "
+                      f"{PROFILE_EVERY} of the timed steps) = "
                       f"{h['kernel_mean_ms']:.4f} ms, "
                       f"{ALG_BYTES_PER_ATOM} algorithmic B/atom"},
         "cpu_baseline": cpu,

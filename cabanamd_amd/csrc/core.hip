@@ -54,6 +54,27 @@ cmd_status fill_i32( int *p, int v, size_t n, hipStream_t st )
     return CMD_OK;
 }
 
+__global__ void k_zero2( int *__restrict__ p, int *__restrict__ q, size_t n )
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for ( ; i < n; i += stride )
+    {
+        p[i] = 0;
+        q[i] = 0;
+    }
+}
+
+cmd_status zero2_i32( int *p, int *q, size_t n, hipStream_t st )
+{
+    if ( n == 0 )
+        return CMD_OK;
+    int nb = (int)std::min<size_t>( ( n + TPB - 1 ) / TPB, 65536 );
+    k_zero2<<<nb, TPB, 0, st>>>( p, q, n );
+    CMD_LAUNCH_CHECK();
+    return CMD_OK;
+}
+
 cmd_status fill_f64( double *p, double v, size_t n, hipStream_t st )
 {
     if ( n == 0 )
@@ -343,8 +364,7 @@ static cmd_status cell_sort_reserve( int n, int ncell, CellSort &cs, hipStream_t
     CMD_TRY( cs.start.reserve( ncell + 1 ) );
     CMD_TRY( cs.perm.reserve( std::max( n, 1 ) ) );
     CMD_TRY( cs.tmp.reserve( std::max( n, 1 ) ) );
-    CMD_TRY( fill_i32( cs.count.p, 0, ncell, st ) );
-    CMD_TRY( fill_i32( cs.cursor.p, 0, ncell, st ) );
+    CMD_TRY( zero2_i32( cs.count.p, cs.cursor.p, ncell, st ) );
     return CMD_OK;
 }
 
@@ -675,42 +695,64 @@ __device__ __forceinline__ int halo_mask( const HaloFaces &H, double4 p )
     return m;
 }
 
-// Images per set and block (cnt[j * nblk + block]), the totals per set, and
-// each atom's phase mask.
+static HaloFaces halo_faces( const cmd_system *s, double comm_depth )
+{
+    HaloFaces H;
+    for ( int d = 0; d < 3; ++d )
+    {
+        H.hi[d] = s->dom.local_hi[d] - comm_depth;
+        H.lo[d] = s->dom.local_lo[d] + comm_depth;
+        H.L[d] = s->dom.L[d];
+    }
+    return H;
+}
+
+// The totals per set (tot, unless null) and, in the pass over sorted atoms
+// (MASKS), each atom's phase mask and the images per set and block
+// (cnt[j * nblk + block]).  That pass takes one atom per thread; the
+// totals-only pass strides over the atoms with as many blocks as launched
+// (fewer blocks, fewer global atomics on the 26 totals).
+template <bool MASKS>
 __global__ __launch_bounds__( TPB ) void k_halo_count( int n, const double4 *__restrict__ x,
                                                        HaloFaces H, int *__restrict__ mask,
                                                        int nblk, int *__restrict__ cnt,
                                                        int *__restrict__ tot )
 {
     __shared__ int s_c[HALO_SETS];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     if ( threadIdx.x < HALO_SETS )
         s_c[threadIdx.x] = 0;
     __syncthreads();
-    int m = 0;
-    if ( i < n )
+    const int stride = MASKS ? n : gridDim.x * blockDim.x;
+    for ( int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride )
     {
-        m = halo_mask( H, x[i] );
-        mask[i] = m;
-    }
-    if ( __builtin_amdgcn_ballot_w64( m != 0 ) )
-    {
-#pragma unroll
-        for ( int j = 0; j < HALO_SETS; ++j )
+        const int i = b0 + threadIdx.x;
+        int m = 0;
+        if ( i < n )
         {
-            const unsigned long long b =
-                __builtin_amdgcn_ballot_w64( ( m & halo_set( j ) ) == halo_set( j ) );
-            if ( lane == 0 && b )
-                atomicAdd( &s_c[j], (int)__popcll( b ) );
+            m = halo_mask( H, x[i] );
+            if ( MASKS )
+                mask[i] = m;
+        }
+        if ( __builtin_amdgcn_ballot_w64( m != 0 ) )
+        {
+#pragma unroll
+            for ( int j = 0; j < HALO_SETS; ++j )
+            {
+                const unsigned long long b =
+                    __builtin_amdgcn_ballot_w64( ( m & halo_set( j ) ) == halo_set( j ) );
+                if ( lane == 0 && b )
+                    atomicAdd( &s_c[j], (int)__popcll( b ) );
+            }
         }
     }
     __syncthreads();
     if ( threadIdx.x < HALO_SETS )
     {
         const int c = s_c[threadIdx.x];
-        cnt[threadIdx.x * nblk + blockIdx.x] = c;
-        if ( c )
+        if ( MASKS )
+            cnt[threadIdx.x * nblk + blockIdx.x] = c;
+        if ( c && tot )
             atomicAdd( &tot[threadIdx.x], c );
     }
 }
@@ -719,15 +761,40 @@ __global__ __launch_bounds__( TPB ) void k_halo_count( int n, const double4 *__r
 // off[j * nblk + block] (the exclusive scan of the counts) is where the
 // block's images of set j start.  send_idx records each ghost's source as
 // the phase saw it (the local atom, or the ghost it was sent on from).
-__global__ __launch_bounds__( TPB ) void k_halo_fill( int n, double4 *__restrict__ x,
+//
+// With totals counted before the sort (pre non-null, ng their sum): a
+// block writes nothing unless the scan's total is ng, so no image lands
+// past the ghost rows the host sized, and block 0 sets *flag when a set's
+// images differ from its total.  The images' velocity and force rows are
+// zeroed (cap: the planes' stride).
+__global__ __launch_bounds__( TPB ) void k_halo_fill( int n, int cap, double4 *__restrict__ x,
+                                                      double *__restrict__ v,
+                                                      double *__restrict__ f,
                                                       int *__restrict__ id,
                                                       const int *__restrict__ mask,
                                                       int nblk, const int *__restrict__ off,
                                                       HaloFaces H, int *__restrict__ send_idx,
                                                       int *__restrict__ root,
-                                                      int *__restrict__ code )
+                                                      int *__restrict__ code,
+                                                      const int *__restrict__ pre, int ng,
+                                                      int *__restrict__ flag )
 {
     __shared__ int s_w[HALO_SETS][TPB / 64];
+    if ( pre )
+    {
+        if ( blockIdx.x == 0 && threadIdx.x < HALO_SETS )
+        {
+            const int j = threadIdx.x;
+            if ( off[( j + 1 ) * nblk] - off[j * nblk] != pre[j] )
+                *flag = 1;
+        }
+        if ( off[HALO_SETS * nblk] != ng )
+        {
+            if ( threadIdx.x == 0 )
+                *flag = 1;
+            return; // block-uniform
+        }
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = i < n ? mask[i] : 0;
@@ -780,11 +847,50 @@ __global__ __launch_bounds__( TPB ) void k_halo_fill( int n, double4 *__restrict
                 p.z += H.L[2];
             x[n + k] = p;
             id[n + k] = idv;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+            {
+                v[(size_t)d * cap + n + k] = 0.0;
+                f[(size_t)d * cap + n + k] = 0.0;
+            }
             root[k] = i;
             code[k] = c;
             send_idx[k] = halo_source( j ) < 0 ? i : n + pos[halo_source( j ) < 0 ? 0 : halo_source( j )];
         }
     }
+}
+
+cmd_status halo_precount( cmd_system *s, double comm_depth )
+{
+    s->halo_pre = false;
+    const int n = s->n_local;
+    if ( n == 0 || !s->has_domain )
+        return CMD_OK;
+    CMD_TRY( s->hw.init( HW_WORDS ) );
+    CMD_TRY( s->halo_pre_tot.reserve( HALO_SETS ) );
+    CMD_TRY( fill_i32( s->halo_pre_tot.p, 0, HALO_SETS, s->stream ) );
+    k_halo_count<false><<<std::min( nblocks_for( n ), 1024 ), TPB, 0, s->stream>>>(
+        n, s->x.p, halo_faces( s, comm_depth ), nullptr, 0, nullptr, s->halo_pre_tot.p );
+    CMD_LAUNCH_CHECK();
+    CMD_HIP( hipMemcpyAsync( s->hw.p + HW_HALO, s->halo_pre_tot.p, HALO_SETS * sizeof( int ),
+                             hipMemcpyDeviceToHost, s->stream ) );
+    CMD_HIP( hipEventRecord( s->hw.ev, s->stream ) );
+    s->halo_pre = true;
+    s->halo_pre_n = n;
+    s->halo_pre_depth = comm_depth;
+    return CMD_OK;
+}
+
+cmd_status halo_precount_check( cmd_system *s )
+{
+    if ( !s->halo_pre_check )
+        return CMD_OK;
+    s->halo_pre_check = false;
+    CMD_HIP( hipEventSynchronize( s->hw.ev ) );
+    CMD_REQUIRE( s->hw.p[HW_HALO_FLAG] == 0, CMD_ERR_STATE,
+                 "the halo pass over sorted atoms found other ghost totals than the "
+                 "count before the sort" );
+    return CMD_OK;
 }
 
 // update_halo (comm_mpi_impl.h:369-408), resolved: every ghost coordinate is
@@ -1361,18 +1467,15 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
     CMD_REQUIRE( s->has_domain, CMD_ERR_STATE, "system has no domain" );
     s->xg_valid = false;
     const int n = s->n_local;
+    // totals counted before the sort of these positions (halo_precount)
+    const bool pre = s->halo_pre && s->halo_pre_n == n && s->halo_pre_depth == comm_depth;
+    s->halo_pre = false;
     s->n_ghost = 0;
     for ( int k = 0; k < 6; ++k )
         s->counts[k] = 0;
     if ( n == 0 )
         return CMD_OK;
-    HaloFaces H;
-    for ( int d = 0; d < 3; ++d )
-    {
-        H.hi[d] = s->dom.local_hi[d] - comm_depth;
-        H.lo[d] = s->dom.local_lo[d] + comm_depth;
-        H.L[d] = s->dom.L[d];
-    }
+    const HaloFaces H = halo_faces( s, comm_depth );
     const int nblk = nblocks_for( n );
     const size_t nc = (size_t)HALO_SETS * nblk;
     // counts, their scan (nc + 1 entries: the scan appends the total), the
@@ -1380,13 +1483,27 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
     // arrays, which may be regrown below)
     CMD_TRY( s->halo_blk.reserve( 2 * nc + 1 + HALO_SETS + n ) );
     int *cnt = s->halo_blk.p, *off = cnt + nc, *tot = off + nc + 1, *mask = tot + HALO_SETS;
-    CMD_TRY( fill_i32( tot, 0, HALO_SETS, s->stream ) );
-    k_halo_count<<<nblk, TPB, 0, s->stream>>>( n, s->x.p, H, mask, nblk, cnt, tot );
+    if ( !pre )
+        CMD_TRY( fill_i32( tot, 0, HALO_SETS, s->stream ) );
+    k_halo_count<true><<<nblk, TPB, 0, s->stream>>>( n, s->x.p, H, mask, nblk, cnt,
+                                                     pre ? nullptr : tot );
     CMD_LAUNCH_CHECK();
     CMD_TRY( scan_exclusive( cnt, off, (int)nc, s->sc, s->stream, nullptr ) );
     int th[HALO_SETS];
-    CMD_HIP( hipMemcpyAsync( th, tot, sizeof( th ), hipMemcpyDeviceToHost, s->stream ) );
-    CMD_HIP( hipStreamSynchronize( s->stream ) );
+    if ( pre )
+    {
+        // the count before the sort is long done by now; the sorted pass
+        // above stays queued behind the sort
+        CMD_HIP( hipEventSynchronize( s->hw.ev ) );
+        for ( int j = 0; j < HALO_SETS; ++j )
+            th[j] = s->hw.p[HW_HALO + j];
+        s->hw.p[HW_HALO_FLAG] = 0;
+    }
+    else
+    {
+        CMD_HIP( hipMemcpyAsync( th, tot, sizeof( th ), hipMemcpyDeviceToHost, s->stream ) );
+        CMD_HIP( hipStreamSynchronize( s->stream ) );
+    }
     int ng = 0;
     for ( int j = 0; j < HALO_SETS; ++j )
     {
@@ -1395,16 +1512,21 @@ cmd_status cmd_comm_exchange_halo( cmd_system *s, double comm_depth )
     }
     if ( n + ng > s->cap )
         CMD_TRY( s->reserve( (int)( ( n + ng ) * 1.1 ) + 64 ) );
-    if ( ng > 0 )
+    // the halo gathers positions and types only (comm_mpi_impl.h:346-347);
+    // the ghosts' velocity and force rows are set to zero so that downloads
+    // are defined and full-list kernels, which never write them, leave them 0
+    if ( ng > 0 || pre )
     {
-        k_halo_fill<<<nblk, TPB, 0, s->stream>>>( n, s->x.p, s->id.p, mask, nblk, off,
-                                                  H, s->send_idx.p, s->ghost_root.p,
-                                                  s->ghost_code.p );
+        k_halo_fill<<<nblk, TPB, 0, s->stream>>>(
+            n, s->cap, s->x.p, s->v.p, s->f.p, s->id.p, mask, nblk, off, H, s->send_idx.p,
+            s->ghost_root.p, s->ghost_code.p, pre ? s->halo_pre_tot.p : nullptr, ng,
+            pre ? s->hw.d + HW_HALO_FLAG : nullptr );
         CMD_LAUNCH_CHECK();
-        // the halo gathers positions and types only (comm_mpi_impl.h:346-347);
-        // ghost velocity rows are set to zero so that downloads are defined
-        for ( int c = 0; c < 3; ++c )
-            CMD_TRY( fill_f64( s->v.p + (size_t)c * s->cap + n, 0.0, ng, s->stream ) );
+    }
+    if ( pre )
+    {
+        CMD_HIP( hipEventRecord( s->hw.ev, s->stream ) );
+        s->halo_pre_check = true;
     }
     s->n_ghost = ng;
     return CMD_OK;

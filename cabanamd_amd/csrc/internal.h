@@ -119,6 +119,62 @@ struct DevBuf
 };
 
 // Reusable temporary storage for device-wide scans and reductions.
+// Pinned host words that the device writes (by copies, or by kernels through
+// the device address d), and an event recorded after the last such write.
+struct HostWords
+{
+    int *p = nullptr; // host address
+    int *d = nullptr; // device address of the same words
+    hipEvent_t ev = nullptr;
+    HostWords() = default;
+    HostWords( const HostWords & ) = delete;
+    HostWords &operator=( const HostWords & ) = delete;
+    ~HostWords()
+    {
+        if ( ev )
+            (void)hipEventDestroy( ev );
+        if ( p )
+            (void)hipHostFree( p );
+    }
+    // allocates the words (zeroed) and the event on first use
+    cmd_status init( size_t words )
+    {
+        if ( p )
+            return CMD_OK;
+        void *h = nullptr;
+        hipError_t e = hipHostMalloc( &h, words * sizeof( int ), hipHostMallocDefault );
+        if ( e != hipSuccess )
+        {
+            set_error( "hipHostMalloc of %zu bytes failed: %s", words * sizeof( int ),
+                       hipGetErrorString( e ) );
+            return CMD_ERR_OOM;
+        }
+        p = static_cast<int *>( h );
+        for ( size_t k = 0; k < words; ++k )
+            p[k] = 0;
+        void *dp = nullptr;
+        e = hipHostGetDevicePointer( &dp, h, 0 );
+        if ( e == hipSuccess )
+            e = hipEventCreateWithFlags( &ev, hipEventDisableTiming );
+        if ( e != hipSuccess )
+        {
+            set_error( "pinned result words: %s", hipGetErrorString( e ) );
+            return CMD_ERR_HIP;
+        }
+        d = static_cast<int *>( dp );
+        return CMD_OK;
+    }
+};
+
+// Layout of a system's pinned result words (cmd_system::hw).
+enum
+{
+    HW_HALO = 0,       // 26 ghost totals per phase set, counted before the sort
+    HW_HALO_FLAG = 26, // 1: the sorted halo pass found other totals
+    HW_TILE = 32,      // the tiled build's result words (TILE_RES_*)
+    HW_WORDS = 64
+};
+
 struct Scratch
 {
     DevBuf<int64_t> blk;      // per-block partial sums of scans
@@ -137,6 +193,8 @@ cmd_status scan_exclusive( const int *in, int64_t *out, int n, Scratch &sc,
 
 // Fill kernels (graph-safe replacement for hipMemsetAsync).
 cmd_status fill_i32( int *p, int v, size_t n, hipStream_t st );
+// p[0 .. n) = q[0 .. n) = 0 in one launch.
+cmd_status zero2_i32( int *p, int *q, size_t n, hipStream_t st );
 cmd_status fill_f64( double *p, double v, size_t n, hipStream_t st );
 
 struct MassTable
@@ -231,6 +289,15 @@ struct cmd_system
     // halo build scratch: per-block image counts of the 26 phase sets, their
     // scan, the totals and the atoms' phase masks
     cmd::DevBuf<int> halo_blk;
+    // ghost totals per phase set counted before a sort (halo_precount), for
+    // positions that only the sort changes until the halo pass consumes
+    // them; hw holds their host copy
+    cmd::DevBuf<int> halo_pre_tot;
+    bool halo_pre = false;       // the totals await the next halo pass
+    bool halo_pre_check = false; // that pass's mismatch flag is still unread
+    int halo_pre_n = 0;
+    double halo_pre_depth = 0.0;
+    cmd::HostWords hw;
 
     cmd::CellSort bins;
     // grid of the last sorting create_binning (valid until atoms change)
@@ -415,6 +482,14 @@ cmd_status ghost_refresh( cmd_system *S );
 // System::permute of the owned atoms: slot k takes atom perm[k] (x, v, f, id).
 cmd_status permute_local( cmd_system *S, const int *perm );
 cmd_status wrap_local( cmd_system *S );
+// The halo's ghost totals of the current positions, counted before a sort
+// that only permutes them: the next cmd_comm_exchange_halo at the same depth
+// takes its ghost count from the pinned copy of these totals (waiting for
+// this count's event, not for the sorted pass) and checks on the device that
+// the sorted pass finds the same totals; halo_precount_check reads that
+// result once the stream has passed it.
+cmd_status halo_precount( cmd_system *S, double comm_depth );
+cmd_status halo_precount_check( cmd_system *S );
 cmd_status force_compute_plain( cmd_force *F, cmd_system *S, cmd_neigh *N );
 cmd_status neigh_build( cmd_neigh *N, cmd_system *S );
 cmd_status neigh_csr( cmd_neigh *N );
@@ -440,8 +515,10 @@ enum
     TILE_RES_MAXROW = 5,   // longest row of the build
     TILE_RES_OVER = 6,     // 1: a block's needs exceeded the launch sizes
     TILE_RES_INSIDE = 7,   // 1: every position lies in the binning grid
-    TILE_RES_WORDS = 8
+    TILE_RES_WAVES = 8,    // slab waves of all tiles (written by the build)
+    TILE_RES_WORDS = 9
 };
+static_assert( HW_TILE + TILE_RES_WORDS <= HW_WORDS, "tile results fit the pinned words" );
 cmd_status tiles_csr( cmd_neigh *N, int64_t *nnz );
 cmd_status tiles_ghost_refresh( cmd_system *S );
 } // namespace cmd

@@ -59,6 +59,10 @@ struct cmd_md
     cmd::DevBuf<double> partA, partB, trows;
     double timers[6] = { 0, 0, 0, 0, 0, 0 };
     bool profile = false;
+    // above 1: events time one force phase in prof_every and no other phase,
+    // so that their markers stay out of most steps
+    int prof_every = 1;
+    int prof_seen = 0;
     int prof_launches = 0;
     double prof_ms = 0.0;
     struct Ev
@@ -91,14 +95,23 @@ struct cmd_md
             pool.pop_back();
             return e;
         }
+        return new_event();
+    }
+    // timing only: no system-scope fence at the marker, which would write
+    // back and invalidate the caches between the timed launches
+    static hipEvent_t new_event()
+    {
         hipEvent_t e = nullptr;
-        (void)hipEventCreate( &e );
+        if ( hipEventCreateWithFlags( &e, hipEventDisableSystemFence ) != hipSuccess )
+            return nullptr;
         return e;
     }
     // Phase timing by HIP events on the system's stream (profile mode).
     int begin( int phase )
     {
         if ( !profile )
+            return -1;
+        if ( prof_every > 1 && ( phase != cmd::PH_FORCE || prof_seen++ % prof_every != 0 ) )
             return -1;
         Ev e{ get_event(), get_event(), phase };
         (void)hipEventRecord( e.a, S->stream );
@@ -137,20 +150,20 @@ static cmd_status rebuild( cmd_md *md )
 {
     cmd_system *S = md->S;
     int k = md->begin( PH_OTHER );
+    // the ghost totals of these positions, counted ahead of the sort (which
+    // only permutes them): the halo pass then has its ghost count without
+    // waiting for the sort, so the rebuild's launches queue up behind it
+    CMD_TRY( halo_precount( S, md->neigh_cut ) );
     // the time loop computes the forces afresh after a rebuild before it
     // reads them: they are not carried through the sort
     CMD_TRY( binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut, 1, 1, 0, 1,
                              nullptr, false ) );
     md->end( k );
     k = md->begin( PH_COMM );
+    // ghost velocity and force rows come out zero (full-list kernels never
+    // write the force rows)
     CMD_TRY( cmd_comm_exchange_halo( S, md->neigh_cut ) );
     md->end( k );
-    // ghost force rows are never written by full-list kernels: keep them 0
-    // (issued here, ahead of the list build's wait)
-    if ( !md->N->half )
-        for ( int c = 0; c < 3; ++c )
-            CMD_TRY( fill_f64( S->f.p + (size_t)c * S->cap + S->n_local, 0.0,
-                               S->n_ghost, S->stream ) );
     k = md->begin( PH_NEIGH );
     if ( md->retune )
     {
@@ -161,7 +174,8 @@ static cmd_status rebuild( cmd_md *md )
     else
         CMD_TRY( cmd_neigh_build( md->N, S ) );
     md->end( k );
-    return CMD_OK;
+    // the list build has waited on the device: the halo's check is in
+    return halo_precount_check( S );
 }
 
 static void thermo_row( cmd_md *md, int step, double mv2, double pe, double w )
@@ -951,6 +965,19 @@ cmd_status cmd_md_profile( cmd_md *md, int enable )
 {
     CMD_REQUIRE( md, CMD_ERR_ARG, "md is NULL" );
     md->profile = enable != 0;
+    md->prof_every = std::max( enable, 1 );
+    md->prof_seen = 0;
+    // events are created here rather than inside the steps they time (a
+    // creation costs tens of microseconds of host time); more are created
+    // on demand
+    if ( md->profile )
+        while ( md->pool.size() < 512 )
+        {
+            hipEvent_t e = md->new_event();
+            if ( !e )
+                break;
+            md->pool.push_back( e );
+        }
     md->prof_launches = 0;
     md->prof_ms = 0.0;
     for ( int k = 0; k < 6; ++k )

@@ -31,6 +31,7 @@
 #include "tiles_dev.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -342,7 +343,8 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const int *__restrict__ info, const int *__restrict__ woff,
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2, int rec,
     int cap, int slot_cap, int rows_cap, int waves_cap, unsigned short *__restrict__ slab,
-    int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over )
+    int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over,
+    int *__restrict__ waves_total )
 {
     extern __shared__ double s_x[];
     __shared__ TileHead H;
@@ -356,6 +358,8 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const int W = T.W, S = W + 2, NA = W * W;
     const int sub = xcd_tile( blockIdx.x, gridDim.x );
     const int t = sub / NA, c = sub % NA, ca = c / W, cb = c % W;
+    if ( blockIdx.x == 0 && threadIdx.x == 0 )
+        *waves_total = woff[T.ntiles]; // read back with the build's results
     load_head( info, t, W, H );
     const int nat_c = H.info[TI_AN + c];
     if ( nat_c == 0 )
@@ -913,14 +917,22 @@ static cmd_status tile_info_launch( cmd_neigh *N, cmd_system *S, int W, int E )
 }
 
 // Reads the result words (res, TILE_RES_WORDS of them) and the slab waves
-// of all tiles (h5) in one wait.
-static cmd_status tile_results( cmd_neigh *N, cmd_system *S, int *res, int *h5 )
+// of all tiles (h5) in one wait, through the system's pinned words: after a
+// build (built), which stores the slab waves among the result words, in one
+// copy.
+static cmd_status tile_results( cmd_neigh *N, cmd_system *S, int *res, int *h5, bool built )
 {
-    CMD_HIP( hipMemcpyAsync( res, S->sc.ires.p, TILE_RES_WORDS * sizeof( int ),
+    CMD_TRY( S->hw.init( HW_WORDS ) );
+    int *hr = S->hw.p + HW_TILE;
+    CMD_HIP( hipMemcpyAsync( hr, S->sc.ires.p, TILE_RES_WORDS * sizeof( int ),
                              hipMemcpyDeviceToHost, S->stream ) );
-    CMD_HIP( hipMemcpyAsync( h5, N->grid_sort.start.p + N->tg.ntiles, sizeof( int ),
-                             hipMemcpyDeviceToHost, S->stream ) );
+    if ( !built )
+        CMD_HIP( hipMemcpyAsync( hr + TILE_RES_WAVES, N->grid_sort.start.p + N->tg.ntiles,
+                                 sizeof( int ), hipMemcpyDeviceToHost, S->stream ) );
     CMD_HIP( hipStreamSynchronize( S->stream ) );
+    for ( int k = 0; k < TILE_RES_WORDS; ++k )
+        res[k] = hr[k];
+    *h5 = res[TILE_RES_WAVES];
     return CMD_OK;
 }
 
@@ -939,18 +951,37 @@ static size_t tile_slab_entries( const cmd_neigh *N, int nw )
 }
 
 // LDS of one block (hard limit, less the static tables) and of one CU.
+// The device attributes are read once per device.
 static void lds_limits( size_t *block, size_t *cu )
 {
+    constexpr int NDEV = 64;
+    static std::atomic<int> cached_lim[NDEV], cached_lcu[NDEV];
     int dev = 0, lim = 0, lcu = 0;
-    if ( hipGetDevice( &dev ) != hipSuccess ||
-         hipDeviceGetAttribute( &lim, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                dev ) != hipSuccess ||
-         lim <= 0 )
-        lim = 65536;
-    if ( hipDeviceGetAttribute( &lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                dev ) != hipSuccess ||
-         lcu < lim )
-        lcu = lim;
+    if ( hipGetDevice( &dev ) != hipSuccess || dev < 0 || dev >= NDEV )
+        dev = -1;
+    if ( dev >= 0 )
+    {
+        lim = cached_lim[dev].load();
+        lcu = cached_lcu[dev].load();
+    }
+    if ( lim <= 0 || lcu <= 0 )
+    {
+        if ( dev < 0 ||
+             hipDeviceGetAttribute( &lim, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                    dev ) != hipSuccess ||
+             lim <= 0 )
+            lim = 65536;
+        if ( dev < 0 ||
+             hipDeviceGetAttribute( &lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                    dev ) != hipSuccess ||
+             lcu < lim )
+            lcu = lim;
+        if ( dev >= 0 )
+        {
+            cached_lim[dev].store( lim );
+            cached_lcu[dev].store( lcu );
+        }
+    }
     const size_t statics = 8192; // tile header, cell and range tables
     *block = (size_t)lim > statics ? (size_t)lim - statics : 0;
     *cu = (size_t)lcu;
@@ -979,12 +1010,14 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
         k_tile_build<G, true><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
-            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
+            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
+            res + TILE_RES_WAVES );
     else
         k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
-            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER );
+            waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
+            res + TILE_RES_WAVES );
 }
 
 static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_cap,
@@ -1111,7 +1144,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
         for ( ;; )
         {
             CMD_TRY( tile_info_launch( N, S, W, E ) );
-            CMD_TRY( tile_results( N, S, res, &h[5] ) );
+            CMD_TRY( tile_results( N, S, res, &h[5], false ) );
             for ( int k = 0; k < 5; ++k )
                 h[k] = res[TILE_RES_INFO + k];
             N->tg.bw = force_waves( N, h[2] );
@@ -1135,7 +1168,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
         // the slab sized for the next build's launch too (record_spec)
         CMD_TRY( N->tslab.reserve( tile_slab_entries( N, h[5] + std::max( 64, h[5] / 50 ) ) ) );
         CMD_TRY( launch_build_g( N, S, h[3] + 1, h[4], h[5], lds ) );
-        CMD_TRY( tile_results( N, S, res, &h[5] ) );
+        CMD_TRY( tile_results( N, S, res, &h[5], true ) );
         const int mr = res[TILE_RES_MAXROW];
         N->max_row = mr;
         if ( mr > N->max_neigh_guess )
@@ -1179,7 +1212,7 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     CMD_TRY( N->tslab.reserve( tile_slab_entries( N, P.slab_waves ) ) );
     CMD_TRY( launch_build_g( N, S, P.slot_cap, P.rows, P.slab_waves, P.lds ) );
     int res[TILE_RES_WORDS], h[6] = { 0, 0, 0, 0, 0, 0 };
-    CMD_TRY( tile_results( N, S, res, &h[5] ) );
+    CMD_TRY( tile_results( N, S, res, &h[5], true ) );
     for ( int k = 0; k < 5; ++k )
         h[k] = res[TILE_RES_INFO + k];
     if ( res[TILE_RES_INSIDE] != 1 || res[TILE_RES_OVER] != 0 || h[1] != S->n_local )

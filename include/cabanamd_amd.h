@@ -415,7 +415,10 @@ cmd_status cmd_md_check_correctness( cmd_md *md, const char *refdir, int step,
  * force, neigh, comm, integrate, other, total. */
 cmd_status cmd_md_timers( const cmd_md *md, double out[6] );
 /* Per-launch HIP-event timing of the fused force kernel on the system's
- * stream: enable before run(); returns launches and mean ms. */
+ * stream: enable before run(); returns launches and mean ms.  enable = 1
+ * times every phase of every step (the phase timers); enable = k > 1 times
+ * one force phase in k and no other phase, which keeps the events' markers
+ * (a few microseconds of idle device each) out of most steps. */
 cmd_status cmd_md_profile( cmd_md *md, int enable );
 cmd_status cmd_md_kernel_stats( const cmd_md *md, int *launches,
                                 double *mean_ms );

@@ -103,3 +103,25 @@ def test_cli_dumpbinary_and_correctness(gpu, tmp_path):
     rows = [ln.split() for ln in open(out).read().splitlines()[1:]]
     assert [row[0] for row in rows] == ["0", "10", "20"]
     assert all(float(v) == 0.0 for row in rows for v in row[1:])
+
+
+def test_kernel_profile_every_kth_launch(gpu):
+    """cmd_md_profile(k > 1) times one fused force launch in k and no other
+    phase; k = 1 times every phase of every step."""
+    M = gpu
+    p = M.lattice_params(ncells=(8, 8, 8), thermo=0, nsteps=40, gaussian=True)
+    md = M.CbnMD(p, precision=M.PREC_MIXED, lanes=2)
+    md.init()
+    md.profile(4)
+    md.run(20)
+    n4, ms4 = md.kernel_stats()
+    t4 = md.timers()
+    assert n4 == 5 and ms4 > 0
+    assert t4["neigh"] == 0 and t4["comm"] == 0 and t4["force"] > 0
+    md.profile(1)
+    md.run(20)
+    n1, ms1 = md.kernel_stats()
+    t1 = md.timers()
+    assert n1 == 20 and ms1 > 0
+    assert t1["neigh"] > 0  # the rebuild at step 40
+    md.profile(0)
