@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -171,11 +172,12 @@ __global__ void k_tile_info( TileGrid T, const int *__restrict__ ls,
 static constexpr int MAXR = 9 * 3 * 2; // stencil ranges of one cell
 static constexpr int BCH = 10;         // candidate chunks of 64 held in registers
 
-// Bytes of the coordinate planes of `slots` staged atoms, rounded up so that
-// what follows is 16-byte aligned.
+// Bytes of the coordinate planes of `slots` staged atoms (elements of type
+// PT), rounded up so that what follows is 16-byte aligned.
+template <class PT>
 __host__ __device__ constexpr size_t tile_planes_bytes( int slots )
 {
-    return ( (size_t)slots * 3 * sizeof( double ) + 15 ) & ~(size_t)15;
+    return ( (size_t)slots * 3 * sizeof( PT ) + 15 ) & ~(size_t)15;
 }
 
 // Entries of one LDS row of the list build: the slab's entries per atom
@@ -256,6 +258,109 @@ __device__ __forceinline__ void test_pair( const double ( &px )[BCH], const doub
     }
 }
 
+// The full-list build holds coordinates in FP32 about the block's origin
+// (half the LDS and registers of FP64).  A candidate whose FP32 r^2 lies
+// at or below lo (rc^2 less the FP32 error bound) is within rc in FP64
+// too; one above hi (rc^2 plus the bound) is not; one in between is tested
+// again in FP64 on the positions in memory, as the reference evaluates it.
+// What the re-test needs: the positions, the block's 9-column segment
+// offsets (LDS) and the tile head's words, to map a build slot to its atom.
+struct ExactCtx
+{
+    const double4 *x, *xg;
+    const int *bseg;
+    const int *info;
+    int ca, cb, S;
+    double rc2;
+};
+
+// Of the lanes in q (candidates at build slot sj, the atom at local index
+// ia), those within rc in FP64.
+__device__ __forceinline__ unsigned long long exact_lanes( const ExactCtx &X,
+                                                           unsigned long long q, int sj, int ia )
+{
+    bool ok = false;
+    if ( __builtin_amdgcn_inverse_ballot_w64( q ) )
+    {
+        const int seg = find_seg( X.bseg, 18, sj );
+        const int qb = seg >> 1, off = sj - X.bseg[seg];
+        const int qf = ( X.ca + qb / 3 ) * X.S + X.cb + qb % 3;
+        const double4 c = ( seg & 1 ) ? X.xg[X.info[TI_GB + qf] + off]
+                                      : X.x[X.info[TI_LB + qf] + off];
+        const double4 a = X.x[ia];
+        const double dx = a.x - c.x;
+        const double dy = a.y - c.y;
+        const double dz = a.z - c.z;
+        ok = dx * dx + dy * dy + dz * dz <= X.rc2;
+    }
+    return __builtin_amdgcn_ballot_w64( ok ) & q;
+}
+
+// The FP32 tests of two atoms (candidates vs and vs + 1 of the pass, local
+// indices ia and ia + 1) against NC register chunks, interleaved as in the
+// FP64 form; lanes in the band are settled by exact_lanes.
+template <int NC>
+__device__ __forceinline__ void test_pair32( const float ( &px )[BCH], const float ( &py )[BCH],
+                                             const float ( &pz )[BCH], const float4 pa,
+                                             const float4 pb, int vs, float lo, float hi,
+                                             const ExactCtx &X, const int ( &dj )[BCH], int v0,
+                                             int ia, unsigned long long ( &ma )[NC],
+                                             unsigned long long ( &mb )[NC] )
+{
+    const int csa = vs >> 6, csb = ( vs + 1 ) >> 6;
+    const unsigned long long nsa = ~( 1ull << ( vs & 63 ) );
+    const unsigned long long nsb = ~( 1ull << ( ( vs + 1 ) & 63 ) );
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for ( int c = 0; c < NC; ++c )
+    {
+        const float dxa = pa.x - px[c], dxb = pb.x - px[c];
+        const float dya = pa.y - py[c], dyb = pb.y - py[c];
+        const float dza = pa.z - pz[c], dzb = pb.z - pz[c];
+        const float ra = fmaf( dza, dza, fmaf( dya, dya, dxa * dxa ) );
+        const float rb = fmaf( dzb, dzb, fmaf( dyb, dyb, dxb * dxb ) );
+        unsigned long long sa = __builtin_amdgcn_ballot_w64( ra <= lo );
+        unsigned long long sb = __builtin_amdgcn_ballot_w64( rb <= lo );
+        const unsigned long long qa = __builtin_amdgcn_ballot_w64( ra <= hi ) & ~sa;
+        const unsigned long long qb = __builtin_amdgcn_ballot_w64( rb <= hi ) & ~sb;
+        if ( qa | qb )
+        {
+            const int sj = v0 + c * 64 + lane + dj[c];
+            if ( qa )
+                sa |= exact_lanes( X, qa, sj, ia );
+            if ( qb )
+                sb |= exact_lanes( X, qb, sj, ia + 1 );
+        }
+        ma[c] = sa & ( c == csa ? nsa : ~0ull );
+        mb[c] = sb & ( c == csb ? nsb : ~0ull );
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void test_chunks32( const float ( &px )[BCH], const float ( &py )[BCH],
+                                               const float ( &pz )[BCH], const float4 pa, int vs,
+                                               float lo, float hi, const ExactCtx &X,
+                                               const int ( &dj )[BCH], int v0, int ia,
+                                               unsigned long long ( &m )[NC] )
+{
+    const int cself = vs >> 6;
+    const unsigned long long notself = ~( 1ull << ( vs & 63 ) );
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for ( int c = 0; c < NC; ++c )
+    {
+        const float dx = pa.x - px[c];
+        const float dy = pa.y - py[c];
+        const float dz = pa.z - pz[c];
+        const float r = fmaf( dz, dz, fmaf( dy, dy, dx * dx ) );
+        unsigned long long sa = __builtin_amdgcn_ballot_w64( r <= lo );
+        const unsigned long long q = __builtin_amdgcn_ballot_w64( r <= hi ) & ~sa;
+        if ( q )
+            sa |= exact_lanes( X, q, v0 + c * 64 + lane + dj[c], ia );
+        m[c] = sa & ( c == cself ? notself : ~0ull );
+    }
+}
+
 // The accepted candidates of one atom's masks appended to its row in chunk
 // order.  A row holds cap entries and a spare slot at index cap: lanes that
 // do not accept, and accepted entries that would fall past the row's
@@ -296,27 +401,52 @@ __device__ __forceinline__ double4 lane_pos( const double4 &a, int l )
                          readlane_f64( a.z, l ), 0.0 );
 }
 
+__device__ __forceinline__ float readlane_f32( float v, int l )
+{
+    return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), l ) );
+}
+
+__device__ __forceinline__ float4 lane_pos( const float4 &a, int l )
+{
+    return make_float4( readlane_f32( a.x, l ), readlane_f32( a.y, l ),
+                        readlane_f32( a.z, l ), 0.0f );
+}
+
 // The atoms 0 .. na - 1 of a group of a cell's atoms (lane l of `at` holds
 // atom l's position; candidates vs0 + l of this pass, LDS rows row0 + l * RS)
 // against NC register chunks of candidates, two atoms at a time.  An atom's
 // position and row length are read out of their lanes as wave-uniform
 // values, so the loop waits on no memory.  Lane l of cnt holds atom l's row
-// length.
-template <int NC, int G, bool HALF>
-__device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const double ( &py )[BCH],
-                                             const double ( &pz )[BCH], const int ( &po )[BCH],
-                                             const double4 &at, int na,
-                                             int vs0, unsigned short *row0, int RS, double rc2,
-                                             unsigned cap, int &cnt )
+// length.  HALF: FP64 coordinates and tests; otherwise FP32 coordinates about
+// the block's origin with the band of B (pass start v0, local index i0 of
+// the group's first atom) settled in FP64 (ExactCtx, dj: the chunks'
+// candidate-to-slot offsets).
+struct BandArgs
+{
+    float lo, hi;
+    int v0, i0;
+};
+
+template <int NC, int G, bool HALF, class PT, class P4>
+__device__ __forceinline__ void build_atoms( const PT ( &px )[BCH], const PT ( &py )[BCH],
+                                             const PT ( &pz )[BCH], const int ( &po )[BCH],
+                                             const P4 &at, int na, int vs0,
+                                             unsigned short *row0, int RS, double rc2,
+                                             unsigned cap, int &cnt, const ExactCtx &X,
+                                             const int ( &dj )[BCH], const BandArgs &B )
 {
     int l = 0;
     for ( ; l + 1 < na; l += 2 )
     {
-        const double4 pa = lane_pos( at, l ), pb = lane_pos( at, l + 1 );
+        const P4 pa = lane_pos( at, l ), pb = lane_pos( at, l + 1 );
         const unsigned ka0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
         const unsigned kb0 = (unsigned)__builtin_amdgcn_readlane( cnt, l + 1 );
         unsigned long long ma[NC], mb[NC];
-        test_pair<NC, HALF>( px, py, pz, pa, pb, vs0 + l, rc2, ma, mb );
+        if constexpr ( HALF )
+            test_pair<NC, HALF>( px, py, pz, pa, pb, vs0 + l, rc2, ma, mb );
+        else
+            test_pair32<NC>( px, py, pz, pa, pb, vs0 + l, B.lo, B.hi, X, dj, B.v0, B.i0 + l,
+                             ma, mb );
         const unsigned ka = place_row<NC>( po, ma, row0 + l * RS, ka0, cap );
         const unsigned kb = place_row<NC>( po, mb, row0 + ( l + 1 ) * RS, kb0, cap );
         const int lane = threadIdx.x & 63;
@@ -324,9 +454,12 @@ __device__ __forceinline__ void build_atoms( const double ( &px )[BCH], const do
     }
     if ( l < na )
     {
-        const double4 pa = lane_pos( at, l );
+        const P4 pa = lane_pos( at, l );
         unsigned long long ma[NC];
-        test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
+        if constexpr ( HALF )
+            test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
+        else
+            test_chunks32<NC>( px, py, pz, pa, vs0 + l, B.lo, B.hi, X, dj, B.v0, B.i0 + l, ma );
         const unsigned ka = place_row<NC>( po, ma, row0 + l * RS,
                                            (unsigned)__builtin_amdgcn_readlane( cnt, l ), cap );
         if ( ( threadIdx.x & 63 ) == l )
@@ -344,9 +477,14 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2, int rec,
     int cap, int slot_cap, int rows_cap, int waves_cap, unsigned short *__restrict__ slab,
     int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over,
-    int *__restrict__ waves_total )
+    int *__restrict__ waves_total, float band_lo, float band_hi )
 {
-    extern __shared__ double s_x[];
+    // coordinate planes: FP64 for the half list (its direction test compares
+    // coordinates exactly), FP32 about the block's origin otherwise
+    using PT = std::conditional_t<HALF, double, float>;
+    using P4 = std::conditional_t<HALF, double4, float4>;
+    extern __shared__ double s_dyn[];
+    PT *s_x = reinterpret_cast<PT *>( s_dyn );
     __shared__ TileHead H;
     __shared__ int s_bseg[19];                // the block's 9-column segment offsets
     __shared__ int s_dl[9], s_dg[9];          // force slot - build slot, per column
@@ -368,6 +506,10 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     tile_coords( T, t, a0, b0, e0, e1 );
     const int a = a0 + ca, b = b0 + cb;
     const int ea = H.info[TI_EA], nst = H.info[TI_NST];
+    // the FP32 planes' origin: the centre of the atom column's staged cells
+    const double ox = HALF ? 0.0 : T.g.min[0] + ( a + 0.5 ) / T.g.rd[0];
+    const double oy = HALF ? 0.0 : T.g.min[1] + ( b + 0.5 ) / T.g.rd[1];
+    const double oz = HALF ? 0.0 : T.g.min[2] + ( ea + 0.5 * nst ) / T.g.rd[2];
     // the block's 3 x 3 columns: build column qb = da * 3 + db is staged
     // column (ca + da) * S + cb + db of the tile
     if ( threadIdx.x == 0 )
@@ -410,7 +552,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         cs_g = gs[cl];
     }
     // coordinate planes of the 3 x 3 columns' runs, then the sentinel
-    double *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
+    PT *s_y = s_x + slot_cap, *s_z = s_x + 2 * slot_cap;
     const int btotal = s_bseg[18];
     {
         constexpr int B = 8;
@@ -430,9 +572,9 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
             for ( int u = 0; u < B; ++u )
             {
                 const int k = min( k0 + u * (int)blockDim.x, btotal );
-                s_x[k] = p[u].x;
-                s_y[k] = p[u].y;
-                s_z[k] = p[u].z;
+                s_x[k] = (PT)( p[u].x - ox );
+                s_y[k] = (PT)( p[u].y - oy );
+                s_z[k] = (PT)( p[u].z - oz );
             }
         }
         if ( has_cs )
@@ -445,15 +587,15 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         __syncthreads();
         if ( threadIdx.x == 0 )
         {
-            s_x[btotal] = TILE_FAR;
-            s_y[btotal] = TILE_FAR;
-            s_z[btotal] = TILE_FAR;
+            s_x[btotal] = (PT)TILE_FAR;
+            s_y[btotal] = (PT)TILE_FAR;
+            s_z[btotal] = (PT)TILE_FAR;
         }
     }
     // after the planes (rounded to 16 bytes): the rows of this atom column's
     // atoms
     unsigned short *chunk = reinterpret_cast<unsigned short *>(
-        reinterpret_cast<char *>( s_x ) + tile_planes_bytes( slot_cap ) );
+        reinterpret_cast<char *>( s_x ) + tile_planes_bytes<PT>( slot_cap ) );
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
     const int wsz = R4 * 256; // slab entries of one slab wave
     // the stride is an odd number of words, which spreads the copy-out's
@@ -476,6 +618,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     int mrow = 0;
     int *rs = s_rs[wv], *rf = s_rf[wv], *rp = s_rp[wv];
     const int bsent = btotal;
+    const ExactCtx X{ x, xg, s_bseg, H.info, ca, cb, S, rc2 };
     for ( int e = e0 + wv; e < e1; e += nwv )
     {
         const int ce = e - ea;
@@ -601,7 +744,11 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         {
             const int na = min( 64, c1 - i0 );
             const int sa = i0 + min( lane, na - 1 ) - dl4;
-            const double4 at = make_double4( s_x[sa], s_y[sa], s_z[sa], 0.0 ); // lane l: atom i0 + l
+            P4 at; // lane l: atom i0 + l
+            at.x = s_x[sa];
+            at.y = s_y[sa];
+            at.z = s_z[sa];
+            at.w = 0;
             int cnt = 0; // row length of atom i0 + lane
             int cur = 0; // range holding the pass's first candidate
             for ( int v0 = 0; v0 < total; v0 += BCH * 64 )
@@ -640,7 +787,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                         df[ch] = in ? f : df[ch];
                     }
                 }
-                double px[BCH], py[BCH], pz[BCH];
+                PT px[BCH], py[BCH], pz[BCH];
                 int po[BCH]; // the force tile's record byte offset, the value a row stores
 #pragma unroll
                 for ( int ch = 0; ch < BCH; ++ch )
@@ -657,21 +804,22 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                 const int vs0 = vbase + ( i0 - c0 ) - v0;
                 unsigned short *row0 = chunk + ( i0 - al ) * RS;
                 const unsigned ucap = (unsigned)cap;
+                const BandArgs B{ band_lo, band_hi, v0, i0 };
                 if ( nch <= 4 )
                     build_atoms<4, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
-                                             ucap, cnt );
+                                             ucap, cnt, X, dj, B );
                 else if ( nch <= 6 )
                     build_atoms<6, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
-                                             ucap, cnt );
+                                             ucap, cnt, X, dj, B );
                 else if ( nch <= 8 )
                     build_atoms<8, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
-                                             ucap, cnt );
+                                             ucap, cnt, X, dj, B );
                 else if ( nch == 9 )
                     build_atoms<9, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
-                                             ucap, cnt );
+                                             ucap, cnt, X, dj, B );
                 else
                     build_atoms<BCH, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS,
-                                               rc2, ucap, cnt );
+                                               rc2, ucap, cnt, X, dj, B );
             }
             if ( lane < na )
             {
@@ -1002,6 +1150,21 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
                           int waves_cap, size_t lds )
 {
     const double rc2 = N->neigh_cut * N->neigh_cut;
+    // the full-list build tests in FP32 about each block's origin: every
+    // staged coordinate lies within amax of it (1.5 cells across, half the
+    // staged cells along the column), so the FP32 r^2 is within
+    // 128 amax^2 2^-24 of the FP64 one (roundings of the coordinates, their
+    // differences, the squares and the sums); candidates within that band of
+    // rc^2 are tested again in FP64
+    const CellGrid &g = N->tg.g;
+    const double amax = 1.001 * std::max( { 1.5 / g.rd[0], 1.5 / g.rd[1],
+                                            0.5 * ( N->tg.E + 2 ) / g.rd[2] } );
+    double band = 128.0 * amax * amax * std::ldexp( 1.0, -24 ) + 1e-6 * rc2;
+    // CMD_BUILD_BAND widens the band (never narrows it), so that tests can
+    // send most candidates through the FP64 re-test
+    if ( const char *e = std::getenv( "CMD_BUILD_BAND" ) )
+        band = std::max( band, std::atof( e ) );
+    const float lo = (float)( rc2 - band ), hi = (float)( rc2 + band );
     int *res = S->sc.ires.p;
     // one block per atom column of a tile, one wave per cell of the column
     const int threads = 64 * std::min( std::max( N->tg.E, 1 ), BTPB / 64 );
@@ -1011,13 +1174,13 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
-            res + TILE_RES_WAVES );
+            res + TILE_RES_WAVES, lo, hi );
     else
         k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
-            res + TILE_RES_WAVES );
+            res + TILE_RES_WAVES, lo, hi );
 }
 
 static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_cap,
@@ -1045,16 +1208,16 @@ static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int
 // LDS of the build block: coordinate planes of its 3 x 3 columns and the
 // sentinel slot, then the rows of its atom column (each with a spare slot
 // past its cap entries).
-static size_t build_lds_cap( int G, int cap, int slot_cap, int rows )
+static size_t build_lds_cap( int G, int cap, int slot_cap, int rows, bool half )
 {
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
-    return tile_planes_bytes( slot_cap ) +
+    return ( half ? tile_planes_bytes<double>( slot_cap ) : tile_planes_bytes<float>( slot_cap ) ) +
            ( (size_t)std::max( rows, 1 ) * tile_row_stride( R4, G ) + 8 ) * sizeof( unsigned short );
 }
 
 static size_t build_lds( const cmd_neigh *N, int slot_cap, int rows )
 {
-    return build_lds_cap( N->lanes, N->cap, slot_cap, rows );
+    return build_lds_cap( N->lanes, N->cap, slot_cap, rows, N->half != 0 );
 }
 
 // Tile height asked for: the set one, else 4 cells for single columns and 3
@@ -1100,7 +1263,7 @@ static void record_spec( cmd_neigh *N, int W, int E, const int h[6], int mr, siz
     P.slot_cap = std::min( h[3] + 1 + std::max( 16, h[3] / 32 ), MAX_SLOTS + 1 );
     P.rows = h[4] + std::max( 2, h[4] / 20 );
     P.slab_waves = h[5] + std::max( 64, h[5] / 50 );
-    P.lds = build_lds_cap( N->lanes, P.cap, P.slot_cap, P.rows );
+    P.lds = build_lds_cap( N->lanes, P.cap, P.slot_cap, P.rows, N->half != 0 );
     P.valid = P.lds <= hard;
 }
 
