@@ -739,7 +739,10 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
         // the cell's atoms, up to 64 at a time (lane l holds atom a0 + l),
         // from the centre column's planes: build slot = atom index - dl4
         const int dl4 = H.info[TI_LB + ( ca + 1 ) * S + cb + 1] - s_bseg[8];
-        const int c0 = s_lcs[4][ce] + dl4, c1 = s_lcs[4][ce + 1] + dl4;
+        // wave-uniform (LDS reads land in vector registers): kept scalar so
+        // that the per-chunk self masks and row offsets stay scalar work
+        const int c0 = __builtin_amdgcn_readfirstlane( s_lcs[4][ce] + dl4 );
+        const int c1 = __builtin_amdgcn_readfirstlane( s_lcs[4][ce + 1] + dl4 );
         for ( int i0 = c0; i0 < c1; i0 += 64 )
         {
             const int na = min( 64, c1 - i0 );
@@ -801,7 +804,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                     pz[ch] = s_z[sj];
                 }
                 const int nch = min( BCH, ( total - v0 + 63 ) >> 6 );
-                const int vs0 = vbase + ( i0 - c0 ) - v0;
+                const int vs0 = __builtin_amdgcn_readfirstlane( vbase + ( i0 - c0 ) - v0 );
                 unsigned short *row0 = chunk + ( i0 - al ) * RS;
                 const unsigned ucap = (unsigned)cap;
                 const BandArgs B{ band_lo, band_hi, v0, i0 };
@@ -1269,14 +1272,17 @@ static void record_spec( cmd_neigh *N, int W, int E, const int h[6], int mr, siz
 
 // With CMD_VERBOSE set: the layout a tiled build chose and the sizes that
 // set the force kernel's LDS (largest staged neighbourhood) and blocks per CU.
-static void report_layout( const cmd_neigh *N, const char *how, int W, int E, const int h[6] )
+static void report_layout( const cmd_neigh *N, const char *how, int W, int E, const int h[6],
+                           size_t build_lds )
 {
     if ( !std::getenv( "CMD_VERBOSE" ) )
         return;
     std::fprintf( stderr,
                   "# %s tiled build %dx%dx%d: tile slots max %d, 3x3 column slots max %d, "
-                  "column atoms max %d, slab waves %d, force blocks/CU %d, waves/block %d\n",
-                  how, W, W, E, h[0], h[3], h[4], h[5], N->tile_blocks, N->tg.bw );
+                  "column atoms max %d, slab waves %d, force blocks/CU %d, waves/block %d, "
+                  "row capacity %d, build LDS %zu B\n",
+                  how, W, W, E, h[0], h[3], h[4], h[5], N->tile_blocks, N->tg.bw, N->cap,
+                  build_lds );
 }
 
 cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
@@ -1343,7 +1349,7 @@ cmd_status tiles_build( cmd_neigh *N, cmd_system *S, bool *built )
             N->built_rec = N->rec;
             N->gperm_view = S->gsort.perm.p;
             record_spec( N, W, E, h, mr, hard );
-            report_layout( N, "exact", W, E, h );
+            report_layout( N, "exact", W, E, h, lds );
             return CMD_OK;
         }
         // rows overflowed: larger rows, and the LDS fit rechecked
@@ -1404,7 +1410,7 @@ cmd_status tiles_build_spec( cmd_neigh *N, cmd_system *S, bool *built )
     N->gperm_view = S->gsort.perm.p;
     *built = true;
     record_spec( N, P.W, P.E, h, mr, hard );
-    report_layout( N, "speculative", P.W, P.E, h );
+    report_layout( N, "speculative", P.W, P.E, h, P.lds );
     return CMD_OK;
 }
 
