@@ -1058,7 +1058,9 @@ static cmd_status tile_info_launch( cmd_neigh *N, cmd_system *S, int W, int E )
     CMD_TRY( N->grid_sort.start.reserve( T.ntiles + 1 ) ); // scan of waves
     CMD_TRY( fill_i32( S->sc.ires.p + TILE_RES_INFO, 0, 5, S->stream ) );
     const int apw = 64 / N->lanes;
-    k_tile_info<<<( T.ntiles + TPB - 1 ) / TPB, TPB, 0, S->stream>>>(
+    // one wave per block: a thread does a whole tile (about a hundred
+    // dependent loads), so the tiles are spread over as many CUs as possible
+    k_tile_info<<<( T.ntiles + 63 ) / 64, 64, 0, S->stream>>>(
         T, S->bins.start.p, S->gsort.start.p, apw, N->tile_info.p, waves.p,
         S->sc.ires.p + TILE_RES_INFO );
     CMD_LAUNCH_CHECK();
