@@ -156,8 +156,13 @@ static cmd_status rebuild( cmd_md *md )
     CMD_TRY( halo_precount( S, md->neigh_cut ) );
     // the time loop computes the forces afresh after a rebuild before it
     // reads them: they are not carried through the sort
-    CMD_TRY( binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut, 1, 1, 0, 1,
-                             nullptr, false ) );
+    const cmd_status bs = binning_create( S, md->neigh_cut, md->neigh_cut, md->neigh_cut, 1,
+                                          1, 0, 1, nullptr, false );
+    if ( bs != CMD_OK )
+    {
+        S->halo_pre = false; // no halo pass consumes these totals
+        return bs;
+    }
     md->end( k );
     k = md->begin( PH_COMM );
     // ghost velocity and force rows come out zero (full-list kernels never
