@@ -647,3 +647,31 @@ def test_init_tuning_restores_initial_state(gpu):
     ta, tb = a.thermo(), b.thermo()
     assert np.array_equal(ta[:, 0], [0, 20, 40])
     assert np.allclose(ta[:, :4], tb[:, :4], rtol=1e-10, atol=1e-13)
+
+
+def test_halo_zeroes_ghost_velocity_and_force_rows(gpu):
+    """Comm::exchange_halo gathers positions and types into the ghosts
+    (comm_mpi_impl.h:346-347); the engine sets their velocity and force rows
+    to zero, also over force rows a half-list force wrote before."""
+    M = gpu
+    ncells = (6, 6, 6)
+    _, x, v, t, i = _liquid_state(ncells)
+    hi, gc = _domain_for(ncells)
+    s = M.System(1)
+    s.create_domain([0, 0, 0], hi, gc)
+    s.upload(x, v, None, t, i)
+    M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
+    comm = M.Comm(s, 2.8)
+    comm.exchange_halo()
+    nl = M.NeighborVerlet(2.8, True, 50, lanes=1)
+    nl.create(s)
+    F = M.ForceLJ(s, precision=M.PREC_FP64)
+    F.init_coeff([["pair_coeff", "1", "1", "1.0", "1.0", "2.5"]])
+    F.compute(s, nl)  # a half list on the global slab: reactions on ghost rows
+    n = len(x)
+    _, _, f1, _, _ = s.download()
+    assert len(f1) > n and np.abs(f1[n:]).max() > 0
+    comm.exchange_halo()
+    _, v2, f2, _, _ = s.download()
+    assert len(f2) > n
+    assert not np.any(v2[n:]) and not np.any(f2[n:])
