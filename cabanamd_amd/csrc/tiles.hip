@@ -32,7 +32,6 @@
 
 #include <algorithm>
 #include <atomic>
-#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -187,82 +186,15 @@ __host__ __device__ constexpr int tile_row_stride( int R4, int G )
     return R4 * 4 * G + 2;
 }
 
-// The test masks of one atom against NC register chunks of candidates:
-// r = (dx^2 + dy^2) + dz^2 <= rc^2 in FP64, as the reference evaluates it.
-template <int NC, bool HALF>
-__device__ __forceinline__ void test_chunks( const double ( &px )[BCH], const double ( &py )[BCH],
-                                             const double ( &pz )[BCH], const double4 pa, int vs,
-                                             double rc2, unsigned long long ( &m )[NC] )
-{
-    // the atom itself: candidate vs of this pass (chunk vs >> 6, lane
-    // vs & 63; outside the pass when the chunk is not in 0 .. NC - 1)
-    const int cself = vs >> 6;
-    const unsigned long long notself = ~( 1ull << ( vs & 63 ) );
-#pragma unroll
-    for ( int c = 0; c < NC; ++c )
-    {
-        const double dx = pa.x - px[c];
-        const double dy = pa.y - py[c];
-        const double dz = pa.z - pz[c];
-        const double r = dx * dx + dy * dy + dz * dz;
-        // ballots of the plain compares are their lane masks as they stand
-        // (a ballot of a combined predicate is rematerialised first)
-        m[c] = __builtin_amdgcn_ballot_w64( r <= rc2 ) & ( c == cself ? notself : ~0ull );
-        if ( HALF )
-        {
-            const bool up = ( px[c] > pa.x ) |
-                            ( ( px[c] == pa.x ) &
-                              ( ( py[c] > pa.y ) | ( ( py[c] == pa.y ) & ( pz[c] > pa.z ) ) ) );
-            m[c] &= __builtin_amdgcn_ballot_w64( up );
-        }
-    }
-}
-
-// The same for two atoms (candidates vs and vs + 1), their operations
-// interleaved so that each dependent FP64 operation has the other atom's
-// independent one between it and its operand's producer.
-template <int NC, bool HALF>
-__device__ __forceinline__ void test_pair( const double ( &px )[BCH], const double ( &py )[BCH],
-                                           const double ( &pz )[BCH], const double4 pa,
-                                           const double4 pb, int vs, double rc2,
-                                           unsigned long long ( &ma )[NC],
-                                           unsigned long long ( &mb )[NC] )
-{
-    const int csa = vs >> 6, csb = ( vs + 1 ) >> 6;
-    const unsigned long long nsa = ~( 1ull << ( vs & 63 ) );
-    const unsigned long long nsb = ~( 1ull << ( ( vs + 1 ) & 63 ) );
-#pragma unroll
-    for ( int c = 0; c < NC; ++c )
-    {
-        const double dxa = pa.x - px[c], dxb = pb.x - px[c];
-        const double dya = pa.y - py[c], dyb = pb.y - py[c];
-        const double dza = pa.z - pz[c], dzb = pb.z - pz[c];
-        const double xa = dxa * dxa, xb = dxb * dxb;
-        const double ya = dya * dya, yb = dyb * dyb;
-        const double za = dza * dza, zb = dzb * dzb;
-        const double sa = xa + ya, sb = xb + yb;
-        const double ra = sa + za, rb = sb + zb;
-        ma[c] = __builtin_amdgcn_ballot_w64( ra <= rc2 ) & ( c == csa ? nsa : ~0ull );
-        mb[c] = __builtin_amdgcn_ballot_w64( rb <= rc2 ) & ( c == csb ? nsb : ~0ull );
-        if ( HALF )
-        {
-            const bool upa = ( px[c] > pa.x ) |
-                             ( ( px[c] == pa.x ) &
-                               ( ( py[c] > pa.y ) | ( ( py[c] == pa.y ) & ( pz[c] > pa.z ) ) ) );
-            const bool upb = ( px[c] > pb.x ) |
-                             ( ( px[c] == pb.x ) &
-                               ( ( py[c] > pb.y ) | ( ( py[c] == pb.y ) & ( pz[c] > pb.z ) ) ) );
-            ma[c] &= __builtin_amdgcn_ballot_w64( upa );
-            mb[c] &= __builtin_amdgcn_ballot_w64( upb );
-        }
-    }
-}
-
-// The full-list build holds coordinates in FP32 about the block's origin
-// (half the LDS and registers of FP64).  A candidate whose FP32 r^2 lies
-// at or below lo (rc^2 less the FP32 error bound) is within rc in FP64
-// too; one above hi (rc^2 plus the bound) is not; one in between is tested
-// again in FP64 on the positions in memory, as the reference evaluates it.
+// The list build holds coordinates in FP32 about the block's origin (half
+// the LDS and registers of FP64).  A candidate whose FP32 r^2 lies at or
+// below lo (rc^2 less the FP32 error bound) is within rc in FP64 too; one
+// above hi (rc^2 plus the bound) is not; one in between is tested again in
+// FP64 on the positions in memory, as the reference evaluates it.  The
+// half list's direction test (the candidate above the atom in x, then y,
+// then z, comparing FP64 coordinates) follows the sign of the FP32 x
+// difference where it exceeds eps (its error bound), and is settled in
+// FP64 with the distance otherwise.
 // What the re-test needs: the positions, the block's 9-column segment
 // offsets (LDS) and the tile head's words, to map a build slot to its atom.
 struct ExactCtx
@@ -275,7 +207,8 @@ struct ExactCtx
 };
 
 // Of the lanes in q (candidates at build slot sj, the atom at local index
-// ia), those within rc in FP64.
+// ia), those within rc in FP64 (and, HALF, above the atom).
+template <bool HALF>
 __device__ __forceinline__ unsigned long long exact_lanes( const ExactCtx &X,
                                                            unsigned long long q, int sj, int ia )
 {
@@ -292,6 +225,9 @@ __device__ __forceinline__ unsigned long long exact_lanes( const ExactCtx &X,
         const double dy = a.y - c.y;
         const double dz = a.z - c.z;
         ok = dx * dx + dy * dy + dz * dz <= X.rc2;
+        if ( HALF )
+            ok = ok && ( ( c.x > a.x ) |
+                         ( ( c.x == a.x ) & ( ( c.y > a.y ) | ( ( c.y == a.y ) & ( c.z > a.z ) ) ) ) );
     }
     return __builtin_amdgcn_ballot_w64( ok ) & q;
 }
@@ -299,12 +235,13 @@ __device__ __forceinline__ unsigned long long exact_lanes( const ExactCtx &X,
 // The FP32 tests of two atoms (candidates vs and vs + 1 of the pass, local
 // indices ia and ia + 1) against NC register chunks, interleaved as in the
 // FP64 form; lanes in the band are settled by exact_lanes.
-template <int NC>
+template <int NC, bool HALF>
 __device__ __forceinline__ void test_pair32( const float ( &px )[BCH], const float ( &py )[BCH],
                                              const float ( &pz )[BCH], const float4 pa,
                                              const float4 pb, int vs, float lo, float hi,
-                                             const ExactCtx &X, const int ( &dj )[BCH], int v0,
-                                             int ia, unsigned long long ( &ma )[NC],
+                                             float eps, const ExactCtx &X,
+                                             const int ( &dj )[BCH], int v0, int ia,
+                                             unsigned long long ( &ma )[NC],
                                              unsigned long long ( &mb )[NC] )
 {
     const int csa = vs >> 6, csb = ( vs + 1 ) >> 6;
@@ -319,27 +256,41 @@ __device__ __forceinline__ void test_pair32( const float ( &px )[BCH], const flo
         const float dza = pa.z - pz[c], dzb = pb.z - pz[c];
         const float ra = fmaf( dza, dza, fmaf( dya, dya, dxa * dxa ) );
         const float rb = fmaf( dzb, dzb, fmaf( dyb, dyb, dxb * dxb ) );
-        unsigned long long sa = __builtin_amdgcn_ballot_w64( ra <= lo );
-        unsigned long long sb = __builtin_amdgcn_ballot_w64( rb <= lo );
-        const unsigned long long qa = __builtin_amdgcn_ballot_w64( ra <= hi ) & ~sa;
-        const unsigned long long qb = __builtin_amdgcn_ballot_w64( rb <= hi ) & ~sb;
+        unsigned long long sa, sb, qa, qb;
+        if ( HALF )
+        {
+            // settled: inside lo with a certain x direction
+            const bool ka = ( ra <= lo ) & ( fabsf( dxa ) > eps );
+            const bool kb = ( rb <= lo ) & ( fabsf( dxb ) > eps );
+            sa = __builtin_amdgcn_ballot_w64( ka & ( dxa < 0.f ) );
+            sb = __builtin_amdgcn_ballot_w64( kb & ( dxb < 0.f ) );
+            qa = __builtin_amdgcn_ballot_w64( ra <= hi ) & ~__builtin_amdgcn_ballot_w64( ka );
+            qb = __builtin_amdgcn_ballot_w64( rb <= hi ) & ~__builtin_amdgcn_ballot_w64( kb );
+        }
+        else
+        {
+            sa = __builtin_amdgcn_ballot_w64( ra <= lo );
+            sb = __builtin_amdgcn_ballot_w64( rb <= lo );
+            qa = __builtin_amdgcn_ballot_w64( ra <= hi ) & ~sa;
+            qb = __builtin_amdgcn_ballot_w64( rb <= hi ) & ~sb;
+        }
         if ( qa | qb )
         {
             const int sj = v0 + c * 64 + lane + dj[c];
             if ( qa )
-                sa |= exact_lanes( X, qa, sj, ia );
+                sa |= exact_lanes<HALF>( X, qa, sj, ia );
             if ( qb )
-                sb |= exact_lanes( X, qb, sj, ia + 1 );
+                sb |= exact_lanes<HALF>( X, qb, sj, ia + 1 );
         }
         ma[c] = sa & ( c == csa ? nsa : ~0ull );
         mb[c] = sb & ( c == csb ? nsb : ~0ull );
     }
 }
 
-template <int NC>
+template <int NC, bool HALF>
 __device__ __forceinline__ void test_chunks32( const float ( &px )[BCH], const float ( &py )[BCH],
                                                const float ( &pz )[BCH], const float4 pa, int vs,
-                                               float lo, float hi, const ExactCtx &X,
+                                               float lo, float hi, float eps, const ExactCtx &X,
                                                const int ( &dj )[BCH], int v0, int ia,
                                                unsigned long long ( &m )[NC] )
 {
@@ -353,10 +304,20 @@ __device__ __forceinline__ void test_chunks32( const float ( &px )[BCH], const f
         const float dy = pa.y - py[c];
         const float dz = pa.z - pz[c];
         const float r = fmaf( dz, dz, fmaf( dy, dy, dx * dx ) );
-        unsigned long long sa = __builtin_amdgcn_ballot_w64( r <= lo );
-        const unsigned long long q = __builtin_amdgcn_ballot_w64( r <= hi ) & ~sa;
+        unsigned long long sa, q;
+        if ( HALF )
+        {
+            const bool k = ( r <= lo ) & ( fabsf( dx ) > eps );
+            sa = __builtin_amdgcn_ballot_w64( k & ( dx < 0.f ) );
+            q = __builtin_amdgcn_ballot_w64( r <= hi ) & ~__builtin_amdgcn_ballot_w64( k );
+        }
+        else
+        {
+            sa = __builtin_amdgcn_ballot_w64( r <= lo );
+            q = __builtin_amdgcn_ballot_w64( r <= hi ) & ~sa;
+        }
         if ( q )
-            sa |= exact_lanes( X, q, v0 + c * 64 + lane + dj[c], ia );
+            sa |= exact_lanes<HALF>( X, q, v0 + c * 64 + lane + dj[c], ia );
         m[c] = sa & ( c == cself ? notself : ~0ull );
     }
 }
@@ -423,7 +384,7 @@ __device__ __forceinline__ float4 lane_pos( const float4 &a, int l )
 // candidate-to-slot offsets).
 struct BandArgs
 {
-    float lo, hi;
+    float lo, hi, eps;
     int v0, i0;
 };
 
@@ -442,11 +403,8 @@ __device__ __forceinline__ void build_atoms( const PT ( &px )[BCH], const PT ( &
         const unsigned ka0 = (unsigned)__builtin_amdgcn_readlane( cnt, l );
         const unsigned kb0 = (unsigned)__builtin_amdgcn_readlane( cnt, l + 1 );
         unsigned long long ma[NC], mb[NC];
-        if constexpr ( HALF )
-            test_pair<NC, HALF>( px, py, pz, pa, pb, vs0 + l, rc2, ma, mb );
-        else
-            test_pair32<NC>( px, py, pz, pa, pb, vs0 + l, B.lo, B.hi, X, dj, B.v0, B.i0 + l,
-                             ma, mb );
+        test_pair32<NC, HALF>( px, py, pz, pa, pb, vs0 + l, B.lo, B.hi, B.eps, X, dj, B.v0,
+                               B.i0 + l, ma, mb );
         const unsigned ka = place_row<NC>( po, ma, row0 + l * RS, ka0, cap );
         const unsigned kb = place_row<NC>( po, mb, row0 + ( l + 1 ) * RS, kb0, cap );
         const int lane = threadIdx.x & 63;
@@ -456,10 +414,8 @@ __device__ __forceinline__ void build_atoms( const PT ( &px )[BCH], const PT ( &
     {
         const P4 pa = lane_pos( at, l );
         unsigned long long ma[NC];
-        if constexpr ( HALF )
-            test_chunks<NC, HALF>( px, py, pz, pa, vs0 + l, rc2, ma );
-        else
-            test_chunks32<NC>( px, py, pz, pa, vs0 + l, B.lo, B.hi, X, dj, B.v0, B.i0 + l, ma );
+        test_chunks32<NC, HALF>( px, py, pz, pa, vs0 + l, B.lo, B.hi, B.eps, X, dj, B.v0,
+                                 B.i0 + l, ma );
         const unsigned ka = place_row<NC>( po, ma, row0 + l * RS,
                                            (unsigned)__builtin_amdgcn_readlane( cnt, l ), cap );
         if ( ( threadIdx.x & 63 ) == l )
@@ -477,12 +433,11 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const double4 *__restrict__ x, const double4 *__restrict__ xg, double rc2, int rec,
     int cap, int slot_cap, int rows_cap, int waves_cap, unsigned short *__restrict__ slab,
     int *__restrict__ counts, int *__restrict__ max_row, int *__restrict__ over,
-    int *__restrict__ waves_total, float band_lo, float band_hi )
+    int *__restrict__ waves_total, float band_lo, float band_hi, float band_eps )
 {
-    // coordinate planes: FP64 for the half list (its direction test compares
-    // coordinates exactly), FP32 about the block's origin otherwise
-    using PT = std::conditional_t<HALF, double, float>;
-    using P4 = std::conditional_t<HALF, double4, float4>;
+    // coordinate planes in FP32 about the block's origin (tests above)
+    using PT = float;
+    using P4 = float4;
     extern __shared__ double s_dyn[];
     PT *s_x = reinterpret_cast<PT *>( s_dyn );
     __shared__ TileHead H;
@@ -507,9 +462,9 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
     const int a = a0 + ca, b = b0 + cb;
     const int ea = H.info[TI_EA], nst = H.info[TI_NST];
     // the FP32 planes' origin: the centre of the atom column's staged cells
-    const double ox = HALF ? 0.0 : T.g.min[0] + ( a + 0.5 ) / T.g.rd[0];
-    const double oy = HALF ? 0.0 : T.g.min[1] + ( b + 0.5 ) / T.g.rd[1];
-    const double oz = HALF ? 0.0 : T.g.min[2] + ( ea + 0.5 * nst ) / T.g.rd[2];
+    const double ox = T.g.min[0] + ( a + 0.5 ) / T.g.rd[0];
+    const double oy = T.g.min[1] + ( b + 0.5 ) / T.g.rd[1];
+    const double oz = T.g.min[2] + ( ea + 0.5 * nst ) / T.g.rd[2];
     // the block's 3 x 3 columns: build column qb = da * 3 + db is staged
     // column (ca + da) * S + cb + db of the tile
     if ( threadIdx.x == 0 )
@@ -807,7 +762,7 @@ __global__ __launch_bounds__( BTPB ) __attribute__( ( amdgpu_waves_per_eu( 2, 4 
                 const int vs0 = __builtin_amdgcn_readfirstlane( vbase + ( i0 - c0 ) - v0 );
                 unsigned short *row0 = chunk + ( i0 - al ) * RS;
                 const unsigned ucap = (unsigned)cap;
-                const BandArgs B{ band_lo, band_hi, v0, i0 };
+                const BandArgs B{ band_lo, band_hi, band_eps, v0, i0 };
                 if ( nch <= 4 )
                     build_atoms<4, G, HALF>( px, py, pz, po, at, na, vs0, row0, RS, rc2,
                                              ucap, cnt, X, dj, B );
@@ -1155,7 +1110,7 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
                           int waves_cap, size_t lds )
 {
     const double rc2 = N->neigh_cut * N->neigh_cut;
-    // the full-list build tests in FP32 about each block's origin: every
+    // the list build tests in FP32 about each block's origin: every
     // staged coordinate lies within amax of it (1.5 cells across, half the
     // staged cells along the column), so the FP32 r^2 is within
     // 128 amax^2 2^-24 of the FP64 one (roundings of the coordinates, their
@@ -1170,6 +1125,9 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
     if ( const char *e = std::getenv( "CMD_BUILD_BAND" ) )
         band = std::max( band, std::atof( e ) );
     const float lo = (float)( rc2 - band ), hi = (float)( rc2 + band );
+    // the half list's x direction: an FP32 difference of two coordinates
+    // within amax of the origin is within 4 amax 2^-24 of the FP64 one
+    const float eps = (float)( 8.0 * amax * std::ldexp( 1.0, -24 ) );
     int *res = S->sc.ires.p;
     // one block per atom column of a tile, one wave per cell of the column
     const int threads = 64 * std::min( std::max( N->tg.E, 1 ), BTPB / 64 );
@@ -1179,13 +1137,13 @@ static void launch_build( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_ca
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
-            res + TILE_RES_WAVES, lo, hi );
+            res + TILE_RES_WAVES, lo, hi, eps );
     else
         k_tile_build<G, false><<<nb, threads, lds, S->stream>>>(
             N->tg, S->bins.start.p, S->gsort.start.p, N->tile_info.p,
             N->grid_sort.start.p, S->x.p, S->xg.p, rc2, N->rec, N->cap, slot_cap, rows_cap,
             waves_cap, N->tslab.p, N->counts.p, res + TILE_RES_MAXROW, res + TILE_RES_OVER,
-            res + TILE_RES_WAVES, lo, hi );
+            res + TILE_RES_WAVES, lo, hi, eps );
 }
 
 static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int rows_cap,
@@ -1213,16 +1171,16 @@ static cmd_status launch_build_g( cmd_neigh *N, cmd_system *S, int slot_cap, int
 // LDS of the build block: coordinate planes of its 3 x 3 columns and the
 // sentinel slot, then the rows of its atom column (each with a spare slot
 // past its cap entries).
-static size_t build_lds_cap( int G, int cap, int slot_cap, int rows, bool half )
+static size_t build_lds_cap( int G, int cap, int slot_cap, int rows )
 {
     const int R4 = ( ( cap + G - 1 ) / G + 3 ) / 4;
-    return ( half ? tile_planes_bytes<double>( slot_cap ) : tile_planes_bytes<float>( slot_cap ) ) +
+    return tile_planes_bytes<float>( slot_cap ) +
            ( (size_t)std::max( rows, 1 ) * tile_row_stride( R4, G ) + 8 ) * sizeof( unsigned short );
 }
 
 static size_t build_lds( const cmd_neigh *N, int slot_cap, int rows )
 {
-    return build_lds_cap( N->lanes, N->cap, slot_cap, rows, N->half != 0 );
+    return build_lds_cap( N->lanes, N->cap, slot_cap, rows );
 }
 
 // Tile height asked for: the set one, else 4 cells for single columns and 3
@@ -1268,7 +1226,7 @@ static void record_spec( cmd_neigh *N, int W, int E, const int h[6], int mr, siz
     P.slot_cap = std::min( h[3] + 1 + std::max( 16, h[3] / 32 ), MAX_SLOTS + 1 );
     P.rows = h[4] + std::max( 2, h[4] / 20 );
     P.slab_waves = h[5] + std::max( 64, h[5] / 50 );
-    P.lds = build_lds_cap( N->lanes, P.cap, P.slot_cap, P.rows, N->half != 0 );
+    P.lds = build_lds_cap( N->lanes, P.cap, P.slot_cap, P.rows );
     P.valid = P.lds <= hard;
 }
 

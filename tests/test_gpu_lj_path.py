@@ -238,12 +238,14 @@ def test_neighbor_on_binning_grid_exact(gpu, half, lanes, width):
     assert np.array_equal(idx, idx_o)
 
 
-@pytest.mark.parametrize("width", [1, 2, 3])
-def test_full_list_build_band_retest_exact(gpu, monkeypatch, width):
-    """The full-list build tests candidates in FP32 and sends those whose
-    FP32 r^2 lies within a band of rc^2 to an FP64 re-test on the positions
-    in memory.  With the band widened to 2 (most candidates between
-    r = 2.43 and 3.13 re-tested) the rows still equal the oracle's."""
+@pytest.mark.parametrize("half,width", [(False, 1), (False, 2), (False, 3), (True, 1),
+                                        (True, 2)])
+def test_list_build_band_retest_exact(gpu, monkeypatch, half, width):
+    """The tiled build tests candidates in FP32 and sends those whose FP32
+    r^2 lies within a band of rc^2 (and, for half lists, those whose x
+    direction the FP32 difference cannot settle) to an FP64 re-test on the
+    positions in memory.  With the band widened to 2 (most candidates
+    between r = 2.43 and 3.13 re-tested) the rows still equal the oracle's."""
     monkeypatch.setenv("CMD_BUILD_BAND", "2.0")
     M = gpu
     ncells = (9, 8, 7)
@@ -255,13 +257,13 @@ def test_full_list_build_band_retest_exact(gpu, monkeypatch, width):
     xo = x[perm]
     xg, _, _, _, _ = O.ghost_build(xo, t[perm], i[perm], d, 2.8)
     gmin, rd, ng = O.bin_grid(d, [2.8] * 3, 1)
-    off_o, idx_o = O.neigh_build_grid(xg, len(x), 2.8, False, gmin, rd, ng, 1)
+    off_o, idx_o = O.neigh_build_grid(xg, len(x), 2.8, half, gmin, rd, ng, 1)
     s = M.System(1)
     s.create_domain([0, 0, 0], hi, gc)
     s.upload(x, v, None, t, i)
     M.Binning(s).create_binning(2.8, 2.8, 2.8, 1, True, False, True)
     M.Comm(s, 2.8).exchange_halo()
-    nl = M.NeighborVerlet(2.8, False, 50, lanes=2, tile_width=width)
+    nl = M.NeighborVerlet(2.8, half, 50, lanes=2, tile_width=width)
     nl.create(s)
     off, idx = nl.get()
     assert nl.layout()["tiled"]
