@@ -233,8 +233,9 @@ __device__ __forceinline__ unsigned long long exact_lanes( const ExactCtx &X,
 }
 
 // The FP32 tests of two atoms (candidates vs and vs + 1 of the pass, local
-// indices ia and ia + 1) against NC register chunks, interleaved as in the
-// FP64 form; lanes in the band are settled by exact_lanes.
+// indices ia and ia + 1) against NC register chunks, their operations
+// interleaved so that each dependent operation has the other atom's
+// independent one beside it; lanes in the band are settled by exact_lanes.
 template <int NC, bool HALF>
 __device__ __forceinline__ void test_pair32( const float ( &px )[BCH], const float ( &py )[BCH],
                                              const float ( &pz )[BCH], const float4 pa,
@@ -378,8 +379,8 @@ __device__ __forceinline__ float4 lane_pos( const float4 &a, int l )
 // against NC register chunks of candidates, two atoms at a time.  An atom's
 // position and row length are read out of their lanes as wave-uniform
 // values, so the loop waits on no memory.  Lane l of cnt holds atom l's row
-// length.  HALF: FP64 coordinates and tests; otherwise FP32 coordinates about
-// the block's origin with the band of B (pass start v0, local index i0 of
+// length.  Coordinates are FP32 about the block's origin, with the band of
+// B (and, HALF, the x-direction bound eps; pass start v0, local index i0 of
 // the group's first atom) settled in FP64 (ExactCtx, dj: the chunks'
 // candidate-to-slot offsets).
 struct BandArgs
